@@ -1,0 +1,90 @@
+// kernels.h -- device work descriptors and launch entry points (internal).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace sipgpu {
+
+// ---------------------------------------------------------------- permute --
+constexpr int kPermMaxDims = 8;
+
+struct PermArgs {
+    const double *in;
+    double *out;
+    int nU, nV;                   // tile-row dims (leading input dims), other dims
+    int extU[kPermMaxDims];       // extents of U dims (input order)
+    int soutU[kPermMaxDims];      // their output strides
+    int extV[kPermMaxDims];       // extents of V dims (output order)
+    int sinV[kPermMaxDims];       // their input strides
+    int soutV[kPermMaxDims];      // their output strides
+    int sizeU, sizeV;
+    int TU, TV, pitch;            // tile extents and LDS row pitch
+    int tilesU;                   // tiles along U
+    int row_mode;                 // U is contiguous in the output as well
+};
+
+struct PermutePlan;
+hipError_t launch_permute(const PermutePlan &plan, const double *in, double *out,
+                          hipStream_t stream);
+
+// ------------------------------------------------------------- contraction --
+// One operand pair of a task.  Table offsets index the plan's int32 pool.
+struct GemmPair {
+    const double *L;
+    const double *R;
+    int K;
+    int kL, mL, kR, nR;           // offsets of koffL, moffL, koffR, noffR
+    int flags;                    // bit0: L k-fast, bit1: R k-fast
+};
+
+// One destination block: D(m, n) at D[moffD[m] + noffD[n]].
+struct GemmTask {
+    double *D;
+    int M, N;
+    int mD, nD;                   // offsets of moffD, noffD
+    int first, npairs;            // pair range
+    int beta;                     // 0: assign, 1: accumulate
+    int pad;
+};
+
+struct GemmTile {
+    int task, mt, nt, pad;
+};
+
+// When `tiles` is null the launch is a single contraction: task0 with its
+// one pair0, tile t at (t % tilesM, t / tilesM).  This lets a lone
+// contraction run without uploading a work list.
+struct GemmArgs {
+    const GemmTile *tiles;
+    const GemmTask *tasks;
+    const GemmPair *pairs;
+    const int32_t *tab;
+    int ntiles;
+    int tilesM;
+    GemmTask task0;
+    GemmPair pair0;
+};
+
+// Tile configurations of the grouped FP64-MFMA kernel.
+enum GemmConfig { kGemm64x64 = 0, kGemm128x128 = 1, kGemm64x128 = 2, kGemmConfigs = 3 };
+int gemm_tile_m(int cfg);
+int gemm_tile_n(int cfg);
+hipError_t launch_gemm(int cfg, const GemmArgs &args, int grid, hipStream_t stream);
+
+// Small kernels for the non-GEMM contraction classes of the reference
+// driver (tensor_dil_omp.F90:766-777).
+// D[moffD[m]] = 0 + X[moffX[m]] * s', s' = 1 when |s-1| <= 1e-13.
+hipError_t launch_scale_scatter(double *D, const int32_t *moffD, const double *X,
+                                const int32_t *moffX, const double *s, int M,
+                                hipStream_t stream);
+// D[0] = 0 + sum_k L[koffL[k]] * R[koffR[k]]; partial sums in `scratch`
+// (at least kDotBlocks doubles); fixed summation order.
+constexpr int kDotBlocks = 64;
+hipError_t launch_dot(double *D, const double *L, const int32_t *koffL, const double *R,
+                      const int32_t *koffR, int K, double *scratch, hipStream_t stream);
+// D[0] = L[0] * R[0]
+hipError_t launch_scalar_mul(double *D, const double *L, const double *R, hipStream_t stream);
+
+} // namespace sipgpu

@@ -1,0 +1,793 @@
+// runtime.cpp -- libsipgpu's host runtime and C ABI.
+//
+// Drop-in layer: the reference's tensor-kernel symbols (host pointers,
+// synchronous, assign semantics, the reference's ierr codes), staged
+// through a grow-only device arena so no call allocates once warm.
+// Native layer: device blocks, streams, cached contraction plans and
+// planned batches (one grouped FP64-MFMA launch per batch).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <array>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/sipgpu.h"
+#include "kernels.h"
+#include "plan.h"
+
+using namespace sipgpu;
+
+struct sipgpu_stream_s {
+    hipStream_t s = nullptr;
+    double *scratch = nullptr; // kDotBlocks partial sums of contract-to-scalar
+};
+struct sipgpu_event_s {
+    hipEvent_t e = nullptr;
+};
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(int code, const std::string &msg) {
+    g_last_error = msg;
+    return code;
+}
+
+int hip_fail(hipError_t e, const char *what) {
+    return fail(SIPGPU_EDEVICE, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+#define HIP_TRY(expr)                                                                     \
+    do {                                                                                  \
+        hipError_t e_ = (expr);                                                           \
+        if (e_ != hipSuccess) return hip_fail(e_, #expr);                                 \
+    } while (0)
+
+// Grow-only device buffer.  Freed only at process exit by the driver.
+struct DevBuf {
+    void *p = nullptr;
+    size_t cap = 0;
+    int reserve(size_t bytes) {
+        if (bytes <= cap) return 0;
+        if (p) {
+            hipError_t e = hipFree(p);
+            if (e != hipSuccess) return hip_fail(e, "hipFree");
+            p = nullptr;
+            cap = 0;
+        }
+        size_t want = std::max<size_t>(bytes, 1 << 20);
+        hipError_t e = hipMalloc(&p, want);
+        if (e != hipSuccess) return fail(SIPGPU_ENOMEM, "device allocation failed");
+        cap = want;
+        return 0;
+    }
+};
+
+// Device copy of one contraction shape's offset tables.
+struct DeviceShape {
+    ContractionShape shape;
+    int32_t *tab = nullptr; // koffL | moffL | koffR | noffR | moffD | noffD
+    int kL = 0, mL = 0, kR = 0, nR = 0, mD = 0, nD = 0;
+};
+
+struct Context {
+    std::mutex mu;
+    bool ready = false;
+    int device = 0;
+    sipgpu_stream_s stream;
+    DevBuf stage_l, stage_r, stage_d;
+    std::unordered_map<std::string, std::unique_ptr<DeviceShape>> shapes;
+};
+
+Context &ctx() {
+    static Context *c = new Context(); // never destroyed: HIP may be gone at exit
+    return *c;
+}
+
+int ensure_stream_scratch(sipgpu_stream_s &s) {
+    if (s.scratch) return 0;
+    HIP_TRY(hipMalloc(&s.scratch, kDotBlocks * sizeof(double)));
+    return 0;
+}
+
+int init_locked(Context &c, int device) {
+    if (c.ready) return 0;
+    int n = 0;
+    HIP_TRY(hipGetDeviceCount(&n));
+    if (device < 0 || device >= n) return fail(SIPGPU_EARG, "no such device");
+    HIP_TRY(hipSetDevice(device));
+    HIP_TRY(hipStreamCreateWithFlags(&c.stream.s, hipStreamNonBlocking));
+    int rc = ensure_stream_scratch(c.stream);
+    if (rc) return rc;
+    c.device = device;
+    c.ready = true;
+    return 0;
+}
+
+int ensure_init(Context &c) { return init_locked(c, c.device); }
+
+sipgpu_stream_s &resolve(Context &c, sipgpu_stream_t s) { return s ? *s : c.stream; }
+
+std::string shape_key(const int *ptrn, int lr, const int *ld, int rr, const int *rd, int dr,
+                      const int *dd) {
+    std::string k;
+    auto put = [&](int v) { k.append(reinterpret_cast<const char *>(&v), sizeof(v)); };
+    put(lr); put(rr); put(dr);
+    for (int i = 0; i < lr + rr; ++i) put(ptrn[i]);
+    for (int i = 0; i < lr; ++i) put(ld[i]);
+    for (int i = 0; i < rr; ++i) put(rd[i]);
+    for (int i = 0; i < dr; ++i) put(dd[i]);
+    return k;
+}
+
+// Looks up or builds the device tables of a contraction shape.  Returns the
+// reference's ierr (1, -1) or a SIPGPU_E* code.
+int get_shape(Context &c, const int *ptrn, int lr, const int *ld, int rr, const int *rd,
+              int dr, const int *dd, DeviceShape **out) {
+    const std::string key = shape_key(ptrn, lr, ld, rr, rd, dr, dd);
+    auto it = c.shapes.find(key);
+    if (it != c.shapes.end()) { *out = it->second.get(); return 0; }
+    auto ds = std::make_unique<DeviceShape>();
+    int rc = plan_contraction(ptrn, lr, ld, rr, rd, dr, dd, ds->shape, true);
+    if (rc) return rc;
+    const ContractionShape &s = ds->shape;
+    std::vector<int32_t> pool;
+    auto add = [&](const std::vector<int32_t> &t) {
+        const int at = (int)pool.size();
+        pool.insert(pool.end(), t.begin(), t.end());
+        return at;
+    };
+    ds->kL = add(s.koffL); ds->mL = add(s.moffL);
+    ds->kR = add(s.koffR); ds->nR = add(s.noffR);
+    ds->mD = add(s.moffD); ds->nD = add(s.noffD);
+    HIP_TRY(hipMalloc(&ds->tab, pool.size() * sizeof(int32_t)));
+    HIP_TRY(hipMemcpy(ds->tab, pool.data(), pool.size() * sizeof(int32_t),
+                      hipMemcpyHostToDevice));
+    *out = ds.get();
+    c.shapes.emplace(key, std::move(ds));
+    return 0;
+}
+
+// Kernel-side roles of one block pair: D(m, n) = sum_k A(k, m) B(k, n) with
+// (A, B) = (L, R), or (R, L) on the transposed problem when the destination's
+// fastest dim is on the left side.
+void make_task_pair(const ContractionShape &s, int kL, int mL, int kR, int nR, int mD, int nD,
+                    const double *l, const double *r, double *d, int beta, GemmTask &task,
+                    GemmPair &pair) {
+    const int fl = (s.l_kfast ? 1 : 0) | (s.r_kfast ? 2 : 0);
+    if (!s.swap) {
+        task = GemmTask{d, (int)s.M, (int)s.N, mD, nD, 0, 0, beta, 0};
+        pair = GemmPair{l, r, (int)s.K, kL, mL, kR, nR, fl};
+    } else {
+        task = GemmTask{d, (int)s.N, (int)s.M, nD, mD, 0, 0, beta, 0};
+        pair = GemmPair{r, l, (int)s.K, kR, nR, kL, mL, ((fl & 1) << 1) | ((fl & 2) >> 1)};
+    }
+}
+
+int pick_config(int64_t M, int64_t N) {
+    if (M >= 128 && N >= 128) return kGemm128x128;
+    if (N >= 128) return kGemm64x128;
+    return kGemm64x64;
+}
+
+// One contraction on device pointers (no locking; caller holds the lock).
+int contract_device(Context &c, sipgpu_stream_s &st, const int *ptrn, const double *l, int lr,
+                    const int *ld, const double *r, int rr, const int *rd, double *d, int dr,
+                    const int *dd, int accumulate) {
+    DeviceShape *ds = nullptr;
+    int rc = get_shape(c, ptrn, lr, ld, rr, rd, dr, dd, &ds);
+    if (rc) return rc;
+    const ContractionShape &s = ds->shape;
+    if (s.dsize == 0) return 0;
+    const int32_t *tab = ds->tab;
+    if (dr > 0 && lr > 0 && rr > 0) {
+        GemmTask task;
+        GemmPair pair;
+        make_task_pair(s, ds->kL, ds->mL, ds->kR, ds->nR, ds->mD, ds->nD, l, r, d,
+                       accumulate ? 1 : 0, task, pair);
+        const int cfg = pick_config(task.M, task.N);
+        const int bm = gemm_tile_m(cfg), bn = gemm_tile_n(cfg);
+        GemmArgs a{};
+        a.tab = tab;
+        a.tilesM = (task.M + bm - 1) / bm;
+        const int64_t ntiles = (int64_t)a.tilesM * ((task.N + bn - 1) / bn);
+        if (ntiles >= ((int64_t)1 << 31)) return fail(SIPGPU_ESIZE, "too many tiles");
+        a.ntiles = (int)ntiles;
+        task.first = 0;
+        task.npairs = 1;
+        a.task0 = task;
+        a.pair0 = pair;
+        HIP_TRY(launch_gemm(cfg, a, a.ntiles, st.s));
+        return 0;
+    }
+    if (accumulate) return fail(SIPGPU_EARG, "accumulate is only defined for block results");
+    if (dr == 0 && lr > 0 && rr > 0) {
+        rc = ensure_stream_scratch(st);
+        if (rc) return rc;
+        HIP_TRY(launch_dot(d, l, tab + ds->kL, r, tab + ds->kR, (int)s.K, st.scratch, st.s));
+    } else if (dr > 0 && lr > 0 && rr == 0) {
+        HIP_TRY(launch_scale_scatter(d, tab + ds->mD, l, tab + ds->mL, r, (int)s.M, st.s));
+    } else if (dr > 0 && lr == 0 && rr > 0) {
+        HIP_TRY(launch_scale_scatter(d, tab + ds->nD, r, tab + ds->nR, l, (int)s.N, st.s));
+    } else if (dr == 0 && lr == 0 && rr == 0) {
+        HIP_TRY(launch_scalar_mul(d, l, r, st.s));
+    }
+    return 0;
+}
+
+int64_t prod(int n, const int *dims) {
+    int64_t p = 1;
+    for (int i = 0; i < n; ++i) p *= dims[i];
+    return p;
+}
+
+} // namespace
+
+// ---------------------------------------------------------------------------
+// Batches
+// ---------------------------------------------------------------------------
+
+struct sipgpu_batch_s {
+    int cfg = kGemm64x64;
+    int ntiles = 0;     // including XCD padding slots
+    int ntasks = 0, npairs = 0, real_tiles = 0;
+    double flops = 0.0;
+    void *dev = nullptr; // tiles | tasks | pairs | tab
+    GemmArgs args{};
+};
+
+namespace {
+
+constexpr int kXcds = 8; // MI355X: workgroups go round-robin to 8 XCDs
+
+int plan_batch_locked(Context &c, const sipgpu_contraction *list, int n, sipgpu_batch_s &b) {
+    struct Entry {
+        const sipgpu_contraction *e;
+        std::string key;
+    };
+    // Shapes of this batch, deduplicated; tables pooled.
+    std::unordered_map<std::string, int> shape_index;
+    std::vector<const ContractionShape *> shapes;
+    std::vector<std::unique_ptr<ContractionShape>> owned;
+    std::vector<int> shape_of(n);
+    for (int i = 0; i < n; ++i) {
+        const sipgpu_contraction &e = list[i];
+        if (!e.l || !e.r || !e.d) return fail(SIPGPU_EARG, "null block pointer in batch");
+        if (e.lrank < 1 || e.lrank > SIPGPU_MAX_RANK || e.rrank < 1 ||
+            e.rrank > SIPGPU_MAX_RANK || e.drank < 1 || e.drank > SIPGPU_MAX_RANK)
+            return fail(SIPGPU_ERANK, "batch entries must be block = block * block, rank 1..6");
+        std::string key = shape_key(e.ptrn, e.lrank, e.ldims, e.rrank, e.rdims, e.drank, e.ddims);
+        auto it = shape_index.find(key);
+        if (it == shape_index.end()) {
+            auto s = std::make_unique<ContractionShape>();
+            int rc = plan_contraction(e.ptrn, e.lrank, e.ldims, e.rrank, e.rdims, e.drank,
+                                      e.ddims, *s, true);
+            if (rc == 1 || rc == -1)
+                return fail(SIPGPU_EPATTERN, "batch entry " + std::to_string(i) +
+                                                 ": contraction pattern rejected");
+            if (rc) return fail(rc, "batch entry " + std::to_string(i) + ": plan failed");
+            it = shape_index.emplace(key, (int)shapes.size()).first;
+            shapes.push_back(s.get());
+            owned.push_back(std::move(s));
+        }
+        shape_of[i] = it->second;
+    }
+
+    // Hazards: no destination may overlap an operand or another destination.
+    struct Range { uintptr_t lo, hi; int kind, who; };
+    std::vector<Range> rs;
+    for (int i = 0; i < n; ++i) {
+        const ContractionShape &s = *shapes[shape_of[i]];
+        auto rng = [](const double *p, int64_t cnt) {
+            return std::make_pair((uintptr_t)p, (uintptr_t)(p + cnt));
+        };
+        auto L = rng(list[i].l, s.lsize), R = rng(list[i].r, s.rsize), D = rng(list[i].d, s.dsize);
+        rs.push_back({L.first, L.second, 0, i});
+        rs.push_back({R.first, R.second, 0, i});
+        rs.push_back({D.first, D.second, 1, i});
+    }
+    {
+        std::vector<Range> ds;
+        for (auto &r : rs) if (r.kind == 1) ds.push_back(r);
+        std::sort(ds.begin(), ds.end(), [](const Range &a, const Range &b) { return a.lo < b.lo; });
+        for (size_t i = 1; i < ds.size(); ++i) {
+            if (ds[i].lo < ds[i - 1].hi && !(ds[i].lo == ds[i - 1].lo && ds[i].hi == ds[i - 1].hi))
+                return fail(SIPGPU_EHAZARD, "destination blocks partially overlap");
+        }
+        std::vector<Range> ops;
+        for (auto &r : rs) if (r.kind == 0) ops.push_back(r);
+        std::sort(ops.begin(), ops.end(), [](const Range &a, const Range &b) { return a.lo < b.lo; });
+        // prefix maximum of operand ends: a destination [lo, hi) overlaps an
+        // operand iff some operand starting before hi ends after lo
+        std::vector<uintptr_t> maxhi(ops.size());
+        uintptr_t m = 0;
+        for (size_t i = 0; i < ops.size(); ++i) maxhi[i] = m = std::max(m, ops[i].hi);
+        for (auto &d : ds) {
+            auto it = std::lower_bound(ops.begin(), ops.end(), d.hi,
+                                       [](const Range &a, uintptr_t v) { return a.lo < v; });
+            if (it != ops.begin() && maxhi[(it - ops.begin()) - 1] > d.lo)
+                return fail(SIPGPU_EHAZARD, "a batch reads a block that it writes");
+        }
+    }
+
+    // Tasks: entries grouped by destination, in list order.  An assign entry
+    // discards everything before it; the task keeps entries from the last
+    // assign on (beta = 0), or all of them with beta = 1.
+    std::unordered_map<const double *, std::vector<int>> by_dest;
+    std::vector<const double *> dest_order;
+    for (int i = 0; i < n; ++i) {
+        auto &v = by_dest[list[i].d];
+        if (v.empty()) dest_order.push_back(list[i].d);
+        v.push_back(i);
+    }
+
+    std::vector<int32_t> pool;
+    std::unordered_map<int, std::array<int, 6>> shape_tab;
+    auto tabs_of = [&](int si) -> std::array<int, 6> {
+        auto it = shape_tab.find(si);
+        if (it != shape_tab.end()) return it->second;
+        const ContractionShape &s = *shapes[si];
+        std::array<int, 6> at{};
+        const std::vector<int32_t> *ts[6] = {&s.koffL, &s.moffL, &s.koffR,
+                                             &s.noffR, &s.moffD, &s.noffD};
+        for (int t = 0; t < 6; ++t) {
+            at[t] = (int)pool.size();
+            pool.insert(pool.end(), ts[t]->begin(), ts[t]->end());
+        }
+        shape_tab.emplace(si, at);
+        return at;
+    };
+
+    std::vector<GemmTask> tasks;
+    std::vector<GemmPair> pairs;
+    std::vector<double> task_cost;
+    int64_t maxM = 0, maxN = 0, sumM = 0, sumN = 0;
+    b.flops = 0.0;
+    for (const double *dp : dest_order) {
+        const auto &idx = by_dest[dp];
+        size_t start = 0;
+        for (size_t j = 0; j < idx.size(); ++j)
+            if (!list[idx[j]].accumulate) start = j;
+        const int beta = list[idx[start]].accumulate ? 1 : 0;
+        const ContractionShape &s0 = *shapes[shape_of[idx[start]]];
+        const auto t0 = tabs_of(shape_of[idx[start]]);
+        GemmTask task;
+        GemmPair unused;
+        make_task_pair(s0, t0[0], t0[1], t0[2], t0[3], t0[4], t0[5], nullptr, nullptr,
+                       const_cast<double *>(dp), beta, task, unused);
+        task.first = (int)pairs.size();
+        double cost = 0.0;
+        for (size_t j = start; j < idx.size(); ++j) {
+            const int i = idx[j];
+            const ContractionShape &s = *shapes[shape_of[i]];
+            if (s.M != s0.M || s.N != s0.N || s.moffD != s0.moffD || s.noffD != s0.noffD ||
+                s.swap != s0.swap)
+                return fail(SIPGPU_EARG, "entries writing one block disagree on its layout");
+            const auto t = tabs_of(shape_of[i]);
+            GemmTask tk;
+            GemmPair pr;
+            make_task_pair(s, t[0], t[1], t[2], t[3], t[4], t[5], list[i].l, list[i].r,
+                           const_cast<double *>(dp), beta, tk, pr);
+            pairs.push_back(pr);
+            cost += 2.0 * (double)s.M * (double)s.N * (double)s.K;
+        }
+        task.npairs = (int)pairs.size() - task.first;
+        b.flops += cost;
+        tasks.push_back(task);
+        task_cost.push_back(cost);
+        maxM = std::max<int64_t>(maxM, task.M);
+        maxN = std::max<int64_t>(maxN, task.N);
+        sumM += task.M;
+        sumN += task.N;
+    }
+    const int ntasks = (int)tasks.size();
+    b.cfg = pick_config(ntasks ? sumM / ntasks : 0, ntasks ? sumN / ntasks : 0);
+    const int bm = gemm_tile_m(b.cfg), bn = gemm_tile_n(b.cfg);
+
+    // Static schedule: whole tasks to XCDs, heaviest first onto the least
+    // loaded XCD (their tiles then share that XCD's L2), heaviest tiles
+    // first within an XCD.  Slot t of the launch runs on XCD t % 8.
+    std::vector<int> order(ntasks);
+    for (int i = 0; i < ntasks; ++i) order[i] = i;
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int a, int b2) { return task_cost[a] > task_cost[b2]; });
+    std::vector<std::vector<GemmTile>> per_xcd(kXcds);
+    std::vector<double> load(kXcds, 0.0);
+    for (int ti : order) {
+        const int x = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+        const int tm = (tasks[ti].M + bm - 1) / bm, tn = (tasks[ti].N + bn - 1) / bn;
+        for (int q = 0; q < tn; ++q)
+            for (int p = 0; p < tm; ++p) per_xcd[x].push_back(GemmTile{ti, p, q, 0});
+        load[x] += task_cost[ti];
+        b.real_tiles += tm * tn;
+    }
+    size_t per = 0;
+    for (auto &v : per_xcd) per = std::max(per, v.size());
+    std::vector<GemmTile> tiles(per * kXcds, GemmTile{-1, 0, 0, 0});
+    for (int x = 0; x < kXcds; ++x)
+        for (size_t j = 0; j < per_xcd[x].size(); ++j) tiles[j * kXcds + x] = per_xcd[x][j];
+
+    // Upload: tiles | tasks | pairs | tables, each 256-byte aligned.
+    auto align = [](size_t v) { return (v + 255) & ~(size_t)255; };
+    const size_t o_tiles = 0;
+    const size_t o_tasks = align(o_tiles + tiles.size() * sizeof(GemmTile));
+    const size_t o_pairs = align(o_tasks + tasks.size() * sizeof(GemmTask));
+    const size_t o_tab = align(o_pairs + pairs.size() * sizeof(GemmPair));
+    const size_t total = align(o_tab + pool.size() * sizeof(int32_t)) + 256;
+    std::vector<unsigned char> host(total, 0);
+    std::memcpy(host.data() + o_tiles, tiles.data(), tiles.size() * sizeof(GemmTile));
+    std::memcpy(host.data() + o_tasks, tasks.data(), tasks.size() * sizeof(GemmTask));
+    std::memcpy(host.data() + o_pairs, pairs.data(), pairs.size() * sizeof(GemmPair));
+    std::memcpy(host.data() + o_tab, pool.data(), pool.size() * sizeof(int32_t));
+    HIP_TRY(hipMalloc(&b.dev, total));
+    HIP_TRY(hipMemcpy(b.dev, host.data(), total, hipMemcpyHostToDevice));
+    unsigned char *base = static_cast<unsigned char *>(b.dev);
+    b.args = GemmArgs{};
+    b.args.tiles = reinterpret_cast<const GemmTile *>(base + o_tiles);
+    b.args.tasks = reinterpret_cast<const GemmTask *>(base + o_tasks);
+    b.args.pairs = reinterpret_cast<const GemmPair *>(base + o_pairs);
+    b.args.tab = reinterpret_cast<const int32_t *>(base + o_tab);
+    b.args.ntiles = (int)tiles.size();
+    b.ntiles = (int)tiles.size();
+    b.ntasks = ntasks;
+    b.npairs = (int)pairs.size();
+    (void)maxM;
+    (void)maxN;
+    return 0;
+}
+
+} // namespace
+
+extern "C" {
+
+// ---------------------------------------------------------------------------
+// Drop-in layer
+// ---------------------------------------------------------------------------
+
+long long tensor_size_by_shape_(int *num_dim, int *dims, int *ierr) {
+    long long s = 1;
+    *ierr = 0;
+    if (*num_dim > 0) {
+        for (int i = 0; i < *num_dim; ++i) s *= dims[i];
+        if (s <= 0) *ierr = 2;
+    } else if (*num_dim < 0) {
+        *ierr = 1;
+        s = 0;
+    }
+    return s;
+}
+
+void get_contraction_ptrn_(int *drank, int *lrank, int *rrank, int *aces_ptrn, int *my_ptrn,
+                           int *ierr) {
+    *ierr = convert_aces_pattern(*drank, *lrank, *rrank, aces_ptrn, my_ptrn);
+}
+
+void tensor_block_copy__(int *nthreads, int *rank, int *dims, int *dim_transp, double *in,
+                         double *out, int *ierr) {
+    (void)nthreads;
+    *ierr = 0;
+    if (*rank < 0) { *ierr = *rank; return; }
+    Context &c = ctx();
+    std::lock_guard<std::mutex> g(c.mu);
+    int rc = ensure_init(c);
+    if (rc) { *ierr = rc; return; }
+    PermutePlan p;
+    if (*rank == 0) {
+        p.size = 1;
+        p.identity = true;
+    } else {
+        rc = plan_permute(*rank, dims, dim_transp, p);
+        if (rc) { *ierr = rc; return; }
+    }
+    if (p.size <= 0) return;
+    const size_t bytes = (size_t)p.size * sizeof(double);
+    if ((rc = c.stage_l.reserve(bytes)) || (rc = c.stage_d.reserve(bytes))) { *ierr = rc; return; }
+    hipStream_t s = c.stream.s;
+    hipError_t e = hipMemcpyAsync(c.stage_l.p, in, bytes, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess)
+        e = launch_permute(p, static_cast<double *>(c.stage_l.p), static_cast<double *>(c.stage_d.p), s);
+    if (e == hipSuccess) e = hipMemcpyAsync(out, c.stage_d.p, bytes, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) *ierr = hip_fail(e, "tensor_block_copy__");
+}
+
+void tensor_block_contract__(int *nthreads, int *ptrn, double *l, int *lrank, int *ldims,
+                             double *r, int *rrank, int *rdims, double *d, int *drank,
+                             int *ddims, int *ierr) {
+    (void)nthreads;
+    *ierr = 0;
+    const int lr = *lrank, rr = *rrank, dr = *drank;
+    if (lr < 0 || lr > kRefMaxRank || rr < 0 || rr > kRefMaxRank || dr < 0 || dr > kRefMaxRank) {
+        *ierr = -1;
+        return;
+    }
+    if (!pattern_valid(ptrn, lr, ldims, rr, rdims, dr, ddims)) { *ierr = 1; return; }
+    Context &c = ctx();
+    std::lock_guard<std::mutex> g(c.mu);
+    int rc = ensure_init(c);
+    if (rc) { *ierr = rc; return; }
+    const int64_t ls = prod(lr, ldims), rs = prod(rr, rdims), dsz = prod(dr, ddims);
+    if (ls <= 0 || rs <= 0 || dsz <= 0) return; // empty block: nothing to do
+    if ((rc = c.stage_l.reserve(ls * 8)) || (rc = c.stage_r.reserve(rs * 8)) ||
+        (rc = c.stage_d.reserve(dsz * 8))) {
+        *ierr = rc;
+        return;
+    }
+    hipStream_t s = c.stream.s;
+    hipError_t e = hipMemcpyAsync(c.stage_l.p, l, ls * 8, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(c.stage_r.p, r, rs * 8, hipMemcpyHostToDevice, s);
+    if (e != hipSuccess) { *ierr = hip_fail(e, "tensor_block_contract__ upload"); return; }
+    rc = contract_device(c, c.stream, ptrn, static_cast<double *>(c.stage_l.p), lr, ldims,
+                         static_cast<double *>(c.stage_r.p), rr, rdims,
+                         static_cast<double *>(c.stage_d.p), dr, ddims, 0);
+    if (rc) { *ierr = rc; return; }
+    e = hipMemcpyAsync(d, c.stage_d.p, dsz * 8, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) *ierr = hip_fail(e, "tensor_block_contract__ download");
+}
+
+// ---------------------------------------------------------------------------
+// Native layer
+// ---------------------------------------------------------------------------
+
+const char *sipgpu_last_error(void) { return g_last_error.c_str(); }
+const char *sipgpu_version(void) { return "sipgpu 0.1 gfx950"; }
+
+int sipgpu_init(int device) {
+    Context &c = ctx();
+    std::lock_guard<std::mutex> g(c.mu);
+    if (c.ready && c.device != device) return fail(SIPGPU_EARG, "already initialised on another device");
+    return init_locked(c, device);
+}
+
+int sipgpu_device_count(int *count) {
+    if (!count) return fail(SIPGPU_EARG, "null count");
+    hipError_t e = hipGetDeviceCount(count);
+    if (e == hipErrorNoDevice) { *count = 0; return 0; }
+    if (e != hipSuccess) return hip_fail(e, "hipGetDeviceCount");
+    return 0;
+}
+
+int sipgpu_device_synchronize(void) {
+    HIP_TRY(hipDeviceSynchronize());
+    return 0;
+}
+
+int sipgpu_malloc(void **dptr, size_t bytes) {
+    if (!dptr) return fail(SIPGPU_EARG, "null pointer");
+    Context &c = ctx();
+    {
+        std::lock_guard<std::mutex> g(c.mu);
+        int rc = ensure_init(c);
+        if (rc) return rc;
+    }
+    hipError_t e = hipMalloc(dptr, bytes ? bytes : 1);
+    if (e != hipSuccess) return fail(SIPGPU_ENOMEM, "hipMalloc failed");
+    return 0;
+}
+
+int sipgpu_free(void *dptr) {
+    HIP_TRY(hipDeviceSynchronize()); // no launch in flight may still use it
+    HIP_TRY(hipFree(dptr));
+    return 0;
+}
+
+int sipgpu_memcpy_h2d(void *dst, const void *src, size_t bytes) {
+    HIP_TRY(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+    return 0;
+}
+int sipgpu_memcpy_d2h(void *dst, const void *src, size_t bytes) {
+    HIP_TRY(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+    return 0;
+}
+int sipgpu_memcpy_d2d(void *dst, const void *src, size_t bytes) {
+    HIP_TRY(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToDevice));
+    return 0;
+}
+int sipgpu_memset(void *dptr, int value, size_t bytes) {
+    HIP_TRY(hipMemset(dptr, value, bytes));
+    return 0;
+}
+
+int sipgpu_stream_create(sipgpu_stream_t *s) {
+    if (!s) return fail(SIPGPU_EARG, "null stream");
+    Context &c = ctx();
+    {
+        std::lock_guard<std::mutex> g(c.mu);
+        int rc = ensure_init(c);
+        if (rc) return rc;
+    }
+    auto *st = new sipgpu_stream_s();
+    hipError_t e = hipStreamCreateWithFlags(&st->s, hipStreamNonBlocking);
+    if (e != hipSuccess) { delete st; return hip_fail(e, "hipStreamCreate"); }
+    int rc = ensure_stream_scratch(*st);
+    if (rc) { (void)hipStreamDestroy(st->s); delete st; return rc; }
+    *s = st;
+    return 0;
+}
+
+int sipgpu_stream_destroy(sipgpu_stream_t s) {
+    if (!s) return fail(SIPGPU_EARG, "null stream");
+    hipError_t e = hipStreamSynchronize(s->s);
+    if (s->scratch) (void)hipFree(s->scratch);
+    if (e == hipSuccess) e = hipStreamDestroy(s->s);
+    delete s;
+    if (e != hipSuccess) return hip_fail(e, "hipStreamDestroy");
+    return 0;
+}
+
+int sipgpu_stream_synchronize(sipgpu_stream_t s) {
+    Context &c = ctx();
+    if (!s && !c.ready) return 0;
+    HIP_TRY(hipStreamSynchronize(resolve(c, s).s));
+    return 0;
+}
+
+int sipgpu_event_create(sipgpu_event_t *e) {
+    if (!e) return fail(SIPGPU_EARG, "null event");
+    auto *ev = new sipgpu_event_s();
+    hipError_t r = hipEventCreate(&ev->e);
+    if (r != hipSuccess) { delete ev; return hip_fail(r, "hipEventCreate"); }
+    *e = ev;
+    return 0;
+}
+int sipgpu_event_destroy(sipgpu_event_t e) {
+    if (!e) return fail(SIPGPU_EARG, "null event");
+    hipError_t r = hipEventDestroy(e->e);
+    delete e;
+    if (r != hipSuccess) return hip_fail(r, "hipEventDestroy");
+    return 0;
+}
+int sipgpu_event_record(sipgpu_event_t e, sipgpu_stream_t s) {
+    if (!e) return fail(SIPGPU_EARG, "null event");
+    Context &c = ctx();
+    {
+        std::lock_guard<std::mutex> g(c.mu);
+        int rc = ensure_init(c);
+        if (rc) return rc;
+    }
+    HIP_TRY(hipEventRecord(e->e, resolve(c, s).s));
+    return 0;
+}
+int sipgpu_event_synchronize(sipgpu_event_t e) {
+    if (!e) return fail(SIPGPU_EARG, "null event");
+    HIP_TRY(hipEventSynchronize(e->e));
+    return 0;
+}
+int sipgpu_event_elapsed_ms(sipgpu_event_t a, sipgpu_event_t b, float *ms) {
+    if (!a || !b || !ms) return fail(SIPGPU_EARG, "null argument");
+    HIP_TRY(hipEventElapsedTime(ms, a->e, b->e));
+    return 0;
+}
+
+int sipgpu_permute(sipgpu_stream_t s, int rank, const int *dims, const int *dim_transp,
+                   const double *in, double *out) {
+    if (rank < 0) return fail(SIPGPU_ERANK, "negative rank");
+    if (!in || !out) return fail(SIPGPU_EARG, "null block pointer");
+    Context &c = ctx();
+    std::lock_guard<std::mutex> g(c.mu);
+    int rc = ensure_init(c);
+    if (rc) return rc;
+    PermutePlan p;
+    if (rank == 0) {
+        p.size = 1;
+        p.identity = true;
+    } else {
+        rc = plan_permute(rank, dims, dim_transp, p);
+        if (rc) return fail(rc, "invalid permutation");
+    }
+    HIP_TRY(launch_permute(p, in, out, resolve(c, s).s));
+    return 0;
+}
+
+int sipgpu_contract(sipgpu_stream_t s, const int *ptrn, const double *l, int lrank,
+                    const int *ldims, const double *r, int rrank, const int *rdims, double *d,
+                    int drank, const int *ddims, int accumulate) {
+    if (!l || !r || !d || (lrank + rrank > 0 && !ptrn)) return fail(SIPGPU_EARG, "null argument");
+    if (lrank < 0 || lrank > SIPGPU_MAX_RANK || rrank < 0 || rrank > SIPGPU_MAX_RANK ||
+        drank < 0 || drank > SIPGPU_MAX_RANK)
+        return fail(SIPGPU_ERANK, "rank outside 0..6");
+    Context &c = ctx();
+    std::lock_guard<std::mutex> g(c.mu);
+    int rc = ensure_init(c);
+    if (rc) return rc;
+    rc = contract_device(c, resolve(c, s), ptrn, l, lrank, ldims, r, rrank, rdims, d, drank,
+                         ddims, accumulate);
+    if (rc == 1 || rc == -1) return fail(SIPGPU_EPATTERN, "contraction pattern rejected");
+    return rc;
+}
+
+int sipgpu_batch_plan(const sipgpu_contraction *list, int n, sipgpu_batch_t *batch) {
+    if (!batch || (n > 0 && !list) || n < 0) return fail(SIPGPU_EARG, "bad batch arguments");
+    Context &c = ctx();
+    std::lock_guard<std::mutex> g(c.mu);
+    int rc = ensure_init(c);
+    if (rc) return rc;
+    auto b = std::make_unique<sipgpu_batch_s>();
+    rc = plan_batch_locked(c, list, n, *b);
+    if (rc) {
+        if (b->dev) (void)hipFree(b->dev);
+        return rc;
+    }
+    *batch = b.release();
+    return 0;
+}
+
+int sipgpu_batch_execute(sipgpu_batch_t b, sipgpu_stream_t s) {
+    if (!b) return fail(SIPGPU_EARG, "null batch");
+    Context &c = ctx();
+    if (b->ntiles == 0) return 0;
+    HIP_TRY(launch_gemm(b->cfg, b->args, b->ntiles, resolve(c, s).s));
+    return 0;
+}
+
+int sipgpu_batch_destroy(sipgpu_batch_t b) {
+    if (!b) return fail(SIPGPU_EARG, "null batch");
+    // the work list may still be read by a launch in flight on any stream
+    hipError_t e = hipDeviceSynchronize();
+    if (e != hipSuccess) return hip_fail(e, "sipgpu_batch_destroy");
+    if (b->dev) (void)hipFree(b->dev);
+    delete b;
+    return 0;
+}
+
+int sipgpu_batch_info(sipgpu_batch_t b, double *flops, int *ntasks, int *ntiles, int *npairs) {
+    if (!b) return fail(SIPGPU_EARG, "null batch");
+    if (flops) *flops = b->flops;
+    if (ntasks) *ntasks = b->ntasks;
+    if (ntiles) *ntiles = b->real_tiles;
+    if (npairs) *npairs = b->npairs;
+    return 0;
+}
+
+int sipgpu_plan_tables(const int *ptrn, int lrank, const int *ldims, int rrank, const int *rdims,
+                       int drank, const int *ddims, long long *mnk, int *o2n, int *koffL,
+                       int *moffL, int *koffR, int *noffR, int *moffD, int *noffD) {
+    if (lrank < 0 || lrank > kRefMaxRank || rrank < 0 || rrank > kRefMaxRank || drank < 0 ||
+        drank > kRefMaxRank)
+        return fail(SIPGPU_ERANK, "rank outside 0..32");
+    ContractionShape s;
+    const bool tables = koffL || moffL || koffR || noffR || moffD || noffD;
+    int rc = plan_contraction(ptrn, lrank, ldims, rrank, rdims, drank, ddims, s, tables);
+    if (rc == 1 || rc == -1) return fail(SIPGPU_EPATTERN, "contraction pattern rejected");
+    if (rc) return fail(rc, "plan failed");
+    if (mnk) { mnk[0] = s.M; mnk[1] = s.N; mnk[2] = s.K; }
+    if (o2n) {
+        int j = 0;
+        for (int i = 1; i <= lrank; ++i) o2n[j++] = s.lo2n[i];
+        for (int i = 1; i <= rrank; ++i) o2n[j++] = s.ro2n[i];
+        for (int i = 1; i <= drank; ++i) o2n[j++] = s.do2n[i];
+    }
+    auto put = [](int *dst, const std::vector<int32_t> &t) {
+        if (dst) std::copy(t.begin(), t.end(), dst);
+    };
+    put(koffL, s.koffL); put(moffL, s.moffL); put(koffR, s.koffR);
+    put(noffR, s.noffR); put(moffD, s.moffD); put(noffD, s.noffD);
+    return 0;
+}
+
+int sipgpu_plan_permute(int rank, const int *dims, const int *dim_transp, int *info) {
+    if (!info || (rank > 0 && (!dims || !dim_transp))) return fail(SIPGPU_EARG, "null argument");
+    PermutePlan p;
+    if (rank == 0) {
+        p.identity = true;
+    } else {
+        int rc = plan_permute(rank, dims, dim_transp, p);
+        if (rc) return fail(rc, "invalid permutation");
+    }
+    info[0] = p.rank;
+    info[1] = p.nU;
+    info[2] = p.row_mode ? 1 : 0;
+    info[3] = p.identity ? 1 : 0;
+    return 0;
+}
+
+} // extern "C"

@@ -1,0 +1,362 @@
+"""ctypes bindings to libsipgpu.so, the gfx950 SIP contract/permute library.
+
+The drop-in functions mirror the reference's tensor-kernel interface
+(src/sip/tensor_algebra/tensor_ops_c_prototypes.h): same names without the
+trailing underscores, same argument meaning, same ierr codes, host arrays in
+and out.  The native classes wrap device blocks, streams, events and planned
+batches.
+
+There is no CPU fallback: if the library is missing this module raises on
+first use, and every compute call goes to the GPU.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libsipgpu.so")
+MAX_RANK = 6
+
+ERRORS = {
+    -101: "SIPGPU_EPATTERN", -102: "SIPGPU_ERANK", -103: "SIPGPU_EPERM",
+    -104: "SIPGPU_ESIZE", -105: "SIPGPU_EDEVICE", -106: "SIPGPU_ENOMEM",
+    -107: "SIPGPU_EARG", -108: "SIPGPU_EHAZARD", -109: "SIPGPU_EBUDGET",
+    -110: "SIPGPU_ENOTFOUND",
+}
+
+_ip = ctypes.POINTER(ctypes.c_int)
+_dp = ctypes.POINTER(ctypes.c_double)
+_vp = ctypes.c_void_p
+
+EXPORTED = [
+    "tensor_size_by_shape_", "get_contraction_ptrn_", "tensor_block_copy__",
+    "tensor_block_contract__", "sipgpu_last_error", "sipgpu_version", "sipgpu_init",
+    "sipgpu_device_count", "sipgpu_device_synchronize", "sipgpu_malloc", "sipgpu_free",
+    "sipgpu_memcpy_h2d", "sipgpu_memcpy_d2h", "sipgpu_memcpy_d2d", "sipgpu_memset",
+    "sipgpu_stream_create", "sipgpu_stream_destroy", "sipgpu_stream_synchronize",
+    "sipgpu_event_create", "sipgpu_event_destroy", "sipgpu_event_record",
+    "sipgpu_event_synchronize", "sipgpu_event_elapsed_ms", "sipgpu_permute",
+    "sipgpu_contract", "sipgpu_batch_plan", "sipgpu_batch_execute", "sipgpu_batch_destroy",
+    "sipgpu_batch_info", "sipgpu_plan_tables", "sipgpu_plan_permute",
+]
+
+
+class SipGpuError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("%s (%d): %s" % (ERRORS.get(code, "error"), code, msg))
+        self.code = code
+
+
+class Contraction(ctypes.Structure):
+    """sipgpu_contraction of include/sipgpu.h."""
+    _fields_ = [("l", _vp), ("r", _vp), ("d", _vp), ("lrank", ctypes.c_int),
+                ("rrank", ctypes.c_int), ("drank", ctypes.c_int),
+                ("ldims", ctypes.c_int * MAX_RANK), ("rdims", ctypes.c_int * MAX_RANK),
+                ("ddims", ctypes.c_int * MAX_RANK), ("ptrn", ctypes.c_int * (2 * MAX_RANK)),
+                ("accumulate", ctypes.c_int)]
+
+
+_lib = None
+
+
+def lib():
+    """Load libsipgpu.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("libsipgpu.so is not built (%s); run aces4_amd/build.py"
+                               % LIB_PATH)
+        L = ctypes.CDLL(LIB_PATH)
+        L.tensor_size_by_shape_.restype = ctypes.c_longlong
+        L.sipgpu_last_error.restype = ctypes.c_char_p
+        L.sipgpu_version.restype = ctypes.c_char_p
+        L.sipgpu_malloc.argtypes = [ctypes.POINTER(_vp), ctypes.c_size_t]
+        L.sipgpu_free.argtypes = [_vp]
+        for f in ("sipgpu_memcpy_h2d", "sipgpu_memcpy_d2h", "sipgpu_memcpy_d2d"):
+            getattr(L, f).argtypes = [_vp, _vp, ctypes.c_size_t]
+        L.sipgpu_memset.argtypes = [_vp, ctypes.c_int, ctypes.c_size_t]
+        L.sipgpu_stream_create.argtypes = [ctypes.POINTER(_vp)]
+        L.sipgpu_stream_destroy.argtypes = [_vp]
+        L.sipgpu_stream_synchronize.argtypes = [_vp]
+        L.sipgpu_event_create.argtypes = [ctypes.POINTER(_vp)]
+        L.sipgpu_event_destroy.argtypes = [_vp]
+        L.sipgpu_event_record.argtypes = [_vp, _vp]
+        L.sipgpu_event_synchronize.argtypes = [_vp]
+        L.sipgpu_event_elapsed_ms.argtypes = [_vp, _vp, ctypes.POINTER(ctypes.c_float)]
+        L.sipgpu_permute.argtypes = [_vp, ctypes.c_int, _ip, _ip, _vp, _vp]
+        L.sipgpu_contract.argtypes = [_vp, _ip, _vp, ctypes.c_int, _ip, _vp, ctypes.c_int, _ip,
+                                      _vp, ctypes.c_int, _ip, ctypes.c_int]
+        L.sipgpu_batch_plan.argtypes = [ctypes.POINTER(Contraction), ctypes.c_int,
+                                        ctypes.POINTER(_vp)]
+        L.sipgpu_batch_execute.argtypes = [_vp, _vp]
+        L.sipgpu_batch_destroy.argtypes = [_vp]
+        L.sipgpu_batch_info.argtypes = [_vp, ctypes.POINTER(ctypes.c_double),
+                                        ctypes.POINTER(ctypes.c_int),
+                                        ctypes.POINTER(ctypes.c_int),
+                                        ctypes.POINTER(ctypes.c_int)]
+        L.sipgpu_plan_tables.argtypes = [_ip, ctypes.c_int, _ip, ctypes.c_int, _ip,
+                                         ctypes.c_int, _ip, ctypes.POINTER(ctypes.c_longlong),
+                                         _ip, _ip, _ip, _ip, _ip, _ip, _ip]
+        L.sipgpu_plan_permute.argtypes = [ctypes.c_int, _ip, _ip, _ip]
+        _lib = L
+    return _lib
+
+
+def _check(rc):
+    if rc != 0:
+        raise SipGpuError(rc, lib().sipgpu_last_error().decode())
+    return rc
+
+
+def _ints(seq, n=None):
+    vals = [int(x) for x in seq]
+    if n is not None:
+        vals = vals + [0] * (n - len(vals))
+    a = np.array(vals if vals else [0], dtype=np.int32)
+    return a
+
+
+def _iptr(a):
+    return a.ctypes.data_as(_ip)
+
+
+# --------------------------------------------------------------- drop-in ----
+
+def tensor_size_by_shape(dims):
+    d = _ints(dims)
+    ierr = ctypes.c_int(0)
+    s = lib().tensor_size_by_shape_(ctypes.byref(ctypes.c_int(len(dims))), _iptr(d),
+                                    ctypes.byref(ierr))
+    return s, ierr.value
+
+
+def get_contraction_ptrn(drank, lrank, rrank, aces):
+    """ACES index labels [d | l | r] -> (+k / -k pattern, ierr)."""
+    a = _ints(aces)
+    out = np.zeros(max(lrank + rrank, 1), dtype=np.int32)
+    ierr = ctypes.c_int(0)
+    lib().get_contraction_ptrn_(ctypes.byref(ctypes.c_int(drank)),
+                                ctypes.byref(ctypes.c_int(lrank)),
+                                ctypes.byref(ctypes.c_int(rrank)), _iptr(a), _iptr(out),
+                                ctypes.byref(ierr))
+    return out[:lrank + rrank].copy(), ierr.value
+
+
+def tensor_block_copy(dims, dim_transp, x):
+    """Permute a host block on the GPU; returns (out, ierr)."""
+    d = _ints(dims)
+    t = _ints(dim_transp)
+    x = np.ascontiguousarray(x, dtype=np.float64).ravel()
+    out = np.empty_like(x)
+    ierr = ctypes.c_int(0)
+    lib().tensor_block_copy__(ctypes.byref(ctypes.c_int(8)),
+                              ctypes.byref(ctypes.c_int(len(dims))), _iptr(d), _iptr(t),
+                              x.ctypes.data_as(_dp), out.ctypes.data_as(_dp), ctypes.byref(ierr))
+    return out, ierr.value
+
+
+def tensor_block_contract(ptrn, l, ldims, r, rdims, ddims, d_init=None):
+    """D = L (x)_ptrn R on the GPU with the reference's assign semantics."""
+    p = _ints(ptrn)
+    ld, rd, dd = _ints(ldims), _ints(rdims), _ints(ddims)
+    l = np.ascontiguousarray(l, dtype=np.float64).ravel()
+    r = np.ascontiguousarray(r, dtype=np.float64).ravel()
+    dsize = int(np.prod(ddims)) if len(ddims) else 1
+    d = (np.full(dsize, np.nan) if d_init is None
+         else np.ascontiguousarray(d_init, dtype=np.float64).ravel().copy())
+    ierr = ctypes.c_int(0)
+    lib().tensor_block_contract__(ctypes.byref(ctypes.c_int(8)), _iptr(p), l.ctypes.data_as(_dp),
+                                  ctypes.byref(ctypes.c_int(len(ldims))), _iptr(ld),
+                                  r.ctypes.data_as(_dp), ctypes.byref(ctypes.c_int(len(rdims))),
+                                  _iptr(rd), d.ctypes.data_as(_dp),
+                                  ctypes.byref(ctypes.c_int(len(ddims))), _iptr(dd),
+                                  ctypes.byref(ierr))
+    return d, ierr.value
+
+
+def plan_tables(ptrn, ldims, rdims, ddims):
+    """Host-only: (M, N, K), o2n maps and the six offset tables of a shape."""
+    p = _ints(ptrn)
+    ld, rd, dd = _ints(ldims), _ints(rdims), _ints(ddims)
+    mnk = (ctypes.c_longlong * 3)()
+    o2n = np.zeros(len(ldims) + len(rdims) + len(ddims) + 1, np.int32)
+    _check(lib().sipgpu_plan_tables(_iptr(p), len(ldims), _iptr(ld), len(rdims), _iptr(rd),
+                                    len(ddims), _iptr(dd), mnk, _iptr(o2n),
+                                    None, None, None, None, None, None))
+    M, N, K = mnk[0], mnk[1], mnk[2]
+    t = [np.zeros(max(n, 1), np.int32) for n in (K, M, K, N, M, N)]
+    _check(lib().sipgpu_plan_tables(_iptr(p), len(ldims), _iptr(ld), len(rdims), _iptr(rd),
+                                    len(ddims), _iptr(dd), mnk, _iptr(o2n),
+                                    *[_iptr(a) for a in t]))
+    lr, rr = len(ldims), len(rdims)
+    maps = dict(lo2n=list(o2n[:lr]), ro2n=list(o2n[lr:lr + rr]),
+                do2n=list(o2n[lr + rr:lr + rr + len(ddims)]))
+    names = ("koffL", "moffL", "koffR", "noffR", "moffD", "noffD")
+    return (M, N, K), maps, dict(zip(names, t))
+
+
+def plan_permute(dims, dim_transp):
+    """Host-only: (canonical rank, leading tile dims, row mode, plain copy)."""
+    info = np.zeros(4, np.int32)
+    _check(lib().sipgpu_plan_permute(len(dims), _iptr(_ints(dims)), _iptr(_ints(dim_transp)),
+                                     _iptr(info)))
+    return tuple(int(x) for x in info)
+
+
+# ---------------------------------------------------------------- native ----
+
+def init(device=0):
+    _check(lib().sipgpu_init(device))
+
+
+def device_count():
+    n = ctypes.c_int(0)
+    _check(lib().sipgpu_device_count(ctypes.byref(n)))
+    return n.value
+
+
+def synchronize():
+    _check(lib().sipgpu_device_synchronize())
+
+
+class DeviceArray:
+    """A plain HBM allocation of float64 (or raw bytes) owned by Python."""
+
+    def __init__(self, n, dtype=np.float64):
+        self.dtype = np.dtype(dtype)
+        self.n = int(n)
+        self.nbytes = self.n * self.dtype.itemsize
+        p = _vp()
+        _check(lib().sipgpu_malloc(ctypes.byref(p), max(self.nbytes, 1)))
+        self.ptr = p.value
+
+    @classmethod
+    def from_host(cls, x):
+        x = np.ascontiguousarray(x)
+        a = cls(x.size, x.dtype)
+        a.upload(x)
+        return a
+
+    def upload(self, x):
+        x = np.ascontiguousarray(x, dtype=self.dtype).ravel()
+        assert x.size == self.n
+        _check(lib().sipgpu_memcpy_h2d(self.ptr, x.ctypes.data, self.nbytes))
+
+    def download(self):
+        out = np.empty(self.n, dtype=self.dtype)
+        _check(lib().sipgpu_memcpy_d2h(out.ctypes.data, self.ptr, self.nbytes))
+        return out
+
+    def fill_bytes(self, value):
+        _check(lib().sipgpu_memset(self.ptr, value, self.nbytes))
+
+    def at(self, offset):
+        """Device address of element `offset`."""
+        return self.ptr + int(offset) * self.dtype.itemsize
+
+    def free(self):
+        if self.ptr:
+            _check(lib().sipgpu_free(self.ptr))
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Stream:
+    def __init__(self):
+        p = _vp()
+        _check(lib().sipgpu_stream_create(ctypes.byref(p)))
+        self.handle = p.value
+
+    def synchronize(self):
+        _check(lib().sipgpu_stream_synchronize(self.handle))
+
+    def destroy(self):
+        if self.handle:
+            _check(lib().sipgpu_stream_destroy(self.handle))
+            self.handle = None
+
+
+class Event:
+    def __init__(self):
+        p = _vp()
+        _check(lib().sipgpu_event_create(ctypes.byref(p)))
+        self.handle = p.value
+
+    def record(self, stream=None):
+        _check(lib().sipgpu_event_record(self.handle, stream.handle if stream else None))
+
+    def synchronize(self):
+        _check(lib().sipgpu_event_synchronize(self.handle))
+
+    def elapsed_ms(self, later):
+        ms = ctypes.c_float(0)
+        _check(lib().sipgpu_event_elapsed_ms(self.handle, later.handle, ctypes.byref(ms)))
+        return ms.value
+
+
+def permute(rank_dims, dim_transp, din, dout, stream=None):
+    """Permute device block `din` into `dout` (device addresses)."""
+    d = _ints(rank_dims)
+    t = _ints(dim_transp)
+    _check(lib().sipgpu_permute(stream.handle if stream else None, len(rank_dims), _iptr(d),
+                                _iptr(t), din, dout))
+
+
+def contract(ptrn, l, ldims, r, rdims, d, ddims, accumulate=False, stream=None):
+    """Contract device blocks (addresses) asynchronously."""
+    p = _ints(ptrn)
+    ld, rd, dd = _ints(ldims), _ints(rdims), _ints(ddims)
+    _check(lib().sipgpu_contract(stream.handle if stream else None, _iptr(p), l, len(ldims),
+                                 _iptr(ld), r, len(rdims), _iptr(rd), d, len(ddims), _iptr(dd),
+                                 1 if accumulate else 0))
+
+
+def make_contraction(ptrn, l, ldims, r, rdims, d, ddims, accumulate=False):
+    c = Contraction()
+    c.l, c.r, c.d = l, r, d
+    c.lrank, c.rrank, c.drank = len(ldims), len(rdims), len(ddims)
+    for i, v in enumerate(ldims):
+        c.ldims[i] = int(v)
+    for i, v in enumerate(rdims):
+        c.rdims[i] = int(v)
+    for i, v in enumerate(ddims):
+        c.ddims[i] = int(v)
+    for i, v in enumerate(ptrn):
+        c.ptrn[i] = int(v)
+    c.accumulate = 1 if accumulate else 0
+    return c
+
+
+class Batch:
+    """A planned batch of contractions: one grouped FP64-MFMA launch."""
+
+    def __init__(self, contractions):
+        arr = (Contraction * len(contractions))(*contractions)
+        p = _vp()
+        _check(lib().sipgpu_batch_plan(arr, len(contractions), ctypes.byref(p)))
+        self.handle = p.value
+        fl = ctypes.c_double(0)
+        nt, ntl, npr = ctypes.c_int(0), ctypes.c_int(0), ctypes.c_int(0)
+        _check(lib().sipgpu_batch_info(self.handle, ctypes.byref(fl), ctypes.byref(nt),
+                                       ctypes.byref(ntl), ctypes.byref(npr)))
+        self.flops, self.ntasks, self.ntiles, self.npairs = fl.value, nt.value, ntl.value, npr.value
+
+    def execute(self, stream=None):
+        _check(lib().sipgpu_batch_execute(self.handle, stream.handle if stream else None))
+
+    def destroy(self):
+        if self.handle:
+            _check(lib().sipgpu_batch_destroy(self.handle))
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass

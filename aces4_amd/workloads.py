@@ -1,0 +1,190 @@
+"""Synthetic coupled-cluster contraction workloads on segmented blocks.
+
+Each workload is a SIAL-style contraction over segmented index ranges, e.g.
+the ring term R(a,i,b,j) += sum_{k,c} W(k,b,c,j) * T(a,i,c,k): the SIP runs it
+as a `pardo a,i,b,j` over result blocks with an inner `do k; do c` over
+operand blocks, one `contract` super-instruction per block pair.  Here the
+arrays are generated whole (seeded, portable counter-based RNG), cut into
+dense column-major blocks exactly as the SIP's block shapes
+(segment extents per index), and packed into one device arena per array.
+
+Input data are bit-identical on every machine for the uniform draws
+(SplitMix64 is pure integer arithmetic); normal draws use numpy's log/cos
+and are reproducible within one machine.
+"""
+import itertools
+
+import numpy as np
+
+from . import sipgpu as sg
+
+_M64 = np.uint64(0xFFFFFFFFFFFFFFFF)
+
+
+def splitmix64(seed, n, stream=0):
+    """n 64-bit outputs of SplitMix64 for (seed, stream), counter-based."""
+    with np.errstate(over="ignore"):
+        base = np.uint64(seed) * np.uint64(0x9E3779B97F4A7C15) + np.uint64(stream) * np.uint64(
+            0xD1B54A32D192ED03)
+        z = base + (np.arange(1, n + 1, dtype=np.uint64) * np.uint64(0x9E3779B97F4A7C15))
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        z = z ^ (z >> np.uint64(31))
+    return z
+
+
+def uniform(seed, n, stream=0, lo=0.0, hi=1.0):
+    u = (splitmix64(seed, n, stream) >> np.uint64(11)).astype(np.float64) * (2.0 ** -53)
+    return lo + (hi - lo) * u
+
+
+def normal(seed, n, stream=0, sigma=1.0):
+    m = (n + 1) // 2
+    u1 = 1.0 - uniform(seed, m, 2 * stream + 1)  # (0, 1]
+    u2 = uniform(seed, m, 2 * stream + 2)
+    r = np.sqrt(-2.0 * np.log(u1))
+    z = np.concatenate([r * np.cos(2 * np.pi * u2), r * np.sin(2 * np.pi * u2)])[:n]
+    return sigma * z
+
+
+def seg_bounds(segs):
+    b = [0]
+    for s in segs:
+        b.append(b[-1] + s)
+    return b
+
+
+class BlockedArray:
+    """A dense array cut into blocks by per-index segment lists.
+
+    `data` is the full array (Fortran order).  Blocks are packed contiguously
+    (each one column-major) in `host`, in lexicographic block order with the
+    first index fastest; `offset[blk]` is the element offset of a block.
+    """
+
+    def __init__(self, name, segs, data):
+        self.name = name
+        self.segs = [list(s) for s in segs]
+        self.bounds = [seg_bounds(s) for s in self.segs]
+        self.data = data
+        self.offset = {}
+        parts = []
+        off = 0
+        for blk in self.block_ids():
+            sl = tuple(slice(self.bounds[d][blk[d]], self.bounds[d][blk[d] + 1])
+                       for d in range(len(blk)))
+            piece = np.asfortranarray(data[sl]).ravel(order="F")
+            self.offset[blk] = off
+            parts.append(piece)
+            off += piece.size
+        self.host = np.concatenate(parts) if parts else np.zeros(0)
+        self.dev = None
+
+    def block_ids(self):
+        ranges = [range(len(s)) for s in self.segs]
+        # first index fastest, as SIP block ids are enumerated
+        for t in itertools.product(*reversed(ranges)):
+            yield tuple(reversed(t))
+
+    def dims(self, blk):
+        return [self.segs[d][blk[d]] for d in range(len(blk))]
+
+    def block(self, blk, host=None):
+        h = self.host if host is None else host
+        n = int(np.prod(self.dims(blk)))
+        return h[self.offset[blk]:self.offset[blk] + n]
+
+    def to_device(self):
+        self.dev = sg.DeviceArray.from_host(self.host)
+        return self.dev
+
+    def addr(self, blk):
+        return self.dev.at(self.offset[blk])
+
+
+def contraction_entries(dest, left, right, dlab, llab, rlab, accumulate=True):
+    """All block pairs of D(dlab) += sum L(llab) R(rlab), SIP pardo order.
+
+    Returns a list of (dblk, lblk, rblk) tuples plus the contraction pattern
+    (computed by the library's get_contraction_ptrn).
+    """
+    labels = list(dict.fromkeys(list(dlab) + list(llab) + list(rlab)))
+    code = {lab: 1 + i for i, lab in enumerate(labels)}
+    aces = [code[x] for x in list(dlab) + list(llab) + list(rlab)]
+    ptrn, ierr = sg.get_contraction_ptrn(len(dlab), len(llab), len(rlab), aces)
+    if ierr:
+        raise ValueError("bad contraction labels (ierr %d)" % ierr)
+    nseg = {}
+    for arr, labs in ((dest, dlab), (left, llab), (right, rlab)):
+        for d, lab in enumerate(labs):
+            nseg[lab] = len(arr.segs[d])
+    summed = [lab for lab in labels if lab not in dlab]
+    out = []
+    for dblk in dest.block_ids():
+        idx = dict(zip(dlab, dblk))
+        for combo in itertools.product(*[range(nseg[s]) for s in reversed(summed)]):
+            idx.update(zip(reversed(summed), combo))
+            lblk = tuple(idx[x] for x in llab)
+            rblk = tuple(idx[x] for x in rlab)
+            out.append((dblk, lblk, rblk))
+    return out, [int(x) for x in ptrn]
+
+
+def ring(o_segs=(15, 15), v_segs=(30, 30, 30, 30, 15, 15), seed=2):
+    """R(a,i,b,j) += sum_{k,c} W(k,b,c,j) T(a,i,c,k); W ~ U(-.5,.5), T ~ N(0,1e-2^2)."""
+    o, v = sum(o_segs), sum(v_segs)
+    W = uniform(seed, o * v * v * o, stream=1, lo=-0.5, hi=0.5).reshape((o, v, v, o), order="F")
+    T = normal(seed, v * o * v * o, stream=2, sigma=1e-2).reshape((v, o, v, o), order="F")
+    R0 = np.zeros((v, o, v, o), order="F")
+    Wb = BlockedArray("W", [o_segs, v_segs, v_segs, o_segs], W)
+    Tb = BlockedArray("T", [v_segs, o_segs, v_segs, o_segs], T)
+    Rb = BlockedArray("R", [v_segs, o_segs, v_segs, o_segs], R0)
+    entries, ptrn = contraction_entries(Rb, Wb, Tb, "aibj", "kbcj", "aick")
+    return dict(name="ring", D=Rb, L=Wb, R=Tb, entries=entries, ptrn=ptrn,
+                labels=("aibj", "kbcj", "aick"))
+
+
+def ladder(o_segs=(10,), v_segs=(10, 10, 10, 10), seed=1):
+    """R(a,b,i,j) += sum_{c,d} V(a,b,c,d) T(c,d,i,j); V symmetrised, V(ab,cd) = V(cd,ab)."""
+    o, v = sum(o_segs), sum(v_segs)
+    X = normal(seed, v ** 4, stream=1, sigma=0.1).reshape((v, v, v, v), order="F")
+    V = 0.5 * (X + X.transpose(2, 3, 0, 1))
+    T = normal(seed, v * v * o * o, stream=2, sigma=1e-2).reshape((v, v, o, o), order="F")
+    R0 = np.zeros((v, v, o, o), order="F")
+    Vb = BlockedArray("V", [v_segs] * 4, np.asfortranarray(V))
+    Tb = BlockedArray("T", [v_segs, v_segs, o_segs, o_segs], T)
+    Rb = BlockedArray("R", [v_segs, v_segs, o_segs, o_segs], R0)
+    entries, ptrn = contraction_entries(Rb, Vb, Tb, "abij", "abcd", "cdij")
+    return dict(name="ladder", D=Rb, L=Vb, R=Tb, entries=entries, ptrn=ptrn,
+                labels=("abij", "abcd", "cdij"))
+
+
+def batch_for(w, accumulate=True):
+    """A planned device batch for workload `w` (arrays uploaded on demand)."""
+    for k in ("D", "L", "R"):
+        if w[k].dev is None:
+            w[k].to_device()
+    D, L, R, ptrn = w["D"], w["L"], w["R"], w["ptrn"]
+    cs = [sg.make_contraction(ptrn, L.addr(lb), L.dims(lb), R.addr(rb), R.dims(rb), D.addr(db),
+                              D.dims(db), accumulate=accumulate)
+          for db, lb, rb in w["entries"]]
+    return sg.Batch(cs)
+
+
+def flops(w):
+    """Sum of 2*M*N*K over the workload's block pairs."""
+    return sum(2.0 * m * n * k for m, n, k in mnk(w))
+
+
+def mnk(w):
+    """(M, N, K) of every block pair."""
+    dl, ll, rl = w["labels"]
+    out = []
+    for db, lb, rb in w["entries"]:
+        ld = dict(zip(ll, w["L"].dims(lb)))
+        rd = dict(zip(rl, w["R"].dims(rb)))
+        M = int(np.prod([ld[x] for x in ll if x in dl] or [1]))
+        N = int(np.prod([rd[x] for x in rl if x in dl] or [1]))
+        K = int(np.prod([rd[x] for x in rl if x not in dl] or [1]))
+        out.append((M, N, K))
+    return out

@@ -1,0 +1,214 @@
+"""Benchmark: FP64-equivalent TFLOP/s of the SIP block-contraction hot path.
+
+Workload (one step): the ring term R(a,i,b,j) += sum_{k,c} W(k,b,c,j) T(a,i,c,k)
+with o = 30 (segments 15,15) and v = 150 (segments 30,30,30,30,15,15): 144
+result blocks, 1728 uneven block pairs, 1.8225e11 flop, all operands and
+results resident in HBM.  A step is one planned batch: one grouped FP64-MFMA
+launch with the index permutations fused into operand loads and the result
+scatter.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W]
+
+Prints one JSON line.  Besides the throughput it reports the kernel's
+roofline position (average launch duration from HIP events on the launch
+stream), per-contraction latency of single unbatched contractions, and the
+reference's CPU path timed on a bounded sample of the same block pairs.
+"""
+import argparse
+import json
+import os
+import subprocess
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+from aces4_amd import sipgpu as sg  # noqa: E402
+from aces4_amd import workloads as wl  # noqa: E402
+
+FP64_PEAK_TFLOPS = 78.6  # MI355X dense FP64 matrix peak
+PROFILE_TRAFFIC = os.path.join(ROOT, "profiles", "ring_gemm_traffic.txt")
+
+
+def parse():
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=50)
+    p.add_argument("--warmup", type=int, default=5)
+    p.add_argument("--cpu-seconds", type=float, default=12.0,
+                   help="budget for the CPU baseline sample")
+    p.add_argument("--no-cpu", action="store_true")
+    p.add_argument("--latency-samples", type=int, default=200)
+    return p.parse_args()
+
+
+def cpu_info():
+    try:
+        model = [l.split(":", 1)[1].strip() for l in open("/proc/cpuinfo")
+                 if l.startswith("model name")][0]
+    except Exception:
+        model = "unknown"
+    return os.cpu_count(), model
+
+
+def cpu_baseline(w, budget_s):
+    """The reference's own kernels (oracle/_ref) when present, else the C
+    restatement: per block pair TMP = W*T (tensor_block_contract, 8 OpenMP
+    threads as MAX_OMP_THREADS) then R += TMP (tensor_block_add), over a
+    seeded random sample of the workload's block pairs until `budget_s`."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from checker import Oracle, RefLib
+    ref = RefLib() if RefLib.available() else None
+    kind = "reference" if ref else "port"
+    o = Oracle()
+    D, L, R, ptrn = w["D"], w["L"], w["R"], w["ptrn"]
+    rng = np.random.default_rng(4)
+    order = rng.permutation(len(w["entries"]))
+    mnk = wl.mnk(w)
+    out = D.host.copy()
+    flops = 0.0
+    npairs = 0
+    t0 = time.perf_counter()
+    for idx in order:
+        db, lb, rb = w["entries"][idx]
+        if ref is not None:
+            tmp, e = ref.contract(ptrn, L.block(lb), L.dims(lb), R.block(rb), R.dims(rb),
+                                  D.dims(db), nthreads=8)
+            blk = out[D.offset[db]:D.offset[db] + tmp.size]
+            ref.add(D.dims(db), blk, tmp)
+        else:
+            tmp, e = o.contract(ptrn, L.block(lb), L.dims(lb), R.block(rb), R.dims(rb),
+                                D.dims(db))
+            out[D.offset[db]:D.offset[db] + tmp.size] += tmp
+        m, n, k = mnk[idx]
+        flops += 2.0 * m * n * k
+        npairs += 1
+        if time.perf_counter() - t0 > budget_s:
+            break
+    dt = time.perf_counter() - t0
+    return dict(value=flops / dt / 1e12, unit="TFLOP/s", cores=8 if ref else 1, kind=kind,
+                sample="%d of %d block pairs (seeded random), %.2e flop in %.2f s; "
+                       "TMP = W*T then R += TMP per pair; %s" % (
+                           npairs, len(w["entries"]), flops, dt,
+                           "reference tensor_dil_omp.F90 built with -DNO_BLAS (its own GEMM), "
+                           "nthreads=8" if ref else "C restatement, 1 thread"))
+
+
+def traffic_from_profile():
+    """HBM bytes per launch measured by rocprofv3 --pmc (FETCH_SIZE +
+    WRITE_SIZE, scaled by the copy-kernel calibration), if recorded."""
+    try:
+        for line in open(PROFILE_TRAFFIC):
+            if line.startswith("hbm_bytes_per_launch:"):
+                return float(line.split(":", 1)[1])
+    except OSError:
+        pass
+    return None
+
+
+def latency_sample(w, n):
+    """Per-contraction latency (enqueue -> complete) of single unbatched
+    contractions of the workload's block pairs, one at a time."""
+    D, L, R, ptrn = w["D"], w["L"], w["R"], w["ptrn"]
+    rng = np.random.default_rng(9)
+    scratch = sg.DeviceArray(max(int(np.prod(D.dims(db))) for db, _, _ in w["entries"]))
+    lat = []
+    for idx in rng.integers(0, len(w["entries"]), size=n):
+        db, lb, rb = w["entries"][idx]
+        t0 = time.perf_counter()
+        sg.contract(ptrn, L.addr(lb), L.dims(lb), R.addr(rb), R.dims(rb), scratch.ptr,
+                    D.dims(db))
+        sg.synchronize()
+        lat.append((time.perf_counter() - t0) * 1e3)
+    scratch.free()
+    lat = np.array(lat)
+    return dict(p50_ms=float(np.percentile(lat, 50)), p99_ms=float(np.percentile(lat, 99)),
+                samples=int(n), mode="single contraction, unbatched, host enqueue to completion")
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print("bench.py measures one GPU: this build runs the contraction path on a single "
+              "MI355X (multi-GPU sharding is a design note only).", file=sys.stderr)
+        sys.exit(2)
+    sg.init(0)
+    w = wl.ring()
+    batch = wl.batch_for(w)
+    flops = batch.flops
+    stream = None
+    for _ in range(a.warmup):
+        batch.execute(stream)
+    sg.synchronize()
+
+    evs = [sg.Event() for _ in range(a.steps + 1)]
+    sg.synchronize()
+    t0 = time.perf_counter()
+    evs[0].record(stream)
+    for i in range(a.steps):
+        batch.execute(stream)
+        evs[i + 1].record(stream)
+    sg.synchronize()
+    wall = time.perf_counter() - t0
+    launch_ms = np.array([evs[i].elapsed_ms(evs[i + 1]) for i in range(a.steps)])
+    avg_launch_s = float(launch_ms.mean()) / 1e3
+    achieved = flops / avg_launch_s / 1e12
+
+    # quick parity check of one result block against the oracle
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from checker import Oracle
+    from cases import rel_fro
+    o = Oracle()
+    D, L, R, ptrn = w["D"], w["L"], w["R"], w["ptrn"]
+    got = D.dev.download()
+    db0 = next(iter(D.block_ids()))
+    want = np.zeros(int(np.prod(D.dims(db0))))
+    for db, lb, rb in w["entries"]:
+        if db == db0:
+            tmp, _ = o.contract(ptrn, L.block(lb), L.dims(lb), R.block(rb), R.dims(rb),
+                                D.dims(db))
+            want += tmp
+    steps_run = a.warmup + a.steps
+    check = rel_fro(D.block(db0, got), steps_run * want)
+
+    lat = latency_sample(w, a.latency_samples)
+    cores, model = cpu_info()
+    cpu = None if a.no_cpu else cpu_baseline(w, a.cpu_seconds)
+    if cpu:
+        cpu["host"] = "%s, %d logical cores" % (model, cores)
+
+    traffic = traffic_from_profile()
+    res = {
+        "metric": "FP64-equiv TFLOP/s, split vs plain FP64; p50/p99 per-contraction latency",
+        "value": flops * a.steps / wall / 1e12,
+        "unit": "TFLOP/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": wall / a.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (SplitMix64-seeded W ~ U(-0.5,0.5), T ~ N(0,1e-2^2))",
+        "config": {"workload": "ring R(a,i,b,j) += W(k,b,c,j) T(a,i,c,k)", "o": 30, "v": 150,
+                   "o_segments": [15, 15], "v_segments": [30, 30, 30, 30, 15, 15],
+                   "block_pairs": batch.npairs, "result_blocks": batch.ntasks,
+                   "flop_per_step": flops, "path": "plain FP64 MFMA, grouped batch"},
+        "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
+                     "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS,
+                     "traffic": traffic,
+                     "kernel": "gemm_f64_kernel", "avg_launch_ms": avg_launch_s * 1e3},
+        "split": None,
+        "latency": lat,
+        "cpu_baseline": cpu,
+        "check_rel_err": check,
+    }
+    print(json.dumps(res))
+
+
+if __name__ == "__main__":
+    main()

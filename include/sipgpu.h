@@ -1,0 +1,180 @@
+/*
+ * sipgpu.h -- C ABI of libsipgpu.so, the MI355X (gfx950) implementation of the
+ * SIP block-contraction hot path: the `contract` and `permute`
+ * super-instructions that the SIAL interpreter issues on segmented blocks.
+ *
+ * Two layers share one library:
+ *
+ * (1) Drop-in layer.  The exact symbols, argument meaning, assign semantics
+ *     and error codes of the reference's tensor kernel module, so a SIP build
+ *     links libsipgpu.so in place of its tensor_dil_omp library.  All scalars
+ *     are passed by pointer (binary-identical to the reference's C++
+ *     by-reference prototypes), `int` is 32-bit, blocks are dense FP64
+ *     column-major host arrays, and every call is synchronous.
+ *
+ * (2) Native layer (`sipgpu_*`).  Device-resident blocks, HIP streams,
+ *     planned batches of contractions executed as one grouped launch, and the
+ *     device block cache.  All functions return 0 on success or a negative
+ *     SIPGPU_E* code; sipgpu_last_error() gives the message.
+ *
+ * Conventions shared with the reference:
+ *   - a permutation `dim_transp[0..rank]` has dim_transp[0] = sign (ignored)
+ *     and dim_transp[i] = 1-based new position of old dimension i;
+ *   - a contraction pattern `ptrn[0..lrank+rrank-1]` holds +k (uncontracted,
+ *     becomes destination dimension k) or -k (contracted with dimension k of
+ *     the other operand), 1-based, as produced by get_contraction_ptrn_.
+ */
+#ifndef SIPGPU_H
+#define SIPGPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------------ */
+/* Drop-in layer                                                             */
+/* ------------------------------------------------------------------------ */
+
+/* Replaces tensor_size_by_shape_ (tensor_dil_omp.F90:66-82; prototype
+ * tensor_ops_c_prototypes.h:41).  ierr: 1 negative rank, 2 non-positive size. */
+long long tensor_size_by_shape_(int *num_dim, int *dims, int *ierr);
+
+/* Replaces get_contraction_ptrn_ (tensor_dil_omp.F90:84-142; prototype
+ * tensor_ops_c_prototypes.h:55).  aces_ptrn holds drank+lrank+rrank integer
+ * labels (destination, left, right), each label exactly twice.  ierr codes
+ * 1..6 as in the reference. */
+void get_contraction_ptrn_(int *drank, int *lrank, int *rrank, int *aces_ptrn,
+                           int *my_ptrn, int *ierr);
+
+/* Replaces tensor_block_copy__ (tensor_dil_omp.F90:438-660; prototype
+ * tensor_ops_c_prototypes.h:152), the body of the permute super-instruction
+ * (Block::transpose_copy, block.cpp:216-255).  `dims` are the SOURCE
+ * extents.  ierr = rank for a negative rank; SIPGPU_E* for an invalid
+ * permutation or a device failure.  nthreads is accepted and ignored. */
+void tensor_block_copy__(int *nthreads, int *rank, int *dims, int *dim_transp,
+                         double *in, double *out, int *ierr);
+
+/* Replaces tensor_block_contract__ (tensor_dil_omp.F90:662-906; prototype
+ * tensor_ops_c_prototypes.h:178), called by Interpreter::handle_contraction
+ * (interpreter.cpp:1210-1262).  D is overwritten (assign semantics,
+ * tensor_dil_omp.F90:752).  ierr: 1 bad pattern, -1 bad ranks, SIPGPU_E* for
+ * device failures.  nthreads is accepted and ignored. */
+void tensor_block_contract__(int *nthreads, int *ptrn, double *l, int *lrank,
+                             int *ldims, double *r, int *rrank, int *rdims,
+                             double *d, int *drank, int *ddims, int *ierr);
+
+/* ------------------------------------------------------------------------ */
+/* Native layer                                                              */
+/* ------------------------------------------------------------------------ */
+
+#define SIPGPU_MAX_RANK 6
+
+enum {
+    SIPGPU_OK = 0,
+    SIPGPU_EPATTERN = -101,   /* contraction pattern rejected */
+    SIPGPU_ERANK = -102,      /* rank outside 0..SIPGPU_MAX_RANK */
+    SIPGPU_EPERM = -103,      /* dim_transp is not a permutation */
+    SIPGPU_ESIZE = -104,      /* block too large (>= 2^31 elements) */
+    SIPGPU_EDEVICE = -105,    /* HIP runtime error */
+    SIPGPU_ENOMEM = -106,     /* device or host allocation failed */
+    SIPGPU_EARG = -107,       /* invalid argument (null pointer, bad handle) */
+    SIPGPU_EHAZARD = -108,    /* batch writes a block it also reads */
+    SIPGPU_EBUDGET = -109,    /* block cache budget cannot hold the request */
+    SIPGPU_ENOTFOUND = -110   /* block id not present in the cache */
+};
+
+typedef struct sipgpu_stream_s *sipgpu_stream_t;   /* wraps a hipStream_t */
+typedef struct sipgpu_event_s *sipgpu_event_t;     /* wraps a hipEvent_t */
+typedef struct sipgpu_batch_s *sipgpu_batch_t;     /* a planned batch */
+
+const char *sipgpu_last_error(void);
+const char *sipgpu_version(void);
+
+/* Selects the device (default 0) and creates the library's own stream. */
+int sipgpu_init(int device);
+int sipgpu_device_count(int *count);
+int sipgpu_device_synchronize(void);
+
+/* Device memory (plain HBM allocations, no caching). */
+int sipgpu_malloc(void **dptr, size_t bytes);
+int sipgpu_free(void *dptr);
+int sipgpu_memcpy_h2d(void *dst, const void *src, size_t bytes);
+int sipgpu_memcpy_d2h(void *dst, const void *src, size_t bytes);
+int sipgpu_memcpy_d2d(void *dst, const void *src, size_t bytes);
+int sipgpu_memset(void *dptr, int value, size_t bytes);
+
+/* Streams (default priority only) and events. A NULL stream means the
+ * library's own stream. */
+int sipgpu_stream_create(sipgpu_stream_t *s);
+int sipgpu_stream_destroy(sipgpu_stream_t s);
+int sipgpu_stream_synchronize(sipgpu_stream_t s);
+int sipgpu_event_create(sipgpu_event_t *e);
+int sipgpu_event_destroy(sipgpu_event_t e);
+int sipgpu_event_record(sipgpu_event_t e, sipgpu_stream_t s);
+int sipgpu_event_synchronize(sipgpu_event_t e);
+int sipgpu_event_elapsed_ms(sipgpu_event_t start, sipgpu_event_t stop, float *ms);
+
+/* Permute a device block, asynchronously on `s`.  Same meaning as
+ * tensor_block_copy__; `in` and `out` must not overlap. */
+int sipgpu_permute(sipgpu_stream_t s, int rank, const int *dims,
+                   const int *dim_transp, const double *in, double *out);
+
+/* Contract device blocks, asynchronously on `s`.  accumulate = 0 gives the
+ * reference's assign semantics (D = L*R); accumulate = 1 gives D += L*R
+ * (the block_contract_accumulate_op semantics of opcode.h:117, which the
+ * reference interpreter leaves unimplemented). */
+int sipgpu_contract(sipgpu_stream_t s, const int *ptrn, const double *l,
+                    int lrank, const int *ldims, const double *r, int rrank,
+                    const int *rdims, double *d, int drank, const int *ddims,
+                    int accumulate);
+
+/* One contraction of a batch.  Entries that name the same destination block
+ * (same pointer, extents and pattern destination order) form one task: their
+ * products are summed in list order inside one accumulator and written once;
+ * if the first such entry has accumulate = 0 the block is overwritten,
+ * otherwise the sum is added to it. */
+typedef struct {
+    const double *l;
+    const double *r;
+    double *d;
+    int lrank, rrank, drank;
+    int ldims[SIPGPU_MAX_RANK];
+    int rdims[SIPGPU_MAX_RANK];
+    int ddims[SIPGPU_MAX_RANK];
+    int ptrn[2 * SIPGPU_MAX_RANK];
+    int accumulate;
+} sipgpu_contraction;
+
+/* Plans a batch: validates every entry, builds the index-offset tables
+ * (permutations fused into the operand loads and the result scatter),
+ * groups entries into tasks and tiles, and uploads the work list.  The plan
+ * owns only its work list; blocks stay owned by the caller. */
+int sipgpu_batch_plan(const sipgpu_contraction *list, int n, sipgpu_batch_t *batch);
+/* Runs a planned batch as one grouped FP64-MFMA launch on `s`. */
+int sipgpu_batch_execute(sipgpu_batch_t batch, sipgpu_stream_t s);
+int sipgpu_batch_destroy(sipgpu_batch_t batch);
+/* Sum of 2*M*N*K over the batch, and its task / tile / pair counts. */
+int sipgpu_batch_info(sipgpu_batch_t batch, double *flops, int *ntasks,
+                      int *ntiles, int *npairs);
+
+/* Host-only introspection (no device work).  For one contraction shape:
+ * mnk[0..2] = M, N, K of the product; o2n receives lo2n[1..lrank],
+ * ro2n[1..rrank], do2n[1..drank] (the reference's determine_index_permutations
+ * maps, 1-based, concatenated); the six offset tables are written when their
+ * pointers are non-NULL (sizes K, M, K, N, M, N).  Returns 0 or
+ * SIPGPU_EPATTERN / SIPGPU_ERANK / SIPGPU_ESIZE. */
+int sipgpu_plan_tables(const int *ptrn, int lrank, const int *ldims, int rrank,
+                       const int *rdims, int drank, const int *ddims,
+                       long long *mnk, int *o2n, int *koffL, int *moffL,
+                       int *koffR, int *noffR, int *moffD, int *noffD);
+/* Host-only introspection of a permute plan: info[0] canonical rank,
+ * info[1] leading tile dims, info[2] row mode, info[3] plain copy. */
+int sipgpu_plan_permute(int rank, const int *dims, const int *dim_transp, int *info);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

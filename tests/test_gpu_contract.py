@@ -1,0 +1,137 @@
+"""GPU parity of the contract super-instruction.
+
+Tolerance: relative Frobenius error <= 1e-13 against the reference's golden
+vectors and the oracle (FP64 summation order differs); exact on the
+integer-valued known answers of the reference's test suite; ierr codes and
+assign semantics identical to the reference.
+"""
+import numpy as np
+import pytest
+
+import golden_io as g
+from cases import dims_of, random_case, rel_fro
+from checker import Oracle
+
+from aces4_amd import sipgpu as sg
+from aces4_amd.workloads import normal, uniform
+
+pytestmark = pytest.mark.gpu
+o = Oracle()
+TOL = 1e-13
+
+
+@pytest.mark.parametrize("key", g.keys("con"))
+def test_dropin_contract_golden(key):
+    c = g.contract_case(key)
+    print("contract", key, c["ldims"], c["rdims"], c["ddims"], flush=True)
+    d, ierr = sg.tensor_block_contract(c["ptrn"], c["l"], c["ldims"], c["r"], c["rdims"],
+                                       c["ddims"], d_init=c["d0"])
+    assert ierr == 0
+    assert rel_fro(d, c["d"]) <= TOL
+    if not c["ldims"] or not c["rdims"]:
+        # tensor x scalar and scalar x scalar are elementwise: bit-exact
+        assert np.array_equal(d, c["d"])
+
+
+def test_dropin_contract_error_codes():
+    for err in g.manifest()["contract_errors"]:
+        n = lambda dims: int(np.prod(dims))
+        d0 = np.zeros(n(err["ddims"]))
+        _, ierr = sg.tensor_block_contract(err["ptrn"], np.ones(n(err["ldims"])), err["ldims"],
+                                           np.ones(n(err["rdims"])), err["rdims"], err["ddims"],
+                                           d_init=d0)
+        assert ierr == err["ierr"]
+
+
+def test_kat_contraction_small_tests_exact():
+    a = g.cyclic(15 ** 4, 1)
+    b = g.cyclic(225, 1)
+    A = a.reshape((15,) * 4, order="F").astype(np.int64)
+    B = b.reshape((15, 15), order="F").astype(np.int64)
+    want = np.einsum("ijkl,jk->il", A, B).ravel(order="F").astype(np.float64)
+    d, ierr = sg.tensor_block_contract([1, -1, -2, 2, -2, -3], a, [15] * 4, b, [15, 15], [15, 15])
+    assert ierr == 0 and np.array_equal(d, want)
+    a = g.cyclic(9 * 5 * 5 * 9, 1)
+    b = g.cyclic(36, 1)
+    want = np.einsum("mpil,la->mpai", a.reshape((9, 5, 5, 9), order="F").astype(np.int64),
+                     b.reshape((9, 4), order="F").astype(np.int64)).ravel(order="F")
+    ptrn, _ = sg.get_contraction_ptrn(4, 2, 4, [1, 2, 3, 4, 5, 3, 1, 2, 4, 5])
+    d, ierr = sg.tensor_block_contract(ptrn, b, [9, 4], a, [9, 5, 5, 9], [9, 5, 4, 5])
+    assert ierr == 0 and np.array_equal(d, want.astype(np.float64))
+    x, ierr = sg.tensor_block_contract([-1, -2, -1, -2], g.cyclic(64, 1), [8, 8], g.cyclic(64, 5),
+                                       [8, 8], [])
+    assert ierr == 0
+    assert x[0] == float(np.dot(g.cyclic(64, 1).astype(np.int64), g.cyclic(64, 5).astype(np.int64)))
+
+
+def test_assign_semantics():
+    rng = np.random.default_rng(3)
+    L, R = rng.standard_normal(9 * 13), rng.standard_normal(13 * 11)
+    d7, ierr = sg.tensor_block_contract([1, -1, -2, 2], L, [9, 13], R, [13, 11], [9, 11],
+                                        d_init=np.full(99, 7.0))
+    assert ierr == 0
+    want = (L.reshape(9, 13, order="F") @ R.reshape(13, 11, order="F")).ravel(order="F")
+    assert rel_fro(d7, want) < TOL
+
+
+def test_dropin_random_vs_oracle():
+    rng = np.random.default_rng(41)
+    for i in range(300):
+        dl, ll, rl, ext = random_case(rng, max_ext=9, allow_scalar=True)
+        ptrn, _ = sg.get_contraction_ptrn(len(dl), len(ll), len(rl), dl + ll + rl)
+        ld, rd, dd = dims_of(ll, ext), dims_of(rl, ext), dims_of(dl, ext)
+        print("contract", i, ld, rd, dd, list(ptrn), flush=True)
+        L = rng.standard_normal(int(np.prod(ld)))
+        R = rng.standard_normal(int(np.prod(rd)))
+        want, e = o.contract(ptrn, L, ld, R, rd, dd)
+        got, ierr = sg.tensor_block_contract(ptrn, L, ld, R, rd, dd)
+        assert e == 0 and ierr == 0
+        assert rel_fro(got, want) <= TOL, (ld, rd, dd, list(ptrn))
+
+
+LABELED = [
+    # (dest, left, right, extents): benchmark terms at their real block shapes
+    ("abij", "abcd", "cdij", dict(a=10, b=10, c=10, d=10, i=10, j=10)),      # ladder 10^4
+    ("aibj", "kbcj", "aick", dict(a=30, i=15, b=30, j=15, k=15, c=30)),      # ring 450^3
+    ("aibj", "kbcj", "aick", dict(a=15, i=15, b=30, j=15, k=15, c=15)),      # ring, uneven
+    ("abij", "abcd", "cdij", dict(a=40, b=40, c=40, d=40, i=10, j=10)),      # 1600x100x1600
+    ("abcijk", "bcdk", "adij", dict(a=12, b=12, c=12, i=10, j=10, k=10, d=12)),  # triples
+    ("ij", "ik", "kj", dict(i=1, j=1000, k=37)),                             # M = 1
+    ("ij", "ik", "kj", dict(i=999, j=3, k=1)),                               # K = 1
+    ("aibj", "bjck", "ckai", dict(a=17, i=9, b=23, j=5, c=31, k=3)),         # odd extents
+]
+
+
+@pytest.mark.parametrize("spec", LABELED, ids=lambda s: "%s=%s*%s" % s[:3])
+def test_native_contract_shapes(spec):
+    dl, ll, rl, ext = spec
+    code = {lab: 10 + i for i, lab in enumerate(dict.fromkeys(dl + ll + rl))}
+    e2 = {code[k]: v for k, v in ext.items()}
+    dlab, llab, rlab = [code[x] for x in dl], [code[x] for x in ll], [code[x] for x in rl]
+    ptrn, _ = sg.get_contraction_ptrn(len(dl), len(ll), len(rl), dlab + llab + rlab)
+    ld, rd, dd = dims_of(llab, e2), dims_of(rlab, e2), dims_of(dlab, e2)
+    print("contract", spec, flush=True)
+    L = uniform(5, int(np.prod(ld)), lo=-0.5, hi=0.5)
+    R = normal(6, int(np.prod(rd)), sigma=1e-2)
+    D0 = normal(7, int(np.prod(dd)))
+    want, e = o.contract(ptrn, L, ld, R, rd, dd)
+    assert e == 0
+    dL, dR = sg.DeviceArray.from_host(L), sg.DeviceArray.from_host(R)
+    dD = sg.DeviceArray.from_host(D0)
+    sg.contract(ptrn, dL.ptr, ld, dR.ptr, rd, dD.ptr, dd)
+    assert rel_fro(dD.download(), want) <= TOL
+    # accumulate: D += L*R on top of the assigned result
+    sg.contract(ptrn, dL.ptr, ld, dR.ptr, rd, dD.ptr, dd, accumulate=True)
+    assert rel_fro(dD.download(), 2.0 * want) <= TOL
+    for a in (dL, dR, dD):
+        a.free()
+
+
+def test_native_contract_errors():
+    dL, dR, dD = sg.DeviceArray(12), sg.DeviceArray(12), sg.DeviceArray(12)
+    with pytest.raises(sg.SipGpuError) as ei:
+        sg.contract([1, 1], dL.ptr, [3], dR.ptr, [3], dD.ptr, [3])
+    assert ei.value.code == -101
+    with pytest.raises(sg.SipGpuError) as ei:
+        sg.contract([1] * 7 + [-1] * 7, dL.ptr, [1] * 7, dR.ptr, [1] * 7, dD.ptr, [1] * 7)
+    assert ei.value.code == -102

@@ -1,0 +1,148 @@
+"""The C-ABI library on the CPU: it loads, exports every declared symbol, and
+its host-side logic (pattern conversion, index maps, offset tables) matches
+the reference.  No device work happens here."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+import golden_io as g
+from cases import dims_of, random_case, rel_fro
+from checker import Oracle
+
+from aces4_amd import sipgpu as sg
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+o = Oracle()
+
+
+def declared_functions():
+    text = open(os.path.join(ROOT, "include", "sipgpu.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    names = re.findall(r"^\s*(?:const\s+)?[A-Za-z_][A-Za-z0-9_ ]*?[\s\*]+([A-Za-z_][A-Za-z0-9_]*)"
+                       r"\s*\(", text, flags=re.M)
+    return sorted(set(n for n in names if n not in ("if", "while", "sizeof")))
+
+
+def test_library_exports_every_declared_symbol():
+    names = declared_functions()
+    assert "tensor_block_contract__" in names and "sipgpu_batch_plan" in names
+    L = sg.lib()
+    missing = [n for n in names if not hasattr(L, n)]
+    assert not missing, missing
+    assert set(names) == set(sg.EXPORTED)
+
+
+def test_library_is_gfx950_build():
+    assert b"gfx950" in sg.lib().sipgpu_version()
+    blob = open(sg.LIB_PATH, "rb").read()
+    assert b"gfx950" in blob
+
+
+@pytest.mark.parametrize("key", g.keys("pat"))
+def test_pattern_matches_reference(key):
+    dr, lr, rr, aces, ptrn, ierr = g.pattern_case(key)
+    if dr < 0:
+        out = np.zeros(max(lr + rr, 1), np.int32)
+        e = ctypes.c_int(0)
+        a = np.array(aces, np.int32)
+        sg.lib().get_contraction_ptrn_(ctypes.byref(ctypes.c_int(dr)),
+                                       ctypes.byref(ctypes.c_int(lr)),
+                                       ctypes.byref(ctypes.c_int(rr)),
+                                       a.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                                       out.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                                       ctypes.byref(e))
+        assert e.value == ierr
+        return
+    mine, e = sg.get_contraction_ptrn(dr, lr, rr, aces)
+    assert e == ierr
+    if ierr == 0:
+        assert np.array_equal(mine, ptrn)
+
+
+def test_size_by_shape():
+    assert sg.tensor_size_by_shape([3, 4, 5]) == (60, 0)
+    assert sg.tensor_size_by_shape([3, 0]) == (0, 2)
+    assert sg.tensor_size_by_shape([]) == (1, 0)
+
+
+def _table_contract(tabs, mnk, L, R):
+    """D from the offset tables alone: D[moffD[m]+noffD[n]] = sum_k L[..] R[..]."""
+    M, N, K = mnk
+    A = L[tabs["koffL"][:K, None] + tabs["moffL"][None, :M]]   # K x M
+    B = R[tabs["koffR"][:K, None] + tabs["noffR"][None, :N]]   # K x N
+    P = A.T @ B
+    D = np.empty(M * N)
+    D[(tabs["moffD"][:M, None] + tabs["noffD"][None, :N]).ravel()] = P.ravel()
+    return D
+
+
+def test_index_maps_and_tables_match_oracle():
+    rng = np.random.default_rng(21)
+    for _ in range(300):
+        dl, ll, rl, ext = random_case(rng, allow_scalar=False)
+        ptrn, _ = o.get_contraction_ptrn(len(dl), len(ll), len(rl), dl + ll + rl)
+        ld, rd, dd = dims_of(ll, ext), dims_of(rl, ext), dims_of(dl, ext)
+        mnk, maps, tabs = sg.plan_tables(ptrn, ld, rd, dd)
+        ref = o.perms(ptrn, ld, rd, dd)
+        # the reference's O2N maps and matrix sizes, bit for bit
+        assert (mnk[0], mnk[1], mnk[2]) == (ref["M"], ref["N"], ref["K"])
+        assert [int(x) for x in maps["lo2n"]] == ref["lo2n"]
+        assert [int(x) for x in maps["ro2n"]] == ref["ro2n"]
+        assert [int(x) for x in maps["do2n"]] == ref["do2n"]
+        # the tables address every element once and reproduce the contraction
+        M, N, K = mnk
+        for name, n, size in (("moffD", M, 0), ("noffD", N, 0)):
+            assert len(set(tabs[name][:n].tolist())) == n
+        dst = (tabs["moffD"][:M, None] + tabs["noffD"][None, :N]).ravel()
+        assert sorted(dst.tolist()) == list(range(M * N))
+        L = rng.standard_normal(int(np.prod(ld)))
+        R = rng.standard_normal(int(np.prod(rd)))
+        want, e = o.contract(ptrn, L, ld, R, rd, dd)
+        assert e == 0
+        assert rel_fro(_table_contract(tabs, mnk, L, R), want) < 1e-14
+
+
+@pytest.mark.parametrize("key", g.keys("con"))
+def test_tables_reproduce_golden_contractions(key):
+    c = g.contract_case(key)
+    if not (c["ldims"] and c["rdims"] and c["ddims"]):
+        pytest.skip("not a block = block * block contraction")
+    mnk, maps, tabs = sg.plan_tables(c["ptrn"], c["ldims"], c["rdims"], c["ddims"])
+    assert rel_fro(_table_contract(tabs, mnk, c["l"], c["r"]), c["d"]) < 1e-13
+
+
+def test_pattern_errors_are_reported():
+    with pytest.raises(sg.SipGpuError) as ei:
+        sg.plan_tables([1, 1], [3], [3], [3])
+    assert ei.value.code == -101
+    with pytest.raises(sg.SipGpuError) as ei:
+        sg.plan_permute([3, 4], [1, 1, 1])
+    assert ei.value.code == -103
+
+
+def test_permute_plans():
+    # identity and extent-1 dims collapse to a plain copy
+    assert sg.plan_permute([4, 5, 6], [1, 1, 2, 3])[3] == 1
+    assert sg.plan_permute([1, 7, 1], [1, 3, 1, 2])[3] == 1
+    # (3,4,1,2) on a ring block: two canonical dims, transpose mode
+    rank, nU, row, ident = sg.plan_permute([30, 15, 30, 15], [1, 3, 4, 1, 2])
+    assert (rank, row, ident) == (2, 0, 0)
+    # input-fastest dim stays fastest: row mode
+    assert sg.plan_permute([8, 3, 5], [1, 1, 3, 2])[2] == 1
+
+
+def test_device_count_without_gpu_is_not_an_error():
+    assert sg.device_count() >= 0
+
+
+def test_product_path_fails_loudly_without_gpu():
+    if sg.device_count() > 0:
+        pytest.skip("a GPU is present")
+    _, ierr = sg.tensor_block_contract([1, -1, -2, 2], np.ones(6), [2, 3], np.ones(12), [3, 4],
+                                       [2, 4])
+    assert ierr != 0
+    _, ierr = sg.tensor_block_copy([2, 3], [1, 2, 1], np.ones(6))
+    assert ierr != 0
