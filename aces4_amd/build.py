@@ -11,7 +11,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libsipgpu.so")
-SOURCES = ["plan.cpp", "permute.hip", "contract.hip", "runtime.cpp"]
+SOURCES = ["plan.cpp", "permute.hip", "contract.hip", "calib.hip", "runtime.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 FLAGS = [
@@ -23,17 +23,36 @@ FLAGS = [
 ]
 
 
-def build(verbose=False):
-    srcs = [os.path.join(CSRC, s) for s in SOURCES]
-    tmp = OUT + ".tmp"
-    cmd = [HIPCC] + FLAGS + ["-x", "hip"] + srcs + ["-o", tmp]
+# Extra flags per source.  The calibration loop keeps its MFMA accumulators
+# in the unified VGPR file so the loop is nothing but MFMAs.
+PER_FILE = {"calib.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form=1"]}
+NOISE = "is not a recognized feature for this target"
+
+
+def _run(cmd):
     r = subprocess.run(cmd, capture_output=True, text=True)
-    noise = "is not a recognized feature for this target"
-    err = "\n".join(l for l in (r.stdout + r.stderr).splitlines() if noise not in l)
+    err = "\n".join(l for l in (r.stdout + r.stderr).splitlines() if NOISE not in l)
     if r.returncode != 0:
         raise RuntimeError("hipcc failed (%d):\n%s\n%s" % (r.returncode, " ".join(cmd), err))
-    if verbose and err.strip():
-        print(err, file=sys.stderr)
+    return err
+
+
+def build(verbose=False):
+    objdir = os.path.join(HERE, "_build")
+    os.makedirs(objdir, exist_ok=True)
+    objs, msgs = [], []
+    for src in SOURCES:
+        obj = os.path.join(objdir, src + ".o")
+        cmd = ([HIPCC] + [f for f in FLAGS if f != "-shared"] + PER_FILE.get(src, []) +
+               ["-x", "hip", "-c", os.path.join(CSRC, src), "-o", obj])
+        msgs.append(_run(cmd))
+        objs.append(obj)
+    tmp = OUT + ".tmp"
+    msgs.append(_run([HIPCC, "-shared", "-fPIC", "--offload-arch=gfx950"] + objs + ["-o", tmp]))
+    if verbose:
+        text = "\n".join(m for m in msgs if m.strip())
+        if text:
+            print(text, file=sys.stderr)
     os.replace(tmp, OUT)
     return OUT
 

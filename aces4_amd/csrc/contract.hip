@@ -25,52 +25,97 @@ namespace {
 
 typedef double f64x4 __attribute__((ext_vector_type(4)));
 
-constexpr int kBK = 16; // K per LDS stage (4 MFMA k-steps)
+
+typedef const __attribute__((address_space(1))) double *gptr_t;
+typedef const __attribute__((address_space(1))) int32_t *gitab_t;
 
 // Global -> register stage of a BK x BX operand tile.  `kfast` picks the
 // lane order that keeps the global loads coalesced: consecutive lanes walk
 // k when the operand's fastest dim is contracted, x otherwise.
-template <int BX, int NT>
+//
+// Software pipeline: the k-offset lookups of stage s+2 and the data loads of
+// stage s+1 are in flight while stage s runs on the MFMA pipe.  All loads
+// are unconditional (indices clamped into the block); out-of-range values
+// are zeroed only when the stage is written to LDS, so no wait is forced
+// before the MFMA loop.
+template <int BX, int NT, int kBK>
 struct TileLoader {
     static constexpr int kItems = kBK * BX / NT;
-    static_assert(kItems >= 1 && (kBK * BX) % NT == 0, "tile/threads mismatch");
-    int xoff[kItems]; // per-pair x offsets (or -1 when out of range)
-    int kk[kItems];
-    int xx[kItems];
+    static_assert(kItems >= 1 && (kBK * BX) % NT == 0 && NT % kBK == 0 && NT % BX == 0,
+                  "tile/threads mismatch");
+    static constexpr int kXStep = NT / kBK; // kfast: x advance per item
+    static constexpr int kKStep = NT / BX;  // xfast: k advance per item
+    bool kfast;
+    int xoff[kItems]; // kfast: per item; xfast: xoff[0] only
+    unsigned xmask;   // bit i: item i's x is inside the block
+    int ko[kItems];   // k offsets of the next stage to issue (kfast: ko[0])
+    gptr_t src;
+    gitab_t ktab;
+    int K;
 
-    __device__ void setup(bool kfast, int tid) {
+    __device__ void pair(bool kf, const double *s, const int32_t *kt, int Kp,
+                         const int32_t *xtab, int x0, int X, int tid) {
+        kfast = kf;
+        src = (gptr_t)s;
+        ktab = (gitab_t)kt;
+        K = Kp;
+        gitab_t xt = (gitab_t)xtab;
+        xmask = 0;
+        if (kfast) {
 #pragma unroll
-        for (int i = 0; i < kItems; ++i) {
-            const int e = tid + i * NT;
-            if (kfast) { kk[i] = e % kBK; xx[i] = e / kBK; }
-            else       { xx[i] = e % BX;  kk[i] = e / BX; }
+            for (int i = 0; i < kItems; ++i) {
+                const int x = x0 + tid / kBK + i * kXStep;
+                xoff[i] = xt[min(x, X - 1)];
+                xmask |= (x < X ? 1u : 0u) << i;
+            }
+        } else {
+            const int x = x0 + tid % BX;
+            xoff[0] = xt[min(x, X - 1)];
+            xmask = x < X ? ~0u : 0u;
         }
     }
-    __device__ void pair(const int32_t *__restrict__ xtab, int x0, int X) {
+    // Look up the k offsets of the stage starting at k0.
+    __device__ void lookup(int k0, int tid) {
+        if (kfast) {
+            ko[0] = ktab[min(k0 + tid % kBK, K - 1)];
+        } else {
 #pragma unroll
-        for (int i = 0; i < kItems; ++i) {
-            const int x = x0 + xx[i];
-            xoff[i] = x < X ? xtab[x] : -1;
+            for (int i = 0; i < kItems; ++i) ko[i] = ktab[min(k0 + tid / BX + i * kKStep, K - 1)];
         }
     }
-    __device__ void load(double (&v)[kItems], const double *__restrict__ src,
-                         const int32_t *__restrict__ ktab, int k0, int K) const {
+    // Issue the data loads of the looked-up stage.
+    __device__ void issue(double (&v)[kItems]) const {
+        if (kfast) {
 #pragma unroll
-        for (int i = 0; i < kItems; ++i) {
-            const int k = k0 + kk[i];
-            v[i] = 0.0;
-            if (k < K && xoff[i] >= 0) v[i] = src[ktab[k] + xoff[i]];
+            for (int i = 0; i < kItems; ++i) v[i] = src[ko[0] + xoff[i]];
+        } else {
+#pragma unroll
+            for (int i = 0; i < kItems; ++i) v[i] = src[ko[i] + xoff[0]];
         }
     }
-    __device__ void store(double *S, int pitch, const double (&v)[kItems]) const {
+    // Write the stage starting at k0 to LDS, zeroing out-of-range elements.
+    __device__ void store(double *S, int pitch, const double (&v)[kItems], int k0,
+                          int tid) const {
+        if (kfast) {
+            const bool kin = k0 + tid % kBK < K;
 #pragma unroll
-        for (int i = 0; i < kItems; ++i) S[kk[i] * pitch + xx[i]] = v[i];
+            for (int i = 0; i < kItems; ++i) {
+                const bool ok = kin && ((xmask >> i) & 1u);
+                S[(tid % kBK) * pitch + tid / kBK + i * kXStep] = ok ? v[i] : 0.0;
+            }
+        } else {
+#pragma unroll
+            for (int i = 0; i < kItems; ++i) {
+                const bool ok = (k0 + tid / BX + i * kKStep < K) && xmask;
+                S[(tid / BX + i * kKStep) * pitch + tid % BX] = ok ? v[i] : 0.0;
+            }
+        }
     }
 };
 
 // BM x BN workgroup tile, WM x WN waves, each wave a (BM/WM) x (BN/WN) tile
-// of 16x16 MFMA blocks.
-template <int BM, int BN, int WM, int WN>
+// of 16x16 MFMA blocks; K staged through LDS kBK at a time.
+template <int BM, int BN, int WM, int WN, int kBK>
 __global__ __launch_bounds__(WM *WN * 64) void gemm_f64_kernel(GemmArgs args) {
     constexpr int NT = WM * WN * 64;
     constexpr int TM = BM / WM, TN = BN / WN;
@@ -86,10 +131,10 @@ __global__ __launch_bounds__(WM *WN * 64) void gemm_f64_kernel(GemmArgs args) {
     const int wm = wave % WM, wn = wave / WM;
     const int li = lane & 15, lk = lane >> 4;
 
-    TileLoader<BM, NT> la;
-    TileLoader<BN, NT> lb;
-    double va[TileLoader<BM, NT>::kItems];
-    double vb[TileLoader<BN, NT>::kItems];
+    TileLoader<BM, NT, kBK> la;
+    TileLoader<BN, NT, kBK> lb;
+    double va[TileLoader<BM, NT, kBK>::kItems];
+    double vb[TileLoader<BN, NT, kBK>::kItems];
 
     for (int t = blockIdx.x; t < args.ntiles; t += gridDim.x) {
         const GemmTile tile = args.tiles ? args.tiles[t]
@@ -97,6 +142,9 @@ __global__ __launch_bounds__(WM *WN * 64) void gemm_f64_kernel(GemmArgs args) {
         if (tile.task < 0) continue; // padding slot of the XCD schedule
         const GemmTask task = args.tiles ? args.tasks[tile.task] : args.task0;
         const int m0 = tile.mt * BM, n0 = tile.nt * BN;
+        // rows / columns of this wave's sub-tile that lie inside the block
+        const int mlive = __builtin_amdgcn_readfirstlane(task.M - (m0 + wm * TM));
+        const int nlive = __builtin_amdgcn_readfirstlane(task.N - (n0 + wn * TN));
 
         f64x4 acc[FM][FN];
 #pragma unroll
@@ -106,25 +154,32 @@ __global__ __launch_bounds__(WM *WN * 64) void gemm_f64_kernel(GemmArgs args) {
 
         for (int p = task.first; p < task.first + task.npairs; ++p) {
             const GemmPair pr = args.tiles ? args.pairs[p] : args.pair0;
-            const int32_t *ktabL = args.tab + pr.kL;
-            const int32_t *ktabR = args.tab + pr.kR;
-            la.setup(pr.flags & 1, tid);
-            lb.setup(pr.flags & 2, tid);
-            la.pair(args.tab + pr.mL, m0, task.M);
-            lb.pair(args.tab + pr.nR, n0, task.N);
-            la.load(va, pr.L, ktabL, 0, pr.K);
-            lb.load(vb, pr.R, ktabR, 0, pr.K);
+            la.pair(pr.flags & 1, pr.L, args.tab + pr.kL, pr.K, args.tab + pr.mL, m0, task.M, tid);
+            lb.pair(pr.flags & 2, pr.R, args.tab + pr.kR, pr.K, args.tab + pr.nR, n0, task.N, tid);
+            la.lookup(0, tid);
+            lb.lookup(0, tid);
+            la.issue(va);
+            lb.issue(vb);
+            la.lookup(kBK, tid);
+            lb.lookup(kBK, tid);
             for (int k0 = 0; k0 < pr.K; k0 += kBK) {
                 __syncthreads(); // previous stage fully consumed
-                la.store(As, PA, va);
-                lb.store(Bs, PB, vb);
+                la.store(As, PA, va, k0, tid);
+                lb.store(Bs, PB, vb, k0, tid);
                 __syncthreads();
-                if (k0 + kBK < pr.K) { // prefetch the next stage into registers
-                    la.load(va, pr.L, ktabL, k0 + kBK, pr.K);
-                    lb.load(vb, pr.R, ktabR, k0 + kBK, pr.K);
+                if (k0 + kBK < pr.K) {
+                    la.issue(va);
+                    lb.issue(vb);
+                    la.lookup(k0 + 2 * kBK, tid);
+                    lb.lookup(k0 + 2 * kBK, tid);
                 }
+                // Only the k-steps inside the pair's K and the 16x16 blocks
+                // inside the destination are issued (wave-uniform tests): an
+                // edge wave leaves its SIMD to the other resident workgroups.
+                const int ksteps = min(kBK, pr.K - k0 + 3) / 4;
 #pragma unroll
                 for (int ks = 0; ks < kBK / 4; ++ks) {
+                    if (ks >= ksteps) break;
                     const int kr = ks * 4 + lk;
                     double a[FM], b[FN];
 #pragma unroll
@@ -132,19 +187,24 @@ __global__ __launch_bounds__(WM *WN * 64) void gemm_f64_kernel(GemmArgs args) {
 #pragma unroll
                     for (int j = 0; j < FN; ++j) b[j] = Bs[kr * PB + wn * TN + j * 16 + li];
 #pragma unroll
-                    for (int i = 0; i < FM; ++i)
+                    for (int i = 0; i < FM; ++i) {
+                        if (i * 16 >= mlive) break;
 #pragma unroll
-                        for (int j = 0; j < FN; ++j)
+                        for (int j = 0; j < FN; ++j) {
+                            if (j * 16 >= nlive) break;
                             acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], b[j], acc[i][j],
                                                                              0, 0, 0);
+                        }
+                    }
                 }
             }
         }
 
         // Epilogue: scatter the tile into the destination block.
-        const int32_t *mtab = args.tab + task.mD;
-        const int32_t *ntab = args.tab + task.nD;
-        double *__restrict__ D = task.D;
+        gitab_t mtab = (gitab_t)(args.tab + task.mD);
+        gitab_t ntab = (gitab_t)(args.tab + task.nD);
+        __attribute__((address_space(1))) double *D =
+            (__attribute__((address_space(1))) double *)task.D;
 #pragma unroll
         for (int j = 0; j < FN; ++j) {
             const int n = n0 + wn * TN + j * 16 + li;
@@ -213,39 +273,35 @@ __global__ void scalar_mul_kernel(double *D, const double *L, const double *R) {
     if (threadIdx.x == 0 && blockIdx.x == 0) D[0] = L[0] * R[0];
 }
 
-template <int BM, int BN, int WM, int WN>
+template <int BM, int BN, int WM, int WN, int BK>
 hipError_t launch_cfg(const GemmArgs &args, int grid, hipStream_t stream) {
-    hipLaunchKernelGGL((gemm_f64_kernel<BM, BN, WM, WN>), dim3(grid), dim3(WM * WN * 64), 0,
+    hipLaunchKernelGGL((gemm_f64_kernel<BM, BN, WM, WN, BK>), dim3(grid), dim3(WM * WN * 64), 0,
                        stream, args);
     return hipGetLastError();
 }
 
+struct CfgShape {
+    int bm, bn;
+};
+constexpr CfgShape kCfg[kGemmConfigs] = {
+    {64, 64}, {128, 128}, {64, 128}, {64, 64}, {128, 128}, {128, 64}, {32, 64},
+};
+
 } // namespace
 
-int gemm_tile_m(int cfg) {
-    switch (cfg) {
-    case kGemm64x64: return 64;
-    case kGemm128x128: return 128;
-    case kGemm64x128: return 64;
-    default: return 0;
-    }
-}
-
-int gemm_tile_n(int cfg) {
-    switch (cfg) {
-    case kGemm64x64: return 64;
-    case kGemm128x128: return 128;
-    case kGemm64x128: return 128;
-    default: return 0;
-    }
-}
+int gemm_tile_m(int cfg) { return cfg >= 0 && cfg < kGemmConfigs ? kCfg[cfg].bm : 0; }
+int gemm_tile_n(int cfg) { return cfg >= 0 && cfg < kGemmConfigs ? kCfg[cfg].bn : 0; }
 
 hipError_t launch_gemm(int cfg, const GemmArgs &args, int grid, hipStream_t stream) {
     if (args.ntiles <= 0) return hipSuccess;
     switch (cfg) {
-    case kGemm64x64: return launch_cfg<64, 64, 2, 2>(args, grid, stream);
-    case kGemm128x128: return launch_cfg<128, 128, 2, 2>(args, grid, stream);
-    case kGemm64x128: return launch_cfg<64, 128, 2, 2>(args, grid, stream);
+    case 0: return launch_cfg<64, 64, 2, 2, 16>(args, grid, stream);
+    case 1: return launch_cfg<128, 128, 2, 2, 16>(args, grid, stream);
+    case 2: return launch_cfg<64, 128, 2, 2, 16>(args, grid, stream);
+    case 3: return launch_cfg<64, 64, 2, 2, 32>(args, grid, stream);
+    case 4: return launch_cfg<128, 128, 4, 2, 16>(args, grid, stream);
+    case 5: return launch_cfg<128, 64, 2, 2, 16>(args, grid, stream);
+    case 6: return launch_cfg<32, 64, 1, 2, 16>(args, grid, stream);
     default: return hipErrorInvalidValue;
     }
 }

@@ -67,8 +67,10 @@ struct GemmArgs {
     GemmPair pair0;
 };
 
-// Tile configurations of the grouped FP64-MFMA kernel.
-enum GemmConfig { kGemm64x64 = 0, kGemm128x128 = 1, kGemm64x128 = 2, kGemmConfigs = 3 };
+// Tile configurations of the grouped FP64-MFMA kernel (BM x BN, waves, BK):
+//   0: 64x64 2x2 16   1: 128x128 2x2 16   2: 64x128 2x2 16   3: 64x64 2x2 32
+//   4: 128x128 4x2 16 5: 128x64 2x2 16    6: 32x64 1x2 16
+enum GemmConfig { kGemm64x64 = 0, kGemm128x128 = 1, kGemm64x128 = 2, kGemmConfigs = 7 };
 int gemm_tile_m(int cfg);
 int gemm_tile_n(int cfg);
 hipError_t launch_gemm(int cfg, const GemmArgs &args, int grid, hipStream_t stream);
@@ -86,5 +88,9 @@ hipError_t launch_dot(double *D, const double *L, const int32_t *koffL, const do
                       const int32_t *koffR, int K, double *scratch, hipStream_t stream);
 // D[0] = L[0] * R[0]
 hipError_t launch_scalar_mul(double *D, const double *L, const double *R, hipStream_t stream);
+
+// Calibration kernels (calib.hip).
+hipError_t launch_mfma_peak(double *out, long long *ticks, int blocks, int iters, hipStream_t s);
+hipError_t launch_copy(const double *in, double *out, size_t n, hipStream_t s);
 
 } // namespace sipgpu

@@ -11,6 +11,7 @@
 #include <array>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <mutex>
@@ -35,6 +36,7 @@ struct sipgpu_event_s {
 namespace {
 
 thread_local std::string g_last_error;
+double g_last_clock_mhz = 0.0; // shader clock seen by the last MFMA calibration
 
 int fail(int code, const std::string &msg) {
     g_last_error = msg;
@@ -173,6 +175,12 @@ void make_task_pair(const ContractionShape &s, int kL, int mL, int kR, int nR, i
 }
 
 int pick_config(int64_t M, int64_t N) {
+    // SIPGPU_GEMM_CONFIG=<n> pins the tile configuration (tuning runs).
+    static const int forced = [] {
+        const char *e = std::getenv("SIPGPU_GEMM_CONFIG");
+        return e ? std::atoi(e) : -1;
+    }();
+    if (forced >= 0 && forced < kGemmConfigs) return forced;
     if (M >= 128 && N >= 128) return kGemm128x128;
     if (N >= 128) return kGemm64x128;
     return kGemm64x64;
@@ -580,20 +588,28 @@ int sipgpu_free(void *dptr) {
     return 0;
 }
 
+// The plain copies are ordered after all work enqueued so far on every
+// stream (the library's streams do not synchronise with the null stream).
 int sipgpu_memcpy_h2d(void *dst, const void *src, size_t bytes) {
+    HIP_TRY(hipDeviceSynchronize());
     HIP_TRY(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
     return 0;
 }
 int sipgpu_memcpy_d2h(void *dst, const void *src, size_t bytes) {
+    HIP_TRY(hipDeviceSynchronize());
     HIP_TRY(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
     return 0;
 }
 int sipgpu_memcpy_d2d(void *dst, const void *src, size_t bytes) {
+    HIP_TRY(hipDeviceSynchronize());
     HIP_TRY(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToDevice));
+    HIP_TRY(hipDeviceSynchronize());
     return 0;
 }
 int sipgpu_memset(void *dptr, int value, size_t bytes) {
+    HIP_TRY(hipDeviceSynchronize());
     HIP_TRY(hipMemset(dptr, value, bytes));
+    HIP_TRY(hipDeviceSynchronize());
     return 0;
 }
 
@@ -747,6 +763,71 @@ int sipgpu_batch_info(sipgpu_batch_t b, double *flops, int *ntasks, int *ntiles,
     if (npairs) *npairs = b->npairs;
     return 0;
 }
+
+int sipgpu_measure_peaks(double *mfma_f64_tflops, double *copy_gbs, size_t copy_bytes) {
+    Context &c = ctx();
+    std::lock_guard<std::mutex> g(c.mu);
+    int rc = ensure_init(c);
+    if (rc) return rc;
+    hipStream_t s = c.stream.s;
+    hipEvent_t e0, e1;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    hipDeviceProp_t prop;
+    HIP_TRY(hipGetDeviceProperties(&prop, c.device));
+    if (mfma_f64_tflops) {
+        const int blocks = prop.multiProcessorCount * 8, iters = 4000;
+        double *out = nullptr;
+        long long *ticks = nullptr;
+        HIP_TRY(hipMalloc(&out, (size_t)blocks * 256 * sizeof(double)));
+        HIP_TRY(hipMalloc(&ticks, (size_t)blocks * 2 * sizeof(long long)));
+        HIP_TRY(hipMemsetAsync(out, 0, (size_t)blocks * 256 * sizeof(double), s));
+        HIP_TRY(launch_mfma_peak(out, ticks, blocks, 10, s)); // warm-up
+        HIP_TRY(hipEventRecord(e0, s));
+        HIP_TRY(launch_mfma_peak(out, ticks, blocks, iters, s));
+        HIP_TRY(hipEventRecord(e1, s));
+        HIP_TRY(hipEventSynchronize(e1));
+        float ms = 0;
+        HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+        const double flops = (double)blocks * 4 /*waves*/ * iters * 8 * 2.0 * 16 * 16 * 4;
+        *mfma_f64_tflops = flops / (ms * 1e-3) / 1e12;
+        std::vector<long long> t((size_t)blocks * 2);
+        HIP_TRY(hipMemcpy(t.data(), ticks, t.size() * sizeof(long long), hipMemcpyDeviceToHost));
+        int wall_khz = 0;
+        HIP_TRY(hipDeviceGetAttribute(&wall_khz, hipDeviceAttributeWallClockRate, c.device));
+        double cyc = 0, wall = 0;
+        for (int i = 0; i < blocks; ++i) { cyc += (double)t[2 * i]; wall += (double)t[2 * i + 1]; }
+        g_last_clock_mhz = wall > 0 ? cyc / wall * wall_khz / 1e3 : 0.0;
+        HIP_TRY(hipFree(out));
+        HIP_TRY(hipFree(ticks));
+    }
+    if (copy_gbs) {
+        const size_t n = std::max<size_t>(copy_bytes / sizeof(double), 1 << 20) & ~(size_t)1;
+        double *a = nullptr, *b = nullptr;
+        HIP_TRY(hipMalloc(&a, n * sizeof(double)));
+        HIP_TRY(hipMalloc(&b, n * sizeof(double)));
+        HIP_TRY(hipMemsetAsync(a, 0, n * sizeof(double), s));
+        HIP_TRY(launch_copy(a, b, n, s)); // warm-up
+        float best = 1e30f;
+        for (int r = 0; r < 5; ++r) {
+            HIP_TRY(hipEventRecord(e0, s));
+            HIP_TRY(launch_copy(a, b, n, s));
+            HIP_TRY(hipEventRecord(e1, s));
+            HIP_TRY(hipEventSynchronize(e1));
+            float ms = 0;
+            HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+            best = std::min(best, ms);
+        }
+        *copy_gbs = 2.0 * n * sizeof(double) / (best * 1e-3) / 1e9;
+        HIP_TRY(hipFree(a));
+        HIP_TRY(hipFree(b));
+    }
+    HIP_TRY(hipEventDestroy(e0));
+    HIP_TRY(hipEventDestroy(e1));
+    return 0;
+}
+
+double sipgpu_measured_clock_mhz(void) { return g_last_clock_mhz; }
 
 int sipgpu_plan_tables(const int *ptrn, int lrank, const int *ldims, int rrank, const int *rdims,
                        int drank, const int *ddims, long long *mnk, int *o2n, int *koffL,

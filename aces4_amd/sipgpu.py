@@ -15,7 +15,8 @@ import os
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libsipgpu.so")
+# SIPGPU_LIB selects another build of the library (A/B timing of variants).
+LIB_PATH = os.environ.get("SIPGPU_LIB") or os.path.join(_HERE, "libsipgpu.so")
 MAX_RANK = 6
 
 ERRORS = {
@@ -38,7 +39,8 @@ EXPORTED = [
     "sipgpu_event_create", "sipgpu_event_destroy", "sipgpu_event_record",
     "sipgpu_event_synchronize", "sipgpu_event_elapsed_ms", "sipgpu_permute",
     "sipgpu_contract", "sipgpu_batch_plan", "sipgpu_batch_execute", "sipgpu_batch_destroy",
-    "sipgpu_batch_info", "sipgpu_plan_tables", "sipgpu_plan_permute",
+    "sipgpu_batch_info", "sipgpu_plan_tables", "sipgpu_plan_permute", "sipgpu_measure_peaks",
+    "sipgpu_measured_clock_mhz",
 ]
 
 
@@ -99,6 +101,9 @@ def lib():
                                          ctypes.c_int, _ip, ctypes.POINTER(ctypes.c_longlong),
                                          _ip, _ip, _ip, _ip, _ip, _ip, _ip]
         L.sipgpu_plan_permute.argtypes = [ctypes.c_int, _ip, _ip, _ip]
+        L.sipgpu_measured_clock_mhz.restype = ctypes.c_double
+        L.sipgpu_measure_peaks.argtypes = [ctypes.POINTER(ctypes.c_double),
+                                           ctypes.POINTER(ctypes.c_double), ctypes.c_size_t]
         _lib = L
     return _lib
 
@@ -214,6 +219,18 @@ def device_count():
     n = ctypes.c_int(0)
     _check(lib().sipgpu_device_count(ctypes.byref(n)))
     return n.value
+
+
+def measure_peaks(copy_bytes=1 << 30):
+    """Measured (FP64 MFMA TFLOP/s, streaming-copy GB/s) of the device."""
+    m, c = ctypes.c_double(0), ctypes.c_double(0)
+    _check(lib().sipgpu_measure_peaks(ctypes.byref(m), ctypes.byref(c), copy_bytes))
+    return m.value, c.value
+
+
+def measured_clock_mhz():
+    """Shader clock (MHz) seen by the last measure_peaks MFMA loop."""
+    return lib().sipgpu_measured_clock_mhz()
 
 
 def synchronize():

@@ -174,7 +174,7 @@ def main():
     steps_run = a.warmup + a.steps
     check = rel_fro(D.block(db0, got), steps_run * want)
 
-    lat = latency_sample(w, a.latency_samples)
+    lat = latency_sample(w, a.latency_samples) if a.latency_samples > 0 else None
     cores, model = cpu_info()
     cpu = None if a.no_cpu else cpu_baseline(w, a.cpu_seconds)
     if cpu:

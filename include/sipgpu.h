@@ -98,7 +98,9 @@ int sipgpu_init(int device);
 int sipgpu_device_count(int *count);
 int sipgpu_device_synchronize(void);
 
-/* Device memory (plain HBM allocations, no caching). */
+/* Device memory (plain HBM allocations, no caching).  The copies and the
+ * memset are synchronous and ordered after all work already enqueued on any
+ * stream; sipgpu_free waits for that work too. */
 int sipgpu_malloc(void **dptr, size_t bytes);
 int sipgpu_free(void *dptr);
 int sipgpu_memcpy_h2d(void *dst, const void *src, size_t bytes);
@@ -159,6 +161,13 @@ int sipgpu_batch_destroy(sipgpu_batch_t batch);
 /* Sum of 2*M*N*K over the batch, and its task / tile / pair counts. */
 int sipgpu_batch_info(sipgpu_batch_t batch, double *flops, int *ntasks,
                       int *ntiles, int *npairs);
+
+/* Measured device ceilings: FP64 MFMA throughput (TFLOP/s) of an MFMA-only
+ * loop over all CUs, and HBM bandwidth (GB/s, read + write bytes) of a
+ * streaming copy of `copy_bytes` bytes.  Allocates its own buffers. */
+int sipgpu_measure_peaks(double *mfma_f64_tflops, double *copy_gbs, size_t copy_bytes);
+/* Effective shader clock (MHz) during the last MFMA calibration. */
+double sipgpu_measured_clock_mhz(void);
 
 /* Host-only introspection (no device work).  For one contraction shape:
  * mnk[0..2] = M, N, K of the product; o2n receives lo2n[1..lrank],
