@@ -23,9 +23,11 @@ FLAGS = [
 ]
 
 
-# Extra flags per source.  The calibration loop keeps its MFMA accumulators
-# in the unified VGPR file so the loop is nothing but MFMAs.
-PER_FILE = {"calib.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form=1"]}
+# Extra flags per source.  MFMA kernels keep their accumulators in the
+# unified VGPR file ("VGPR form"): the compiler then neither copies them
+# between register banks around loops nor splits the budget between banks.
+VGPR_FORM = ["-mllvm", "-amdgpu-mfma-vgpr-form=1"]
+PER_FILE = {"calib.hip": VGPR_FORM, "contract.hip": VGPR_FORM}
 NOISE = "is not a recognized feature for this target"
 
 

@@ -70,7 +70,13 @@ struct GemmArgs {
 // Tile configurations of the grouped FP64-MFMA kernel (BM x BN, waves, BK):
 //   0: 64x64 2x2 16   1: 128x128 2x2 16   2: 64x128 2x2 16   3: 64x64 2x2 32
 //   4: 128x128 4x2 16 5: 128x64 2x2 16    6: 32x64 1x2 16
-enum GemmConfig { kGemm64x64 = 0, kGemm128x128 = 1, kGemm64x128 = 2, kGemmConfigs = 7 };
+// double-buffered pair-flattened kernel:
+//   7: 128x128 2x2 8  8: 128x128 2x2 16   9: 128x64 2x2 16   10: 64x64 2x2 16
+// first kernel with a register budget (waves per SIMD):
+//   11: 64x64 (4)     12: 128x64 (2)      13: 128x128 (2)
+// direct-to-LDS kernel (2x2 waves, BK 16):
+//   14: 64x64  15: 128x64  16: 128x128  17: 64x128
+enum GemmConfig { kGemm64x64 = 0, kGemm128x128 = 1, kGemm64x128 = 2, kGemmConfigs = 18 };
 int gemm_tile_m(int cfg);
 int gemm_tile_n(int cfg);
 hipError_t launch_gemm(int cfg, const GemmArgs &args, int grid, hipStream_t stream);

@@ -129,6 +129,31 @@ def latency_sample(w, n):
                 samples=int(n), mode="single contraction, unbatched, host enqueue to completion")
 
 
+def permute_bench(reps=20):
+    """Achieved HBM GB/s of the permute kernel (16 B moved per element):
+    a 64^4 block (134 MB) with the full index reversal, and the ring block
+    shape 30x15x30x15 with perm (3,4,1,2), timed with HIP events."""
+    out = {}
+    for name, dims, perm in (("64^4 reversal", [64, 64, 64, 64], [4, 3, 2, 1]),
+                             ("ring block (3,4,1,2)", [30, 15, 30, 15], [3, 4, 1, 2])):
+        n = int(np.prod(dims))
+        din = sg.DeviceArray.from_host(wl.uniform(11, n))
+        dout = sg.DeviceArray(n)
+        e0, e1 = sg.Event(), sg.Event()
+        sg.permute(dims, [1] + perm, din.ptr, dout.ptr)
+        sg.synchronize()
+        e0.record()
+        for _ in range(reps):
+            sg.permute(dims, [1] + perm, din.ptr, dout.ptr)
+        e1.record()
+        e1.synchronize()
+        ms = e0.elapsed_ms(e1) / reps
+        out[name] = {"dims": dims, "perm": perm, "ms": ms, "GBps": 16.0 * n / (ms * 1e-3) / 1e9}
+        din.free()
+        dout.free()
+    return out
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -175,12 +200,17 @@ def main():
     check = rel_fro(D.block(db0, got), steps_run * want)
 
     lat = latency_sample(w, a.latency_samples) if a.latency_samples > 0 else None
+    perm = permute_bench()
     cores, model = cpu_info()
     cpu = None if a.no_cpu else cpu_baseline(w, a.cpu_seconds)
     if cpu:
         cpu["host"] = "%s, %d logical cores" % (model, cores)
 
     traffic = traffic_from_profile()
+    mfma_meas, copy_meas = sg.measure_peaks(1 << 30)
+    measured = {"mfma_f64_tflops": mfma_meas, "clock_mhz": sg.measured_clock_mhz(),
+                "copy_gbs": copy_meas, "copy_bytes": 2 * (1 << 30),
+                "note": "MFMA-only FP64 loop over all CUs; streaming copy of 1 GiB (read+write)"}
     res = {
         "metric": "FP64-equiv TFLOP/s, split vs plain FP64; p50/p99 per-contraction latency",
         "value": flops * a.steps / wall / 1e12,
@@ -201,7 +231,10 @@ def main():
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS,
                      "traffic": traffic,
-                     "kernel": "gemm_f64_kernel", "avg_launch_ms": avg_launch_s * 1e3},
+                     "kernel": "gemm_f64_kernel", "avg_launch_ms": avg_launch_s * 1e3,
+                     "measured_ceiling": measured,
+                     "frac_of_measured": achieved / mfma_meas if mfma_meas else None},
+        "permute": perm,
         "split": None,
         "latency": lat,
         "cpu_baseline": cpu,
