@@ -9,11 +9,15 @@
 
 #include <algorithm>
 #include <array>
+#include <climits>
+#include <numeric>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <map>
 #include <memory>
+#include <tuple>
 #include <mutex>
 #include <string>
 #include <unordered_map>
@@ -21,6 +25,7 @@
 
 #include "../../include/sipgpu.h"
 #include "kernels.h"
+#include "ozaki.h"
 #include "plan.h"
 
 using namespace sipgpu;
@@ -245,6 +250,26 @@ int64_t prod(int n, const int *dims) {
 // Batches
 // ---------------------------------------------------------------------------
 
+// Device state of a split-precision (INT8 / CRT) batch.
+struct SplitData {
+    OzConst c{};
+    int *expo = nullptr;
+    int nexpo = 0;
+    int8_t *planes = nullptr;
+    long long plane_bytes = 0;
+    int8_t *res_planes = nullptr;
+    long long res_plane_bytes = 0;
+    const OzBlock *blocks = nullptr;
+    int nblocks = 0;
+    const int32_t *tab = nullptr;
+    const int *row_start = nullptr;
+    int nrows = 0;
+    const int *quad_start = nullptr;
+    int nquads = 0;
+    OzGemmArgs gargs{};
+    OzRecArgs rargs{};
+};
+
 struct sipgpu_batch_s {
     int cfg = kGemm64x64;
     int ntiles = 0;     // including XCD padding slots
@@ -252,46 +277,19 @@ struct sipgpu_batch_s {
     double flops = 0.0;
     void *dev = nullptr; // tiles | tasks | pairs | tab
     GemmArgs args{};
+    bool split = false;
+    SplitData sd;
 };
 
 namespace {
 
 constexpr int kXcds = 8; // MI355X: workgroups go round-robin to 8 XCDs
 
-int plan_batch_locked(Context &c, const sipgpu_contraction *list, int n, sipgpu_batch_s &b) {
-    struct Entry {
-        const sipgpu_contraction *e;
-        std::string key;
-    };
-    // Shapes of this batch, deduplicated; tables pooled.
-    std::unordered_map<std::string, int> shape_index;
-    std::vector<const ContractionShape *> shapes;
-    std::vector<std::unique_ptr<ContractionShape>> owned;
-    std::vector<int> shape_of(n);
-    for (int i = 0; i < n; ++i) {
-        const sipgpu_contraction &e = list[i];
-        if (!e.l || !e.r || !e.d) return fail(SIPGPU_EARG, "null block pointer in batch");
-        if (e.lrank < 1 || e.lrank > SIPGPU_MAX_RANK || e.rrank < 1 ||
-            e.rrank > SIPGPU_MAX_RANK || e.drank < 1 || e.drank > SIPGPU_MAX_RANK)
-            return fail(SIPGPU_ERANK, "batch entries must be block = block * block, rank 1..6");
-        std::string key = shape_key(e.ptrn, e.lrank, e.ldims, e.rrank, e.rdims, e.drank, e.ddims);
-        auto it = shape_index.find(key);
-        if (it == shape_index.end()) {
-            auto s = std::make_unique<ContractionShape>();
-            int rc = plan_contraction(e.ptrn, e.lrank, e.ldims, e.rrank, e.rdims, e.drank,
-                                      e.ddims, *s, true);
-            if (rc == 1 || rc == -1)
-                return fail(SIPGPU_EPATTERN, "batch entry " + std::to_string(i) +
-                                                 ": contraction pattern rejected");
-            if (rc) return fail(rc, "batch entry " + std::to_string(i) + ": plan failed");
-            it = shape_index.emplace(key, (int)shapes.size()).first;
-            shapes.push_back(s.get());
-            owned.push_back(std::move(s));
-        }
-        shape_of[i] = it->second;
-    }
-
-    // Hazards: no destination may overlap an operand or another destination.
+// No destination may overlap an operand or another destination (partially:
+// entries naming the same block accumulate into it).
+template <class ShapeVec>
+int check_hazards(const sipgpu_contraction *list, int n, const ShapeVec &shapes,
+                  const std::vector<int> &shape_of) {
     struct Range { uintptr_t lo, hi; int kind, who; };
     std::vector<Range> rs;
     for (int i = 0; i < n; ++i) {
@@ -327,6 +325,44 @@ int plan_batch_locked(Context &c, const sipgpu_contraction *list, int n, sipgpu_
                 return fail(SIPGPU_EHAZARD, "a batch reads a block that it writes");
         }
     }
+
+    return 0;
+}
+
+int plan_batch_locked(Context &c, const sipgpu_contraction *list, int n, sipgpu_batch_s &b) {
+    struct Entry {
+        const sipgpu_contraction *e;
+        std::string key;
+    };
+    // Shapes of this batch, deduplicated; tables pooled.
+    std::unordered_map<std::string, int> shape_index;
+    std::vector<const ContractionShape *> shapes;
+    std::vector<std::unique_ptr<ContractionShape>> owned;
+    std::vector<int> shape_of(n);
+    for (int i = 0; i < n; ++i) {
+        const sipgpu_contraction &e = list[i];
+        if (!e.l || !e.r || !e.d) return fail(SIPGPU_EARG, "null block pointer in batch");
+        if (e.lrank < 1 || e.lrank > SIPGPU_MAX_RANK || e.rrank < 1 ||
+            e.rrank > SIPGPU_MAX_RANK || e.drank < 1 || e.drank > SIPGPU_MAX_RANK)
+            return fail(SIPGPU_ERANK, "batch entries must be block = block * block, rank 1..6");
+        std::string key = shape_key(e.ptrn, e.lrank, e.ldims, e.rrank, e.rdims, e.drank, e.ddims);
+        auto it = shape_index.find(key);
+        if (it == shape_index.end()) {
+            auto s = std::make_unique<ContractionShape>();
+            int rc = plan_contraction(e.ptrn, e.lrank, e.ldims, e.rrank, e.rdims, e.drank,
+                                      e.ddims, *s, true);
+            if (rc == 1 || rc == -1)
+                return fail(SIPGPU_EPATTERN, "batch entry " + std::to_string(i) +
+                                                 ": contraction pattern rejected");
+            if (rc) return fail(rc, "batch entry " + std::to_string(i) + ": plan failed");
+            it = shape_index.emplace(key, (int)shapes.size()).first;
+            shapes.push_back(s.get());
+            owned.push_back(std::move(s));
+        }
+        shape_of[i] = it->second;
+    }
+
+    if (int rc = check_hazards(list, n, shapes, shape_of)) return rc;
 
     // Tasks: entries grouped by destination, in list order.  An assign entry
     // discards everything before it; the task keeps entries from the last
@@ -451,6 +487,337 @@ int plan_batch_locked(Context &c, const sipgpu_contraction *list, int n, sipgpu_
     b.npairs = (int)pairs.size();
     (void)maxM;
     (void)maxN;
+    return 0;
+}
+
+} // namespace
+
+namespace {
+
+// Pairwise coprime moduli <= 255, largest first.
+constexpr int kModuli[kOzMaxModuli] = {255, 254, 253, 251, 247, 241, 239, 233,
+                                       229, 227, 223, 211, 199, 197, 193, 191};
+constexpr int kOzMinModuli = 6, kOzMaxUsable = 14; // M < 2^112: exact in 128-bit two's complement
+
+int inv_mod(int a, int p) { // a^-1 mod p (gcd(a, p) = 1)
+    int t = 0, nt = 1, r = p, nr = ((a % p) + p) % p;
+    while (nr) {
+        const int q = r / nr;
+        int tmp = t - q * nt; t = nt; nt = tmp;
+        tmp = r - q * nr; r = nr; nr = tmp;
+    }
+    return t < 0 ? t + p : t;
+}
+
+// Moduli constants for n moduli and the largest task K.  Returns P (bits).
+int oz_constants(int n, long long kmax, OzConst &c) {
+    c = OzConst{};
+    c.n = n;
+    unsigned __int128 M = 1;
+    double log2M = 0.0;
+    for (int i = 0; i < n; ++i) {
+        c.p[i] = kModuli[i];
+        M *= (unsigned __int128)kModuli[i];
+        log2M += std::log2((double)kModuli[i]);
+    }
+    for (int i = 0; i < n; ++i) {
+        const int p = c.p[i];
+        long long mi = 1; // (M / p_i) mod p_i
+        for (int j = 0; j < n; ++j)
+            if (j != i) mi = (mi * (c.p[j] % p)) % p;
+        c.u[i] = inv_mod((int)mi, p);
+        c.t24[i] = (int)((1LL << 24) % p);
+        c.rp[i] = 1.0 / p;
+        const unsigned __int128 Mi = M / (unsigned __int128)p;
+        c.mi[i][0] = (uint64_t)Mi;
+        c.mi[i][1] = (uint64_t)(Mi >> 64);
+    }
+    c.m[0] = (uint64_t)M;
+    c.m[1] = (uint64_t)(M >> 64);
+    // |C| <= K 2^(2P-2) <= M / 4
+    const double lk = std::log2((double)std::max<long long>(kmax, 1));
+    int P = (int)std::floor((log2M - 2.0 - lk) / 2.0) + 1;
+    P = std::min(P, 52);
+    c.P = P;
+    return P;
+}
+
+struct UnionFind {
+    std::vector<int> parent;
+    explicit UnionFind(int n) : parent(n) { std::iota(parent.begin(), parent.end(), 0); }
+    int find(int x) {
+        while (parent[x] != x) x = parent[x] = parent[parent[x]];
+        return x;
+    }
+    void unite(int a, int b) { parent[find(a)] = find(b); }
+};
+
+int plan_split_locked(Context &c, const sipgpu_contraction *list, int n, int nmod,
+                      sipgpu_batch_s &b) {
+    if (nmod < kOzMinModuli || nmod > kOzMaxUsable)
+        return fail(SIPGPU_EARG, "split path: nmoduli must be in 6..14");
+    // shapes
+    std::unordered_map<std::string, int> shape_index;
+    std::vector<std::unique_ptr<ContractionShape>> shapes;
+    std::vector<int> shape_of(n);
+    for (int i = 0; i < n; ++i) {
+        const sipgpu_contraction &e = list[i];
+        if (!e.l || !e.r || !e.d) return fail(SIPGPU_EARG, "null block pointer in batch");
+        if (e.lrank < 1 || e.lrank > SIPGPU_MAX_RANK || e.rrank < 1 ||
+            e.rrank > SIPGPU_MAX_RANK || e.drank < 1 || e.drank > SIPGPU_MAX_RANK)
+            return fail(SIPGPU_ERANK, "batch entries must be block = block * block, rank 1..6");
+        const std::string key =
+            shape_key(e.ptrn, e.lrank, e.ldims, e.rrank, e.rdims, e.drank, e.ddims);
+        auto it = shape_index.find(key);
+        if (it == shape_index.end()) {
+            auto sh = std::make_unique<ContractionShape>();
+            const int rc = plan_contraction(e.ptrn, e.lrank, e.ldims, e.rrank, e.rdims, e.drank,
+                                            e.ddims, *sh, true);
+            if (rc == 1 || rc == -1)
+                return fail(SIPGPU_EPATTERN, "batch entry " + std::to_string(i) +
+                                                 ": contraction pattern rejected");
+            if (rc) return fail(rc, "batch entry " + std::to_string(i) + ": plan failed");
+            it = shape_index.emplace(key, (int)shapes.size()).first;
+            shapes.push_back(std::move(sh));
+        }
+        shape_of[i] = it->second;
+    }
+    if (int rc = check_hazards(list, n, shapes, shape_of)) return rc;
+    // table pool (per shape)
+    std::vector<int32_t> pool;
+    std::vector<std::array<int, 6>> tabs(shapes.size());
+    for (size_t si = 0; si < shapes.size(); ++si) {
+        const ContractionShape &sh = *shapes[si];
+        const std::vector<int32_t> *ts[6] = {&sh.koffL, &sh.moffL, &sh.koffR,
+                                             &sh.noffR, &sh.moffD, &sh.noffD};
+        for (int t = 0; t < 6; ++t) {
+            tabs[si][t] = (int)pool.size();
+            pool.insert(pool.end(), ts[t]->begin(), ts[t]->end());
+        }
+    }
+    // operand instances: (pointer, shape, side)
+    std::map<std::tuple<const double *, int, int>, int> inst;
+    std::vector<OzBlock> blocks;
+    auto operand = [&](const double *ptr, int si, int side) {
+        const auto key = std::make_tuple(ptr, si, side);
+        auto it = inst.find(key);
+        if (it != inst.end()) return it->second;
+        const ContractionShape &sh = *shapes[si];
+        OzBlock ob{};
+        ob.data = ptr;
+        ob.K = (int)sh.K;
+        ob.X = side == 0 ? (int)sh.M : (int)sh.N;
+        ob.Kp = (int)((sh.K + kOzKAlign - 1) / kOzKAlign * kOzKAlign);
+        ob.ktab = tabs[si][side == 0 ? 0 : 2];
+        ob.xtab = tabs[si][side == 0 ? 1 : 3];
+        blocks.push_back(ob);
+        inst.emplace(key, (int)blocks.size() - 1);
+        return (int)blocks.size() - 1;
+    };
+    // tasks grouped by destination (same rules as the FP64 batch)
+    std::unordered_map<const double *, std::vector<int>> by_dest;
+    std::vector<const double *> dest_order;
+    for (int i = 0; i < n; ++i) {
+        auto &v = by_dest[list[i].d];
+        if (v.empty()) dest_order.push_back(list[i].d);
+        v.push_back(i);
+    }
+    std::vector<OzTask> tasks;
+    std::vector<OzPair> pairs;
+    std::vector<int> task_l, task_r; // one operand instance per task (for components)
+    std::vector<double> task_cost;
+    long long kmax = 0;
+    b.flops = 0.0;
+    for (const double *dp : dest_order) {
+        const auto &idx = by_dest[dp];
+        size_t start = 0;
+        for (size_t j = 0; j < idx.size(); ++j)
+            if (!list[idx[j]].accumulate) start = j;
+        const int si0 = shape_of[idx[start]];
+        const ContractionShape &s0 = *shapes[si0];
+        OzTask t{};
+        t.D = const_cast<double *>(dp);
+        t.M = (int)s0.M;
+        t.N = (int)s0.N;
+        t.mD = tabs[si0][4];
+        t.nD = tabs[si0][5];
+        t.first = (int)pairs.size();
+        t.beta = list[idx[start]].accumulate ? 1 : 0;
+        long long ktot = 0, kptot = 0;
+        double cost = 0.0;
+        for (size_t j = start; j < idx.size(); ++j) {
+            const int i = idx[j];
+            const int si = shape_of[i];
+            const ContractionShape &sh = *shapes[si];
+            if (sh.M != s0.M || sh.N != s0.N || sh.moffD != s0.moffD || sh.noffD != s0.noffD)
+                return fail(SIPGPU_EARG, "entries writing one block disagree on its layout");
+            OzPair pr{};
+            pr.a = operand(list[i].l, si, 0);
+            pr.b = operand(list[i].r, si, 1);
+            pr.Kp = blocks[pr.a].Kp;
+            pairs.push_back(pr);
+            ktot += sh.K;
+            kptot += pr.Kp;
+            cost += 2.0 * (double)sh.M * (double)sh.N * (double)sh.K;
+        }
+        if (kptot > 131072)
+            return fail(SIPGPU_ESIZE, "split path: a result block sums more than 131072 k");
+        t.npairs = (int)pairs.size() - t.first;
+        kmax = std::max(kmax, ktot);
+        b.flops += cost;
+        tasks.push_back(t);
+        task_cost.push_back(cost);
+    }
+    const int ntasks = (int)tasks.size();
+    // row / column components: operands of one task share its rows (columns)
+    const int nb = (int)blocks.size();
+    UnionFind uf(nb);
+    for (const OzTask &t : tasks)
+        for (int q = t.first + 1; q < t.first + t.npairs; ++q) {
+            uf.unite(pairs[t.first].a, pairs[q].a);
+            uf.unite(pairs[t.first].b, pairs[q].b);
+        }
+    std::unordered_map<int, int> comp_off;
+    int nexpo = 0;
+    for (int i = 0; i < nb; ++i) {
+        const int r = uf.find(i);
+        auto it = comp_off.find(r);
+        if (it == comp_off.end()) {
+            it = comp_off.emplace(r, nexpo).first;
+            nexpo += blocks[r].X;
+        }
+        if (blocks[i].X != blocks[r].X)
+            return fail(SIPGPU_EARG, "split path: operands sharing rows disagree on extents");
+        blocks[i].exp_off = it->second;
+    }
+    for (OzTask &t : tasks) {
+        t.expA = blocks[pairs[t.first].a].exp_off;
+        t.expB = blocks[pairs[t.first].b].exp_off;
+    }
+    // panels and result planes
+    long long panel = 0;
+    std::vector<int> row_start(nb + 1), quad_start(nb + 1);
+    long long rows = 0, quads = 0;
+    for (int i = 0; i < nb; ++i) {
+        blocks[i].panel_off = panel;
+        panel += ((long long)blocks[i].X * blocks[i].Kp + 255) / 256 * 256;
+        row_start[i] = (int)rows;
+        quad_start[i] = (int)quads;
+        rows += blocks[i].X;
+        quads += (long long)blocks[i].X * (blocks[i].Kp / 4);
+    }
+    if (quads >= INT_MAX) return fail(SIPGPU_ESIZE, "split path: operands too large");
+    row_start[nb] = (int)rows;
+    quad_start[nb] = (int)quads;
+    std::vector<long long> elem_start(ntasks + 1, 0);
+    for (int t = 0; t < ntasks; ++t) {
+        tasks[t].res_off = elem_start[t];
+        elem_start[t + 1] = elem_start[t] + (long long)tasks[t].M * tasks[t].N;
+    }
+    const long long res_elems = elem_start[ntasks];
+    // tiles: whole tasks to XCDs, heaviest first (as for the FP64 batch)
+    std::vector<int> order(ntasks);
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int x, int y) { return task_cost[x] > task_cost[y]; });
+    std::vector<std::vector<OzTile>> per_xcd(kXcds);
+    std::vector<double> load(kXcds, 0.0);
+    int real_tiles = 0;
+    for (int ti : order) {
+        const int x = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+        const int tm = (tasks[ti].M + kOzBM - 1) / kOzBM, tn = (tasks[ti].N + kOzBN - 1) / kOzBN;
+        for (int q = 0; q < tn; ++q)
+            for (int p = 0; p < tm; ++p) per_xcd[x].push_back(OzTile{ti, p, q, 0});
+        load[x] += task_cost[ti];
+        real_tiles += tm * tn;
+    }
+    size_t per = 0;
+    for (auto &v : per_xcd) per = std::max(per, v.size());
+    std::vector<OzTile> tiles(per * kXcds, OzTile{-1, 0, 0, 0});
+    for (int x = 0; x < kXcds; ++x)
+        for (size_t j = 0; j < per_xcd[x].size(); ++j) tiles[j * kXcds + x] = per_xcd[x][j];
+
+    OzConst oc;
+    const int P = oz_constants(nmod, kmax, oc);
+    if (P < 8) return fail(SIPGPU_EARG, "split path: too few moduli for this K");
+
+    // one work allocation
+    auto align = [](size_t v) { return (v + 255) & ~(size_t)255; };
+    size_t off = 0;
+    const size_t o_blocks = off; off = align(off + blocks.size() * sizeof(OzBlock));
+    const size_t o_pairs = off; off = align(off + pairs.size() * sizeof(OzPair));
+    const size_t o_tasks = off; off = align(off + tasks.size() * sizeof(OzTask));
+    const size_t o_tiles = off; off = align(off + tiles.size() * sizeof(OzTile));
+    const size_t o_tab = off; off = align(off + pool.size() * sizeof(int32_t));
+    const size_t o_rows = off; off = align(off + row_start.size() * sizeof(int));
+    const size_t o_quads = off; off = align(off + quad_start.size() * sizeof(int));
+    const size_t o_elems = off; off = align(off + elem_start.size() * sizeof(long long));
+    const size_t o_expo = off; off = align(off + (size_t)nexpo * sizeof(int));
+    const size_t total = off + 256;
+    std::vector<unsigned char> host(total, 0);
+    std::memcpy(host.data() + o_blocks, blocks.data(), blocks.size() * sizeof(OzBlock));
+    std::memcpy(host.data() + o_pairs, pairs.data(), pairs.size() * sizeof(OzPair));
+    std::memcpy(host.data() + o_tasks, tasks.data(), tasks.size() * sizeof(OzTask));
+    std::memcpy(host.data() + o_tiles, tiles.data(), tiles.size() * sizeof(OzTile));
+    std::memcpy(host.data() + o_tab, pool.data(), pool.size() * sizeof(int32_t));
+    std::memcpy(host.data() + o_rows, row_start.data(), row_start.size() * sizeof(int));
+    std::memcpy(host.data() + o_quads, quad_start.data(), quad_start.size() * sizeof(int));
+    std::memcpy(host.data() + o_elems, elem_start.data(), elem_start.size() * sizeof(long long));
+    HIP_TRY(hipMalloc(&b.dev, total));
+    HIP_TRY(hipMemcpy(b.dev, host.data(), total, hipMemcpyHostToDevice));
+    unsigned char *base = static_cast<unsigned char *>(b.dev);
+    SplitData &sd = b.sd;
+    sd.c = oc;
+    sd.plane_bytes = std::max<long long>(panel, 256);
+    sd.res_plane_bytes = (res_elems + 255) / 256 * 256;
+    HIP_TRY(hipMalloc(&sd.planes, (size_t)sd.plane_bytes * nmod));
+    HIP_TRY(hipMalloc(&sd.res_planes, (size_t)sd.res_plane_bytes * nmod));
+    sd.expo = reinterpret_cast<int *>(base + o_expo);
+    sd.nexpo = nexpo;
+    sd.blocks = reinterpret_cast<const OzBlock *>(base + o_blocks);
+    sd.nblocks = nb;
+    sd.tab = reinterpret_cast<const int32_t *>(base + o_tab);
+    sd.row_start = reinterpret_cast<const int *>(base + o_rows);
+    sd.nrows = (int)rows;
+    sd.quad_start = reinterpret_cast<const int *>(base + o_quads);
+    sd.nquads = (int)quads;
+    OzGemmArgs &g = sd.gargs;
+    g.tiles = reinterpret_cast<const OzTile *>(base + o_tiles);
+    g.tasks = reinterpret_cast<const OzTask *>(base + o_tasks);
+    g.pairs = reinterpret_cast<const OzPair *>(base + o_pairs);
+    g.blocks = sd.blocks;
+    g.planes = sd.planes;
+    g.plane_bytes = sd.plane_bytes;
+    g.res_planes = sd.res_planes;
+    g.res_plane_bytes = sd.res_plane_bytes;
+    g.ntiles = (int)tiles.size();
+    g.c = oc;
+    OzRecArgs &r = sd.rargs;
+    r.tasks = g.tasks;
+    r.ntasks = ntasks;
+    r.elem_start = reinterpret_cast<const long long *>(base + o_elems);
+    r.nelems = res_elems;
+    r.tab = sd.tab;
+    r.expo = sd.expo;
+    r.res_planes = sd.res_planes;
+    r.res_plane_bytes = sd.res_plane_bytes;
+    r.c = oc;
+    b.split = true;
+    b.ntiles = (int)tiles.size();
+    b.ntasks = ntasks;
+    b.npairs = (int)pairs.size();
+    b.real_tiles = real_tiles;
+    return 0;
+}
+
+int execute_split(sipgpu_batch_s &b, hipStream_t s) {
+    SplitData &sd = b.sd;
+    HIP_TRY(oz_fill_int(sd.expo, sd.nexpo, INT_MIN, s));
+    HIP_TRY(oz_row_exponents(sd.blocks, sd.nblocks, sd.tab, sd.expo, sd.row_start, sd.nrows, s));
+    HIP_TRY(oz_to_residues(sd.blocks, sd.nblocks, sd.tab, sd.expo, sd.quad_start, sd.nquads,
+                           sd.planes, sd.plane_bytes, sd.c, s));
+    HIP_TRY(oz_gemm(sd.gargs, sd.gargs.ntiles, s));
+    HIP_TRY(oz_reconstruct(sd.rargs, s));
     return 0;
 }
 
@@ -743,6 +1110,7 @@ int sipgpu_batch_execute(sipgpu_batch_t b, sipgpu_stream_t s) {
     if (!b) return fail(SIPGPU_EARG, "null batch");
     Context &c = ctx();
     if (b->ntiles == 0) return 0;
+    if (b->split) return execute_split(*b, resolve(c, s).s);
     HIP_TRY(launch_gemm(b->cfg, b->args, b->ntiles, resolve(c, s).s));
     return 0;
 }
@@ -753,7 +1121,35 @@ int sipgpu_batch_destroy(sipgpu_batch_t b) {
     hipError_t e = hipDeviceSynchronize();
     if (e != hipSuccess) return hip_fail(e, "sipgpu_batch_destroy");
     if (b->dev) (void)hipFree(b->dev);
+    if (b->sd.planes) (void)hipFree(b->sd.planes);
+    if (b->sd.res_planes) (void)hipFree(b->sd.res_planes);
     delete b;
+    return 0;
+}
+
+int sipgpu_batch_plan_split(const sipgpu_contraction *list, int n, int nmoduli,
+                            sipgpu_batch_t *batch) {
+    if (!batch || (n > 0 && !list) || n < 0) return fail(SIPGPU_EARG, "bad batch arguments");
+    Context &c = ctx();
+    std::lock_guard<std::mutex> g(c.mu);
+    int rc = ensure_init(c);
+    if (rc) return rc;
+    auto b = std::make_unique<sipgpu_batch_s>();
+    rc = plan_split_locked(c, list, n, nmoduli, *b);
+    if (rc) {
+        if (b->dev) (void)hipFree(b->dev);
+        if (b->sd.planes) (void)hipFree(b->sd.planes);
+        if (b->sd.res_planes) (void)hipFree(b->sd.res_planes);
+        return rc;
+    }
+    *batch = b.release();
+    return 0;
+}
+
+int sipgpu_batch_split_info(sipgpu_batch_t b, int *nmoduli, int *bits) {
+    if (!b) return fail(SIPGPU_EARG, "null batch");
+    if (nmoduli) *nmoduli = b->split ? b->sd.c.n : 0;
+    if (bits) *bits = b->split ? b->sd.c.P : 53;
     return 0;
 }
 

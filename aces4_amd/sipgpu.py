@@ -40,7 +40,7 @@ EXPORTED = [
     "sipgpu_event_synchronize", "sipgpu_event_elapsed_ms", "sipgpu_permute",
     "sipgpu_contract", "sipgpu_batch_plan", "sipgpu_batch_execute", "sipgpu_batch_destroy",
     "sipgpu_batch_info", "sipgpu_plan_tables", "sipgpu_plan_permute", "sipgpu_measure_peaks",
-    "sipgpu_measured_clock_mhz",
+    "sipgpu_measured_clock_mhz", "sipgpu_batch_plan_split", "sipgpu_batch_split_info",
 ]
 
 
@@ -92,6 +92,10 @@ def lib():
         L.sipgpu_batch_plan.argtypes = [ctypes.POINTER(Contraction), ctypes.c_int,
                                         ctypes.POINTER(_vp)]
         L.sipgpu_batch_execute.argtypes = [_vp, _vp]
+        L.sipgpu_batch_plan_split.argtypes = [ctypes.POINTER(Contraction), ctypes.c_int,
+                                              ctypes.c_int, ctypes.POINTER(_vp)]
+        L.sipgpu_batch_split_info.argtypes = [_vp, ctypes.POINTER(ctypes.c_int),
+                                              ctypes.POINTER(ctypes.c_int)]
         L.sipgpu_batch_destroy.argtypes = [_vp]
         L.sipgpu_batch_info.argtypes = [_vp, ctypes.POINTER(ctypes.c_double),
                                         ctypes.POINTER(ctypes.c_int),
@@ -351,13 +355,21 @@ def make_contraction(ptrn, l, ldims, r, rdims, d, ddims, accumulate=False):
 
 
 class Batch:
-    """A planned batch of contractions: one grouped FP64-MFMA launch."""
+    """A planned batch of contractions: one grouped FP64-MFMA launch, or with
+    `moduli` set, the split-precision (INT8 / CRT) path over that many moduli."""
 
-    def __init__(self, contractions):
+    def __init__(self, contractions, moduli=None):
         arr = (Contraction * len(contractions))(*contractions)
         p = _vp()
-        _check(lib().sipgpu_batch_plan(arr, len(contractions), ctypes.byref(p)))
+        if moduli is None:
+            _check(lib().sipgpu_batch_plan(arr, len(contractions), ctypes.byref(p)))
+        else:
+            _check(lib().sipgpu_batch_plan_split(arr, len(contractions), int(moduli),
+                                                 ctypes.byref(p)))
         self.handle = p.value
+        nm, bits = ctypes.c_int(0), ctypes.c_int(0)
+        _check(lib().sipgpu_batch_split_info(self.handle, ctypes.byref(nm), ctypes.byref(bits)))
+        self.moduli, self.bits = nm.value, bits.value
         fl = ctypes.c_double(0)
         nt, ntl, npr = ctypes.c_int(0), ctypes.c_int(0), ctypes.c_int(0)
         _check(lib().sipgpu_batch_info(self.handle, ctypes.byref(fl), ctypes.byref(nt),

@@ -159,8 +159,9 @@ def ladder(o_segs=(10,), v_segs=(10, 10, 10, 10), seed=1):
                 labels=("abij", "abcd", "cdij"))
 
 
-def batch_for(w, accumulate=True):
-    """A planned device batch for workload `w` (arrays uploaded on demand)."""
+def batch_for(w, accumulate=True, moduli=None):
+    """A planned device batch for workload `w` (arrays uploaded on demand);
+    `moduli` selects the split-precision path."""
     for k in ("D", "L", "R"):
         if w[k].dev is None:
             w[k].to_device()
@@ -168,7 +169,7 @@ def batch_for(w, accumulate=True):
     cs = [sg.make_contraction(ptrn, L.addr(lb), L.dims(lb), R.addr(rb), R.dims(rb), D.addr(db),
                               D.dims(db), accumulate=accumulate)
           for db, lb, rb in w["entries"]]
-    return sg.Batch(cs)
+    return sg.Batch(cs, moduli=moduli)
 
 
 def flops(w):

@@ -42,6 +42,7 @@ def parse():
                    help="budget for the CPU baseline sample")
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--latency-samples", type=int, default=200)
+    p.add_argument("--no-split", action="store_true")
     return p.parse_args()
 
 
@@ -154,6 +155,43 @@ def permute_bench(reps=20):
     return out
 
 
+def split_bench(w, fp64_batch, reps=20, moduli=(14, 13)):
+    """The split-precision path (INT8 MFMA over pairwise-coprime moduli + CRT)
+    on the same batch: FP64-equivalent TFLOP/s (the batch's 2MNK flop over the
+    whole execution: exponents, residue conversion, INT8 GEMMs,
+    reconstruction), and the worst per-block relative Frobenius error against
+    the plain FP64 path."""
+    from cases import rel_fro
+    D = w["D"]
+    zeros = np.zeros_like(D.host)
+    D.dev.upload(zeros)
+    fp64_batch.execute()
+    sg.synchronize()
+    ref = D.dev.download()
+    out = {}
+    for nm in moduli:
+        b = wl.batch_for(w, moduli=nm)
+        D.dev.upload(zeros)
+        b.execute()
+        sg.synchronize()
+        got = D.dev.download()
+        err = max(rel_fro(D.block(db, got), D.block(db, ref)) for db in D.block_ids())
+        b.execute()
+        sg.synchronize()
+        e0, e1 = sg.Event(), sg.Event()
+        e0.record()
+        for _ in range(reps):
+            b.execute()
+        e1.record()
+        e1.synchronize()
+        ms = e0.elapsed_ms(e1) / reps
+        out["moduli_%d" % nm] = {"moduli": nm, "bits": b.bits, "ms": ms,
+                                 "tflops_fp64_equiv": b.flops / (ms * 1e-3) / 1e12,
+                                 "max_block_rel_err_vs_fp64": err}
+        b.destroy()
+    return out
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -199,6 +237,10 @@ def main():
     steps_run = a.warmup + a.steps
     check = rel_fro(D.block(db0, got), steps_run * want)
 
+    split = split_bench(w, batch) if not a.no_split else None
+    if split:
+        best = max(v["tflops_fp64_equiv"] for v in split.values())
+        split["speedup_vs_plain"] = best / achieved
     lat = latency_sample(w, a.latency_samples) if a.latency_samples > 0 else None
     perm = permute_bench()
     cores, model = cpu_info()
@@ -235,7 +277,7 @@ def main():
                      "measured_ceiling": measured,
                      "frac_of_measured": achieved / mfma_meas if mfma_meas else None},
         "permute": perm,
-        "split": None,
+        "split": split,
         "latency": lat,
         "cpu_baseline": cpu,
         "check_rel_err": check,

@@ -158,6 +158,19 @@ int sipgpu_batch_plan(const sipgpu_contraction *list, int n, sipgpu_batch_t *bat
 /* Runs a planned batch as one grouped FP64-MFMA launch on `s`. */
 int sipgpu_batch_execute(sipgpu_batch_t batch, sipgpu_stream_t s);
 int sipgpu_batch_destroy(sipgpu_batch_t batch);
+/* Plans the same batch for the split-precision path: every FP64 product is
+ * emulated with INT8 MFMA (Ozaki scheme II): rows of the left operand and
+ * columns of the right are scaled to P-bit integers, their product is formed
+ * exactly modulo `nmoduli` (6..14) pairwise-coprime moduli <= 255 with INT32
+ * accumulation, and recovered by the Chinese remainder theorem in FP64.  P
+ * is the largest width for which the exact product fits (sipgpu_batch_split_info);
+ * the result's relative error is of order 2^-P.  Executed with
+ * sipgpu_batch_execute; each execution rescales and re-slices its operands.
+ * Blocks that share rows through a result block share their row scales. */
+int sipgpu_batch_plan_split(const sipgpu_contraction *list, int n, int nmoduli,
+                            sipgpu_batch_t *batch);
+/* Moduli count and operand bits P of a split batch (0 and 53 for FP64 batches). */
+int sipgpu_batch_split_info(sipgpu_batch_t batch, int *nmoduli, int *bits);
 /* Sum of 2*M*N*K over the batch, and its task / tile / pair counts. */
 int sipgpu_batch_info(sipgpu_batch_t batch, double *flops, int *ntasks,
                       int *ntiles, int *npairs);
