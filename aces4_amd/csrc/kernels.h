@@ -98,5 +98,8 @@ hipError_t launch_scalar_mul(double *D, const double *L, const double *R, hipStr
 // Calibration kernels (calib.hip).
 hipError_t launch_mfma_peak(double *out, long long *ticks, int blocks, int iters, hipStream_t s);
 hipError_t launch_copy(const double *in, double *out, size_t n, hipStream_t s);
+hipError_t launch_fill_random(double *out, long long n, unsigned long long seed,
+                              unsigned long long stream, int dist, double a, double b,
+                              hipStream_t s);
 
 } // namespace sipgpu

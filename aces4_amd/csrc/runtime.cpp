@@ -285,6 +285,26 @@ namespace {
 
 constexpr int kXcds = 8; // MI355X: workgroups go round-robin to 8 XCDs
 
+// Offset tables of a batch, pooled in one device array; tables with equal
+// contents are stored once (shapes of uneven segments share most of them).
+struct TablePool {
+    std::vector<int32_t> pool;
+    std::unordered_map<std::string, int> index;
+    int add(const std::vector<int32_t> &t) {
+        std::string key(reinterpret_cast<const char *>(t.data()), t.size() * sizeof(int32_t));
+        auto it = index.find(key);
+        if (it != index.end()) return it->second;
+        const int off = (int)pool.size();
+        pool.insert(pool.end(), t.begin(), t.end());
+        index.emplace(std::move(key), off);
+        return off;
+    }
+    std::array<int, 6> add_shape(const ContractionShape &s) {
+        return {add(s.koffL), add(s.moffL), add(s.koffR), add(s.noffR), add(s.moffD),
+                add(s.noffD)};
+    }
+};
+
 // No destination may overlap an operand or another destination (partially:
 // entries naming the same block accumulate into it).
 template <class ShapeVec>
@@ -375,19 +395,13 @@ int plan_batch_locked(Context &c, const sipgpu_contraction *list, int n, sipgpu_
         v.push_back(i);
     }
 
-    std::vector<int32_t> pool;
+    TablePool tp;
+    std::vector<int32_t> &pool = tp.pool;
     std::unordered_map<int, std::array<int, 6>> shape_tab;
     auto tabs_of = [&](int si) -> std::array<int, 6> {
         auto it = shape_tab.find(si);
         if (it != shape_tab.end()) return it->second;
-        const ContractionShape &s = *shapes[si];
-        std::array<int, 6> at{};
-        const std::vector<int32_t> *ts[6] = {&s.koffL, &s.moffL, &s.koffR,
-                                             &s.noffR, &s.moffD, &s.noffD};
-        for (int t = 0; t < 6; ++t) {
-            at[t] = (int)pool.size();
-            pool.insert(pool.end(), ts[t]->begin(), ts[t]->end());
-        }
+        const std::array<int, 6> at = tp.add_shape(*shapes[si]);
         shape_tab.emplace(si, at);
         return at;
     };
@@ -583,33 +597,28 @@ int plan_split_locked(Context &c, const sipgpu_contraction *list, int n, int nmo
         shape_of[i] = it->second;
     }
     if (int rc = check_hazards(list, n, shapes, shape_of)) return rc;
-    // table pool (per shape)
-    std::vector<int32_t> pool;
+    // table pool (per shape, identical tables shared)
+    TablePool tp;
+    std::vector<int32_t> &pool = tp.pool;
     std::vector<std::array<int, 6>> tabs(shapes.size());
-    for (size_t si = 0; si < shapes.size(); ++si) {
-        const ContractionShape &sh = *shapes[si];
-        const std::vector<int32_t> *ts[6] = {&sh.koffL, &sh.moffL, &sh.koffR,
-                                             &sh.noffR, &sh.moffD, &sh.noffD};
-        for (int t = 0; t < 6; ++t) {
-            tabs[si][t] = (int)pool.size();
-            pool.insert(pool.end(), ts[t]->begin(), ts[t]->end());
-        }
-    }
-    // operand instances: (pointer, shape, side)
-    std::map<std::tuple<const double *, int, int>, int> inst;
+    for (size_t si = 0; si < shapes.size(); ++si) tabs[si] = tp.add_shape(*shapes[si]);
+    // operand instances: one residue panel per (block, k table, x table)
+    std::map<std::tuple<const double *, int, int, int, int>, int> inst;
     std::vector<OzBlock> blocks;
     auto operand = [&](const double *ptr, int si, int side) {
-        const auto key = std::make_tuple(ptr, si, side);
+        const ContractionShape &sh = *shapes[si];
+        const int ktab = tabs[si][side == 0 ? 0 : 2], xtab = tabs[si][side == 0 ? 1 : 3];
+        const int X = side == 0 ? (int)sh.M : (int)sh.N;
+        const auto key = std::make_tuple(ptr, ktab, xtab, (int)sh.K, X);
         auto it = inst.find(key);
         if (it != inst.end()) return it->second;
-        const ContractionShape &sh = *shapes[si];
         OzBlock ob{};
         ob.data = ptr;
         ob.K = (int)sh.K;
-        ob.X = side == 0 ? (int)sh.M : (int)sh.N;
+        ob.X = X;
         ob.Kp = (int)((sh.K + kOzKAlign - 1) / kOzKAlign * kOzKAlign);
-        ob.ktab = tabs[si][side == 0 ? 0 : 2];
-        ob.xtab = tabs[si][side == 0 ? 1 : 3];
+        ob.ktab = ktab;
+        ob.xtab = xtab;
         blocks.push_back(ob);
         inst.emplace(key, (int)blocks.size() - 1);
         return (int)blocks.size() - 1;
@@ -1159,6 +1168,18 @@ int sipgpu_batch_info(sipgpu_batch_t b, double *flops, int *ntasks, int *ntiles,
     if (ntasks) *ntasks = b->ntasks;
     if (ntiles) *ntiles = b->real_tiles;
     if (npairs) *npairs = b->npairs;
+    return 0;
+}
+
+int sipgpu_fill_random(sipgpu_stream_t s, double *out, long long n, unsigned long long seed,
+                       unsigned long long stream, int dist, double a, double b) {
+    if (n < 0 || (n > 0 && !out) || (dist != 0 && dist != 1))
+        return fail(SIPGPU_EARG, "fill_random: bad arguments");
+    Context &c = ctx();
+    std::lock_guard<std::mutex> g(c.mu);
+    int rc = ensure_init(c);
+    if (rc) return rc;
+    HIP_TRY(launch_fill_random(out, n, seed, stream, dist, a, b, resolve(c, s).s));
     return 0;
 }
 

@@ -41,6 +41,7 @@ EXPORTED = [
     "sipgpu_contract", "sipgpu_batch_plan", "sipgpu_batch_execute", "sipgpu_batch_destroy",
     "sipgpu_batch_info", "sipgpu_plan_tables", "sipgpu_plan_permute", "sipgpu_measure_peaks",
     "sipgpu_measured_clock_mhz", "sipgpu_batch_plan_split", "sipgpu_batch_split_info",
+    "sipgpu_fill_random",
 ]
 
 
@@ -106,6 +107,9 @@ def lib():
                                          _ip, _ip, _ip, _ip, _ip, _ip, _ip]
         L.sipgpu_plan_permute.argtypes = [ctypes.c_int, _ip, _ip, _ip]
         L.sipgpu_measured_clock_mhz.restype = ctypes.c_double
+        L.sipgpu_fill_random.argtypes = [_vp, _vp, ctypes.c_longlong, ctypes.c_ulonglong,
+                                         ctypes.c_ulonglong, ctypes.c_int, ctypes.c_double,
+                                         ctypes.c_double]
         L.sipgpu_measure_peaks.argtypes = [ctypes.POINTER(ctypes.c_double),
                                            ctypes.POINTER(ctypes.c_double), ctypes.c_size_t]
         _lib = L
@@ -232,6 +236,14 @@ def measure_peaks(copy_bytes=1 << 30):
     return m.value, c.value
 
 
+def fill_random(ptr, n, seed, stream, dist="uniform", a=0.0, b=1.0, s=None):
+    """Seeded data in HBM at device address `ptr` (n doubles): the same
+    SplitMix64 sequence as workloads.uniform / workloads.normal."""
+    d = {"uniform": 0, "normal": 1}[dist]
+    _check(lib().sipgpu_fill_random(None if s is None else s.handle, ptr, int(n), int(seed),
+                                    int(stream), d, float(a), float(b)))
+
+
 def measured_clock_mhz():
     """Shader clock (MHz) seen by the last measure_peaks MFMA loop."""
     return lib().sipgpu_measured_clock_mhz()
@@ -268,6 +280,22 @@ class DeviceArray:
         out = np.empty(self.n, dtype=self.dtype)
         _check(lib().sipgpu_memcpy_d2h(out.ctypes.data, self.ptr, self.nbytes))
         return out
+
+    def download_range(self, offset, count):
+        """Elements [offset, offset + count) as a host array."""
+        assert 0 <= offset and offset + count <= self.n
+        out = np.empty(int(count), dtype=self.dtype)
+        _check(lib().sipgpu_memcpy_d2h(out.ctypes.data, self.at(offset),
+                                       int(count) * self.dtype.itemsize))
+        return out
+
+    def upload_range(self, offset, x):
+        x = np.ascontiguousarray(x, dtype=self.dtype).ravel()
+        assert 0 <= offset and offset + x.size <= self.n
+        _check(lib().sipgpu_memcpy_h2d(self.at(offset), x.ctypes.data, x.nbytes))
+
+    def zero(self):
+        self.fill_bytes(0)
 
     def fill_bytes(self, value):
         _check(lib().sipgpu_memset(self.ptr, value, self.nbytes))

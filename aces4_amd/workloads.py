@@ -189,3 +189,112 @@ def mnk(w):
         K = int(np.prod([rd[x] for x in rl if x not in dl] or [1]))
         out.append((M, N, K))
     return out
+
+
+def random_segments(total, lo, hi, rng):
+    """Segment extents drawn i.i.d. uniform in lo..hi until `total` is
+    covered; the last one is truncated to fit."""
+    segs = []
+    while sum(segs) < total:
+        segs.append(int(rng.integers(lo, hi + 1)))
+    segs[-1] -= sum(segs) - total
+    return segs
+
+
+class DeviceBlocks:
+    """A block-sparse array whose blocks live only in HBM.
+
+    Kept blocks (those whose segments' irreps XOR to zero) are packed
+    contiguously, column-major each, in block-id order; block `j` (ordinal
+    among kept blocks) holds the seeded sequence (seed, stream = tag * 2^32 +
+    j) drawn on the device, so any block can be regenerated or read back for
+    checking without a host copy of the array.
+    """
+
+    def __init__(self, name, segs, irreps, seed, tag, dist, a=0.0, b=1.0, zero=False):
+        self.name = name
+        self.segs = [list(s) for s in segs]
+        self.irreps = [list(r) for r in irreps]
+        self.offset, self.ordinal = {}, {}
+        off = 0
+        for blk in self.all_block_ids():
+            if self.kept(blk):
+                self.ordinal[blk] = len(self.offset)
+                self.offset[blk] = off
+                off += int(np.prod(self.dims(blk)))
+        self.size = off
+        self.seed, self.tag, self.dist, self.a, self.b, self.zero = seed, tag, dist, a, b, zero
+        self.dev = None
+
+    def all_block_ids(self):
+        ranges = [range(len(s)) for s in self.segs]
+        for t in itertools.product(*reversed(ranges)):
+            yield tuple(reversed(t))
+
+    def block_ids(self):
+        return iter(self.offset)
+
+    def kept(self, blk):
+        x = 0
+        for d, s in enumerate(blk):
+            x ^= self.irreps[d][s]
+        return x == 0
+
+    def dims(self, blk):
+        return [self.segs[d][blk[d]] for d in range(len(blk))]
+
+    def to_device(self):
+        self.dev = sg.DeviceArray(max(self.size, 1))
+        if self.zero:
+            self.dev.zero()
+        else:
+            for blk, j in self.ordinal.items():
+                sg.fill_random(self.addr(blk), int(np.prod(self.dims(blk))), self.seed,
+                               (self.tag << 32) + j, self.dist, self.a, self.b)
+        sg.synchronize()
+        return self.dev
+
+    def addr(self, blk):
+        return self.dev.at(self.offset[blk])
+
+    def download_block(self, blk, full=None):
+        n = int(np.prod(self.dims(blk)))
+        if full is not None:
+            return full[self.offset[blk]:self.offset[blk] + n]
+        return self.dev.download_range(self.offset[blk], n)
+
+
+def sparse_ladder(o=40, v=200, seg_lo=12, seg_hi=40, probs=(0.75, 1 / 12, 1 / 12, 1 / 12),
+                  seed=3):
+    """Block-sparse ladder R(a,b,i,j) += sum_{c,d} V(a,b,c,d) T(c,d,i,j) with
+    random uneven segments (extents uniform in seg_lo..seg_hi) and one of 4
+    irreps per segment (probabilities `probs`); a block exists iff the XOR of
+    its segments' irreps is 0.  V ~ N(0, 0.1^2), T ~ N(0, 1e-2^2) per block,
+    generated in HBM; R starts at zero."""
+    rng = np.random.default_rng(seed)
+    vs = random_segments(v, seg_lo, seg_hi, rng)
+    os_ = random_segments(o, seg_lo, seg_hi, rng)
+    irr = np.random.default_rng([seed, 1])  # an independent stream for the irreps
+    vi = [int(x) for x in irr.choice(4, size=len(vs), p=probs)]
+    oi = [int(x) for x in irr.choice(4, size=len(os_), p=probs)]
+    V = DeviceBlocks("V", [vs] * 4, [vi] * 4, seed, 1, "normal", 0.1)
+    T = DeviceBlocks("T", [vs, vs, os_, os_], [vi, vi, oi, oi], seed, 2, "normal", 1e-2)
+    R = DeviceBlocks("R", [vs, vs, os_, os_], [vi, vi, oi, oi], seed, 3, "normal", zero=True)
+    labels = ("abij", "abcd", "cdij")
+    code = {"a": 1, "b": 2, "i": 3, "j": 4, "c": 5, "d": 6}
+    aces = [code[x] for x in "".join(labels)]
+    ptrn, ierr = sg.get_contraction_ptrn(4, 4, 4, aces)
+    assert ierr == 0
+    entries = []
+    nv = len(vs)
+    for rb in R.block_ids():
+        a, b, i, j = rb
+        for d in range(nv):
+            for c in range(nv):
+                vb, tb = (a, b, c, d), (c, d, i, j)
+                if vb in V.offset and tb in T.offset:
+                    entries.append((rb, vb, tb))
+    dense = 2.0 * float(v) ** 4 * float(o) ** 2
+    return dict(name="sparse-ladder", D=R, L=V, R=T, entries=entries,
+                ptrn=[int(x) for x in ptrn], labels=labels, v_segs=vs, o_segs=os_,
+                v_irreps=vi, o_irreps=oi, dense_flops=dense)

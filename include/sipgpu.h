@@ -175,6 +175,15 @@ int sipgpu_batch_split_info(sipgpu_batch_t batch, int *nmoduli, int *bits);
 int sipgpu_batch_info(sipgpu_batch_t batch, double *flops, int *ntasks,
                       int *ntiles, int *npairs);
 
+/* Seeded synthetic data written in HBM, enqueued on `s`: n doubles of the
+ * counter-based SplitMix64 sequence (seed, stream).  dist 0: uniform in
+ * [a, b) from the top 53 bits of each output; dist 1: normal with standard
+ * deviation a (Box-Muller over streams 2*stream+1 and 2*stream+2, cosines
+ * into out[0, ceil(n/2)), sines after).  No reference counterpart: it
+ * generates benchmark and test inputs in place. */
+int sipgpu_fill_random(sipgpu_stream_t s, double *out, long long n, unsigned long long seed,
+                       unsigned long long stream, int dist, double a, double b);
+
 /* Measured device ceilings: FP64 MFMA throughput (TFLOP/s) of an MFMA-only
  * loop over all CUs, and HBM bandwidth (GB/s, read + write bytes) of a
  * streaming copy of `copy_bytes` bytes.  Allocates its own buffers. */

@@ -1,0 +1,57 @@
+"""Host-side structure of the synthetic workloads (no device needed)."""
+import itertools
+
+import numpy as np
+import pytest
+
+from aces4_amd import workloads as wl
+
+
+def test_splitmix64_known_values():
+    # SplitMix64 with state 0 advanced once per output: the published first
+    # outputs of the generator seeded with 0
+    z = wl.splitmix64(0, 3)
+    assert [int(x) for x in z] == [0xE220A8397B1DCDAF, 0x6E789E6AA1B965F4, 0x06C45D188009454F]
+
+
+def test_random_segments_cover_range():
+    rng = np.random.default_rng(0)
+    for total in (40, 200, 13):
+        s = wl.random_segments(total, 12, 40, rng)
+        assert sum(s) == total and all(1 <= x <= 40 for x in s)
+        assert all(12 <= x for x in s[:-1])
+
+
+@pytest.fixture(scope="module")
+def sl():
+    # structure only: the pattern comes from the library's host code
+    return wl.sparse_ladder(o=16, v=48, seg_lo=4, seg_hi=12, seed=3)
+
+
+def test_sparse_ladder_structure(sl):
+    V, T, R = sl["L"], sl["R"], sl["D"]
+    vi, oi = sl["v_irreps"], sl["o_irreps"]
+    assert sum(sl["v_segs"]) == 48 and sum(sl["o_segs"]) == 16
+    for blk in V.all_block_ids():
+        assert (blk in V.offset) == (vi[blk[0]] ^ vi[blk[1]] ^ vi[blk[2]] ^ vi[blk[3]] == 0)
+    # every kept (R, V) combination with a kept T appears exactly once
+    want = set()
+    for rb in R.block_ids():
+        a, b, i, j = rb
+        for c, d in itertools.product(range(len(sl["v_segs"])), repeat=2):
+            if (a, b, c, d) in V.offset and (c, d, i, j) in T.offset:
+                want.add((rb, (a, b, c, d), (c, d, i, j)))
+    assert set(sl["entries"]) == want and len(sl["entries"]) == len(want)
+    # packed offsets are disjoint and cover the arena
+    for arr in (V, T, R):
+        spans = sorted((arr.offset[b], int(np.prod(arr.dims(b)))) for b in arr.block_ids())
+        pos = 0
+        for off, n in spans:
+            assert off == pos
+            pos += n
+        assert pos == arr.size
+
+
+def test_sparse_ladder_pattern(sl):
+    # R(a,b,i,j) = V(a,b,c,d) T(c,d,i,j): a->1, b->2, c,d contracted with T's 1,2
+    assert sl["ptrn"] == [1, 2, -1, -2, -3, -4, 3, 4]
