@@ -9,45 +9,53 @@ namespace sipgpu {
 
 constexpr int kOzMaxModuli = 16;
 constexpr int kOzBM = 128, kOzBN = 128; // INT8 GEMM workgroup tile
-constexpr int kOzKAlign = 64;           // K of one v_mfma_i32_16x16x64_i8
+constexpr int kOzChunk = 16;            // bytes of k per panel chunk
+constexpr int kOzStage = 128;           // bytes of k per GEMM pipeline stage
+constexpr int kOzStageChunks = kOzStage / kOzChunk;
+constexpr int kOzZero = 256;            // zero bytes at the start of every plane
+constexpr int kOzTileX = 64, kOzTileK = 64; // conversion tile (rows x k)
+constexpr int kOzRecChunk = 2048;       // result elements per reconstruction workgroup
 
 // Moduli and reconstruction constants (host-computed).
 struct OzConst {
-    int n;                       // number of moduli
-    int P;                       // bits of the scaled integers (|A_int| <= 2^(P-1))
-    int p[kOzMaxModuli];         // pairwise coprime, <= 255
-    int u[kOzMaxModuli];         // (M / p_i)^-1 mod p_i
-    int t24[kOzMaxModuli];       // 2^24 mod p_i
-    double rp[kOzMaxModuli];     // 1 / p_i (rounded)
+    int n;                        // number of moduli
+    int P;                        // bits of the scaled integers (|A_int| <= 2^(P-1))
+    int p[kOzMaxModuli];          // pairwise coprime, <= 255
+    int u[kOzMaxModuli];          // (M / p_i)^-1 mod p_i
+    double rp[kOzMaxModuli];      // 1 / p_i (rounded)
     uint64_t mi[kOzMaxModuli][2]; // M / p_i exactly, {low, high} 64-bit halves
-    uint64_t m[2];               // M = prod p_i exactly (< 2^112)
+    uint64_t m[2];                // M = prod p_i exactly (< 2^112)
 };
 
 // One operand block as the split path sees it: K x X through its tables,
-// converted once per execution into X row-major residue rows of Kp bytes.
+// converted once per execution into X row-major residue rows of Kp bytes
+// (K rounded up to a whole chunk, zero filled).
 struct OzBlock {
     const double *data;
     int K, X, Kp;
     int ktab, xtab;        // offsets of its k / x tables in the pool
     int exp_off;           // first exponent of its row component
+    int xfast;             // 1: consecutive rows are adjacent in memory
     int pad;
     long long panel_off;   // byte offset inside each modulus plane
 };
 
-struct OzPair {
-    int a, b; // OzBlock indices of the left and right operand
-    int Kp;
-    int pad;
+// 16 bytes of a task's virtual K: where the left and right operand rows
+// hold them (plane byte offsets of row 0; row r adds r * stride).
+struct OzChunkRef {
+    uint32_t a, b;
+    int32_t stride;
+    int32_t pad;
 };
 
 struct OzTask {
     double *D;
     int M, N;
     int mD, nD;           // destination tables
-    int first, npairs;
+    int chunk0, nchunks;  // virtual K: chunks [chunk0, chunk0 + nchunks), whole stages
     int beta;
     int expA, expB;       // exponent offsets of its row / column components
-    int pad;
+    int nfast;            // 1: result elements ordered with n fastest
     long long res_off;    // element offset inside each result plane
 };
 
@@ -55,38 +63,57 @@ struct OzTile {
     int task, mt, nt, pad;
 };
 
+// Conversion work item: rows [x0, x0 + 64) and k [k0, k0 + 64) of a block.
+struct OzConvTile {
+    int block, x0, k0, pad;
+};
+
+// Reconstruction work item: elements [e0, e0 + kOzRecChunk) of a task.
+struct OzRecItem {
+    int task, e0;
+};
+
 struct OzGemmArgs {
     const OzTile *tiles;
     const OzTask *tasks;
-    const OzPair *pairs;
-    const OzBlock *blocks;
+    const OzChunkRef *chunks;
     const int8_t *planes;
     long long plane_bytes;
-    int8_t *res_planes;
+    uint8_t *res_planes;
     long long res_plane_bytes;
     int ntiles;
     OzConst c;
 };
 
+struct OzConvArgs {
+    const OzBlock *blocks;
+    const OzConvTile *items;
+    int nitems;
+    const int32_t *tab;
+    int *expo;
+    int8_t *planes;
+    long long plane_bytes;
+    OzConst c;
+};
+
 struct OzRecArgs {
     const OzTask *tasks;
-    int ntasks;
-    const long long *elem_start; // prefix of M*N over tasks (ntasks + 1)
-    long long nelems;
+    const OzRecItem *items;
+    int nitems;
     const int32_t *tab;
     const int *expo;
-    const int8_t *res_planes;
+    const uint8_t *res_planes;
     long long res_plane_bytes;
     OzConst c;
 };
 
 hipError_t oz_fill_int(int *p, int n, int v, hipStream_t s);
-hipError_t oz_row_exponents(const OzBlock *blocks, int nblocks, const int32_t *tab, int *expo,
-                            const int *row_start, int nrows, hipStream_t s);
-hipError_t oz_to_residues(const OzBlock *blocks, int nblocks, const int32_t *tab, const int *expo,
-                          const int *quad_start, int nquads, int8_t *planes,
-                          long long plane_bytes, const OzConst &c, hipStream_t s);
-hipError_t oz_gemm(const OzGemmArgs &a, int grid, hipStream_t s);
+// Row maxima -> exponents, one workgroup per 64-row tile of a block
+// (`row_items`, all with k0 = 0); each sweeps the block's whole K.
+hipError_t oz_row_exponents(const OzConvArgs &a, const OzConvTile *row_items, int nrow_items,
+                            hipStream_t s);
+hipError_t oz_to_residues(const OzConvArgs &a, hipStream_t s);
+hipError_t oz_gemm(const OzGemmArgs &a, hipStream_t s);
 hipError_t oz_reconstruct(const OzRecArgs &a, hipStream_t s);
 
 } // namespace sipgpu

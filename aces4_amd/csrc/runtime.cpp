@@ -24,24 +24,23 @@
 #include <vector>
 
 #include "../../include/sipgpu.h"
+#include "internal.h"
 #include "kernels.h"
 #include "ozaki.h"
 #include "plan.h"
 
 using namespace sipgpu;
 
-struct sipgpu_stream_s {
-    hipStream_t s = nullptr;
-    double *scratch = nullptr; // kDotBlocks partial sums of contract-to-scalar
-};
 struct sipgpu_event_s {
     hipEvent_t e = nullptr;
 };
 
 namespace {
-
 thread_local std::string g_last_error;
-double g_last_clock_mhz = 0.0; // shader clock seen by the last MFMA calibration
+} // namespace
+
+namespace sipgpu {
+namespace rt {
 
 int fail(int code, const std::string &msg) {
     g_last_error = msg;
@@ -51,6 +50,16 @@ int fail(int code, const std::string &msg) {
 int hip_fail(hipError_t e, const char *what) {
     return fail(SIPGPU_EDEVICE, std::string(what) + ": " + hipGetErrorString(e));
 }
+
+} // namespace rt
+} // namespace sipgpu
+
+namespace {
+
+using sipgpu::rt::BatchImage;
+using sipgpu::rt::fail;
+using sipgpu::rt::hip_fail;
+double g_last_clock_mhz = 0.0; // shader clock seen by the last MFMA calibration
 
 #define HIP_TRY(expr)                                                                     \
     do {                                                                                  \
@@ -257,15 +266,11 @@ struct SplitData {
     int nexpo = 0;
     int8_t *planes = nullptr;
     long long plane_bytes = 0;
-    int8_t *res_planes = nullptr;
+    uint8_t *res_planes = nullptr;
     long long res_plane_bytes = 0;
-    const OzBlock *blocks = nullptr;
-    int nblocks = 0;
-    const int32_t *tab = nullptr;
-    const int *row_start = nullptr;
-    int nrows = 0;
-    const int *quad_start = nullptr;
-    int nquads = 0;
+    const OzConvTile *row_items = nullptr;
+    int nrow_items = 0;
+    OzConvArgs cargs{};
     OzGemmArgs gargs{};
     OzRecArgs rargs{};
 };
@@ -349,7 +354,9 @@ int check_hazards(const sipgpu_contraction *list, int n, const ShapeVec &shapes,
     return 0;
 }
 
-int plan_batch_locked(Context &c, const sipgpu_contraction *list, int n, sipgpu_batch_s &b) {
+} // namespace
+
+int sipgpu::rt::build_batch_image(const sipgpu_contraction *list, int n, BatchImage &b) {
     struct Entry {
         const sipgpu_contraction *e;
         std::string key;
@@ -487,20 +494,33 @@ int plan_batch_locked(Context &c, const sipgpu_contraction *list, int n, sipgpu_
     std::memcpy(host.data() + o_tasks, tasks.data(), tasks.size() * sizeof(GemmTask));
     std::memcpy(host.data() + o_pairs, pairs.data(), pairs.size() * sizeof(GemmPair));
     std::memcpy(host.data() + o_tab, pool.data(), pool.size() * sizeof(int32_t));
-    HIP_TRY(hipMalloc(&b.dev, total));
-    HIP_TRY(hipMemcpy(b.dev, host.data(), total, hipMemcpyHostToDevice));
-    unsigned char *base = static_cast<unsigned char *>(b.dev);
-    b.args = GemmArgs{};
-    b.args.tiles = reinterpret_cast<const GemmTile *>(base + o_tiles);
-    b.args.tasks = reinterpret_cast<const GemmTask *>(base + o_tasks);
-    b.args.pairs = reinterpret_cast<const GemmPair *>(base + o_pairs);
-    b.args.tab = reinterpret_cast<const int32_t *>(base + o_tab);
-    b.args.ntiles = (int)tiles.size();
+    b.host = std::move(host);
+    b.o_tiles = o_tiles;
+    b.o_tasks = o_tasks;
+    b.o_pairs = o_pairs;
+    b.o_tab = o_tab;
     b.ntiles = (int)tiles.size();
     b.ntasks = ntasks;
     b.npairs = (int)pairs.size();
     (void)maxM;
     (void)maxN;
+    return 0;
+}
+
+namespace {
+
+int plan_batch_locked(Context &, const sipgpu_contraction *list, int n, sipgpu_batch_s &b) {
+    BatchImage img;
+    if (int rc = sipgpu::rt::build_batch_image(list, n, img)) return rc;
+    HIP_TRY(hipMalloc(&b.dev, img.host.size()));
+    HIP_TRY(hipMemcpy(b.dev, img.host.data(), img.host.size(), hipMemcpyHostToDevice));
+    b.args = img.args_at(b.dev);
+    b.cfg = img.cfg;
+    b.flops = img.flops;
+    b.ntiles = img.ntiles;
+    b.ntasks = img.ntasks;
+    b.npairs = img.npairs;
+    b.real_tiles = img.real_tiles;
     return 0;
 }
 
@@ -540,7 +560,6 @@ int oz_constants(int n, long long kmax, OzConst &c) {
         for (int j = 0; j < n; ++j)
             if (j != i) mi = (mi * (c.p[j] % p)) % p;
         c.u[i] = inv_mod((int)mi, p);
-        c.t24[i] = (int)((1LL << 24) % p);
         c.rp[i] = 1.0 / p;
         const unsigned __int128 Mi = M / (unsigned __int128)p;
         c.mi[i][0] = (uint64_t)Mi;
@@ -566,7 +585,7 @@ struct UnionFind {
     void unite(int a, int b) { parent[find(a)] = find(b); }
 };
 
-int plan_split_locked(Context &c, const sipgpu_contraction *list, int n, int nmod,
+int plan_split_locked(Context &, const sipgpu_contraction *list, int n, int nmod,
                       sipgpu_batch_s &b) {
     if (nmod < kOzMinModuli || nmod > kOzMaxUsable)
         return fail(SIPGPU_EARG, "split path: nmoduli must be in 6..14");
@@ -602,6 +621,7 @@ int plan_split_locked(Context &c, const sipgpu_contraction *list, int n, int nmo
     std::vector<int32_t> &pool = tp.pool;
     std::vector<std::array<int, 6>> tabs(shapes.size());
     for (size_t si = 0; si < shapes.size(); ++si) tabs[si] = tp.add_shape(*shapes[si]);
+    auto unit_stride = [&](int off, int len) { return len > 1 && pool[off + 1] - pool[off] == 1; };
     // operand instances: one residue panel per (block, k table, x table)
     std::map<std::tuple<const double *, int, int, int, int>, int> inst;
     std::vector<OzBlock> blocks;
@@ -616,9 +636,10 @@ int plan_split_locked(Context &c, const sipgpu_contraction *list, int n, int nmo
         ob.data = ptr;
         ob.K = (int)sh.K;
         ob.X = X;
-        ob.Kp = (int)((sh.K + kOzKAlign - 1) / kOzKAlign * kOzKAlign);
+        ob.Kp = (int)((sh.K + kOzChunk - 1) / kOzChunk * kOzChunk);
         ob.ktab = ktab;
         ob.xtab = xtab;
+        ob.xfast = unit_stride(xtab, X) ? 1 : 0;
         blocks.push_back(ob);
         inst.emplace(key, (int)blocks.size() - 1);
         return (int)blocks.size() - 1;
@@ -631,11 +652,12 @@ int plan_split_locked(Context &c, const sipgpu_contraction *list, int n, int nmo
         if (v.empty()) dest_order.push_back(list[i].d);
         v.push_back(i);
     }
+    struct TaskPairs { std::vector<std::pair<int, int>> ab; };
     std::vector<OzTask> tasks;
-    std::vector<OzPair> pairs;
-    std::vector<int> task_l, task_r; // one operand instance per task (for components)
+    std::vector<TaskPairs> task_pairs;
     std::vector<double> task_cost;
     long long kmax = 0;
+    int npairs = 0;
     b.flops = 0.0;
     for (const double *dp : dest_order) {
         const auto &idx = by_dest[dp];
@@ -650,8 +672,9 @@ int plan_split_locked(Context &c, const sipgpu_contraction *list, int n, int nmo
         t.N = (int)s0.N;
         t.mD = tabs[si0][4];
         t.nD = tabs[si0][5];
-        t.first = (int)pairs.size();
         t.beta = list[idx[start]].accumulate ? 1 : 0;
+        t.nfast = (unit_stride(t.nD, t.N) || t.M == 1) ? 1 : 0;
+        TaskPairs tpairs;
         long long ktot = 0, kptot = 0;
         double cost = 0.0;
         for (size_t j = start; j < idx.size(); ++j) {
@@ -660,31 +683,32 @@ int plan_split_locked(Context &c, const sipgpu_contraction *list, int n, int nmo
             const ContractionShape &sh = *shapes[si];
             if (sh.M != s0.M || sh.N != s0.N || sh.moffD != s0.moffD || sh.noffD != s0.noffD)
                 return fail(SIPGPU_EARG, "entries writing one block disagree on its layout");
-            OzPair pr{};
-            pr.a = operand(list[i].l, si, 0);
-            pr.b = operand(list[i].r, si, 1);
-            pr.Kp = blocks[pr.a].Kp;
-            pairs.push_back(pr);
+            const int ia = operand(list[i].l, si, 0), ib = operand(list[i].r, si, 1);
+            tpairs.ab.emplace_back(ia, ib);
             ktot += sh.K;
-            kptot += pr.Kp;
+            kptot += blocks[ia].Kp;
             cost += 2.0 * (double)sh.M * (double)sh.N * (double)sh.K;
         }
-        if (kptot > 131072)
+        // exact int32 accumulation: sum of |a b| <= Kvirtual * 127^2 < 2^31
+        if (kptot + kOzStage > 131072)
             return fail(SIPGPU_ESIZE, "split path: a result block sums more than 131072 k");
-        t.npairs = (int)pairs.size() - t.first;
+        if ((long long)t.M * t.N >= INT_MAX)
+            return fail(SIPGPU_ESIZE, "split path: result block too large");
+        npairs += (int)tpairs.ab.size();
         kmax = std::max(kmax, ktot);
         b.flops += cost;
         tasks.push_back(t);
+        task_pairs.push_back(std::move(tpairs));
         task_cost.push_back(cost);
     }
     const int ntasks = (int)tasks.size();
     // row / column components: operands of one task share its rows (columns)
     const int nb = (int)blocks.size();
     UnionFind uf(nb);
-    for (const OzTask &t : tasks)
-        for (int q = t.first + 1; q < t.first + t.npairs; ++q) {
-            uf.unite(pairs[t.first].a, pairs[q].a);
-            uf.unite(pairs[t.first].b, pairs[q].b);
+    for (const TaskPairs &tpp : task_pairs)
+        for (size_t q = 1; q < tpp.ab.size(); ++q) {
+            uf.unite(tpp.ab[0].first, tpp.ab[q].first);
+            uf.unite(tpp.ab[0].second, tpp.ab[q].second);
         }
     std::unordered_map<int, int> comp_off;
     int nexpo = 0;
@@ -699,31 +723,48 @@ int plan_split_locked(Context &c, const sipgpu_contraction *list, int n, int nmo
             return fail(SIPGPU_EARG, "split path: operands sharing rows disagree on extents");
         blocks[i].exp_off = it->second;
     }
-    for (OzTask &t : tasks) {
-        t.expA = blocks[pairs[t.first].a].exp_off;
-        t.expB = blocks[pairs[t.first].b].exp_off;
+    for (int t = 0; t < ntasks; ++t) {
+        tasks[t].expA = blocks[task_pairs[t].ab[0].first].exp_off;
+        tasks[t].expB = blocks[task_pairs[t].ab[0].second].exp_off;
     }
-    // panels and result planes
-    long long panel = 0;
-    std::vector<int> row_start(nb + 1), quad_start(nb + 1);
-    long long rows = 0, quads = 0;
+    // panels (after the plane's zero bytes), conversion work
+    long long panel = kOzZero;
+    std::vector<OzConvTile> conv, rows;
     for (int i = 0; i < nb; ++i) {
         blocks[i].panel_off = panel;
         panel += ((long long)blocks[i].X * blocks[i].Kp + 255) / 256 * 256;
-        row_start[i] = (int)rows;
-        quad_start[i] = (int)quads;
-        rows += blocks[i].X;
-        quads += (long long)blocks[i].X * (blocks[i].Kp / 4);
+        for (int x0 = 0; x0 < blocks[i].X; x0 += kOzTileX) {
+            rows.push_back(OzConvTile{i, x0, 0, 0});
+            for (int k0 = 0; k0 < blocks[i].Kp; k0 += kOzTileK)
+                conv.push_back(OzConvTile{i, x0, k0, 0});
+        }
     }
-    if (quads >= INT_MAX) return fail(SIPGPU_ESIZE, "split path: operands too large");
-    row_start[nb] = (int)rows;
-    quad_start[nb] = (int)quads;
-    std::vector<long long> elem_start(ntasks + 1, 0);
+    if (panel >= (1LL << 32) - 65536)
+        return fail(SIPGPU_ESIZE, "split path: operand panels exceed 4 GiB per modulus");
+    // virtual K of every task: its pairs' panel chunks, padded to whole
+    // stages with chunks of the zero bytes
+    std::vector<OzChunkRef> chunks;
     for (int t = 0; t < ntasks; ++t) {
-        tasks[t].res_off = elem_start[t];
-        elem_start[t + 1] = elem_start[t] + (long long)tasks[t].M * tasks[t].N;
+        tasks[t].chunk0 = (int)chunks.size();
+        for (const auto &ab : task_pairs[t].ab) {
+            const OzBlock &A = blocks[ab.first], &B = blocks[ab.second];
+            for (int k = 0; k < A.Kp; k += kOzChunk)
+                chunks.push_back(OzChunkRef{(uint32_t)(A.panel_off + k),
+                                            (uint32_t)(B.panel_off + k), A.Kp, 0});
+        }
+        while ((chunks.size() - tasks[t].chunk0) % kOzStageChunks)
+            chunks.push_back(OzChunkRef{0u, 0u, 0, 0});
+        tasks[t].nchunks = (int)chunks.size() - tasks[t].chunk0;
     }
-    const long long res_elems = elem_start[ntasks];
+    // result planes, reconstruction work
+    long long res_elems = 0;
+    std::vector<OzRecItem> rec;
+    for (int t = 0; t < ntasks; ++t) {
+        tasks[t].res_off = res_elems;
+        const long long mn = (long long)tasks[t].M * tasks[t].N;
+        for (long long e0 = 0; e0 < mn; e0 += kOzRecChunk) rec.push_back(OzRecItem{t, (int)e0});
+        res_elems += mn;
+    }
     // tiles: whole tasks to XCDs, heaviest first (as for the FP64 batch)
     std::vector<int> order(ntasks);
     std::iota(order.begin(), order.end(), 0);
@@ -745,56 +786,68 @@ int plan_split_locked(Context &c, const sipgpu_contraction *list, int n, int nmo
     std::vector<OzTile> tiles(per * kXcds, OzTile{-1, 0, 0, 0});
     for (int x = 0; x < kXcds; ++x)
         for (size_t j = 0; j < per_xcd[x].size(); ++j) tiles[j * kXcds + x] = per_xcd[x][j];
+    if (tiles.size() >= (size_t)INT_MAX) return fail(SIPGPU_ESIZE, "split path: too many tiles");
 
     OzConst oc;
     const int P = oz_constants(nmod, kmax, oc);
     if (P < 8) return fail(SIPGPU_EARG, "split path: too few moduli for this K");
 
-    // one work allocation
+    // one descriptor allocation
     auto align = [](size_t v) { return (v + 255) & ~(size_t)255; };
     size_t off = 0;
     const size_t o_blocks = off; off = align(off + blocks.size() * sizeof(OzBlock));
-    const size_t o_pairs = off; off = align(off + pairs.size() * sizeof(OzPair));
     const size_t o_tasks = off; off = align(off + tasks.size() * sizeof(OzTask));
     const size_t o_tiles = off; off = align(off + tiles.size() * sizeof(OzTile));
+    const size_t o_chunks = off; off = align(off + chunks.size() * sizeof(OzChunkRef));
     const size_t o_tab = off; off = align(off + pool.size() * sizeof(int32_t));
-    const size_t o_rows = off; off = align(off + row_start.size() * sizeof(int));
-    const size_t o_quads = off; off = align(off + quad_start.size() * sizeof(int));
-    const size_t o_elems = off; off = align(off + elem_start.size() * sizeof(long long));
+    const size_t o_conv = off; off = align(off + conv.size() * sizeof(OzConvTile));
+    const size_t o_rows = off; off = align(off + rows.size() * sizeof(OzConvTile));
+    const size_t o_rec = off; off = align(off + rec.size() * sizeof(OzRecItem));
     const size_t o_expo = off; off = align(off + (size_t)nexpo * sizeof(int));
     const size_t total = off + 256;
     std::vector<unsigned char> host(total, 0);
-    std::memcpy(host.data() + o_blocks, blocks.data(), blocks.size() * sizeof(OzBlock));
-    std::memcpy(host.data() + o_pairs, pairs.data(), pairs.size() * sizeof(OzPair));
-    std::memcpy(host.data() + o_tasks, tasks.data(), tasks.size() * sizeof(OzTask));
-    std::memcpy(host.data() + o_tiles, tiles.data(), tiles.size() * sizeof(OzTile));
-    std::memcpy(host.data() + o_tab, pool.data(), pool.size() * sizeof(int32_t));
-    std::memcpy(host.data() + o_rows, row_start.data(), row_start.size() * sizeof(int));
-    std::memcpy(host.data() + o_quads, quad_start.data(), quad_start.size() * sizeof(int));
-    std::memcpy(host.data() + o_elems, elem_start.data(), elem_start.size() * sizeof(long long));
+    auto put = [&](size_t o, const void *src, size_t bytes) {
+        if (bytes) std::memcpy(host.data() + o, src, bytes);
+    };
+    put(o_blocks, blocks.data(), blocks.size() * sizeof(OzBlock));
+    put(o_tasks, tasks.data(), tasks.size() * sizeof(OzTask));
+    put(o_tiles, tiles.data(), tiles.size() * sizeof(OzTile));
+    put(o_chunks, chunks.data(), chunks.size() * sizeof(OzChunkRef));
+    put(o_tab, pool.data(), pool.size() * sizeof(int32_t));
+    put(o_conv, conv.data(), conv.size() * sizeof(OzConvTile));
+    put(o_rows, rows.data(), rows.size() * sizeof(OzConvTile));
+    put(o_rec, rec.data(), rec.size() * sizeof(OzRecItem));
     HIP_TRY(hipMalloc(&b.dev, total));
     HIP_TRY(hipMemcpy(b.dev, host.data(), total, hipMemcpyHostToDevice));
     unsigned char *base = static_cast<unsigned char *>(b.dev);
     SplitData &sd = b.sd;
     sd.c = oc;
-    sd.plane_bytes = std::max<long long>(panel, 256);
-    sd.res_plane_bytes = (res_elems + 255) / 256 * 256;
+    sd.plane_bytes = (panel + 255) / 256 * 256;
+    sd.res_plane_bytes = std::max<long long>((res_elems + 255) / 256 * 256, 256);
     HIP_TRY(hipMalloc(&sd.planes, (size_t)sd.plane_bytes * nmod));
+    // the zero bytes at the start of each plane are never written again
+    HIP_TRY(hipMemset(sd.planes, 0, (size_t)sd.plane_bytes * nmod));
+    HIP_TRY(hipDeviceSynchronize());
     HIP_TRY(hipMalloc(&sd.res_planes, (size_t)sd.res_plane_bytes * nmod));
     sd.expo = reinterpret_cast<int *>(base + o_expo);
     sd.nexpo = nexpo;
-    sd.blocks = reinterpret_cast<const OzBlock *>(base + o_blocks);
-    sd.nblocks = nb;
-    sd.tab = reinterpret_cast<const int32_t *>(base + o_tab);
-    sd.row_start = reinterpret_cast<const int *>(base + o_rows);
-    sd.nrows = (int)rows;
-    sd.quad_start = reinterpret_cast<const int *>(base + o_quads);
-    sd.nquads = (int)quads;
+    sd.row_items = reinterpret_cast<const OzConvTile *>(base + o_rows);
+    sd.nrow_items = (int)rows.size();
+    const int32_t *dtab = reinterpret_cast<const int32_t *>(base + o_tab);
+    const OzTask *dtasks = reinterpret_cast<const OzTask *>(base + o_tasks);
+    OzConvArgs &ca = sd.cargs;
+    ca.blocks = reinterpret_cast<const OzBlock *>(base + o_blocks);
+    ca.items = reinterpret_cast<const OzConvTile *>(base + o_conv);
+    ca.nitems = (int)conv.size();
+    ca.tab = dtab;
+    ca.expo = sd.expo;
+    ca.planes = sd.planes;
+    ca.plane_bytes = sd.plane_bytes;
+    ca.c = oc;
     OzGemmArgs &g = sd.gargs;
     g.tiles = reinterpret_cast<const OzTile *>(base + o_tiles);
-    g.tasks = reinterpret_cast<const OzTask *>(base + o_tasks);
-    g.pairs = reinterpret_cast<const OzPair *>(base + o_pairs);
-    g.blocks = sd.blocks;
+    g.tasks = dtasks;
+    g.chunks = reinterpret_cast<const OzChunkRef *>(base + o_chunks);
     g.planes = sd.planes;
     g.plane_bytes = sd.plane_bytes;
     g.res_planes = sd.res_planes;
@@ -802,11 +855,10 @@ int plan_split_locked(Context &c, const sipgpu_contraction *list, int n, int nmo
     g.ntiles = (int)tiles.size();
     g.c = oc;
     OzRecArgs &r = sd.rargs;
-    r.tasks = g.tasks;
-    r.ntasks = ntasks;
-    r.elem_start = reinterpret_cast<const long long *>(base + o_elems);
-    r.nelems = res_elems;
-    r.tab = sd.tab;
+    r.tasks = dtasks;
+    r.items = reinterpret_cast<const OzRecItem *>(base + o_rec);
+    r.nitems = (int)rec.size();
+    r.tab = dtab;
     r.expo = sd.expo;
     r.res_planes = sd.res_planes;
     r.res_plane_bytes = sd.res_plane_bytes;
@@ -814,7 +866,7 @@ int plan_split_locked(Context &c, const sipgpu_contraction *list, int n, int nmo
     b.split = true;
     b.ntiles = (int)tiles.size();
     b.ntasks = ntasks;
-    b.npairs = (int)pairs.size();
+    b.npairs = npairs;
     b.real_tiles = real_tiles;
     return 0;
 }
@@ -822,15 +874,24 @@ int plan_split_locked(Context &c, const sipgpu_contraction *list, int n, int nmo
 int execute_split(sipgpu_batch_s &b, hipStream_t s) {
     SplitData &sd = b.sd;
     HIP_TRY(oz_fill_int(sd.expo, sd.nexpo, INT_MIN, s));
-    HIP_TRY(oz_row_exponents(sd.blocks, sd.nblocks, sd.tab, sd.expo, sd.row_start, sd.nrows, s));
-    HIP_TRY(oz_to_residues(sd.blocks, sd.nblocks, sd.tab, sd.expo, sd.quad_start, sd.nquads,
-                           sd.planes, sd.plane_bytes, sd.c, s));
-    HIP_TRY(oz_gemm(sd.gargs, sd.gargs.ntiles, s));
+    HIP_TRY(oz_row_exponents(sd.cargs, sd.row_items, sd.nrow_items, s));
+    HIP_TRY(oz_to_residues(sd.cargs, s));
+    HIP_TRY(oz_gemm(sd.gargs, s));
     HIP_TRY(oz_reconstruct(sd.rargs, s));
     return 0;
 }
 
 } // namespace
+
+int sipgpu::rt::ensure_ready() {
+    Context &c = ctx();
+    std::lock_guard<std::mutex> g(c.mu);
+    return ensure_init(c);
+}
+
+int sipgpu::rt::device_id() { return ctx().device; }
+
+hipStream_t sipgpu::rt::stream_of(sipgpu_stream_t s) { return s ? s->s : ctx().stream.s; }
 
 extern "C" {
 

@@ -1,0 +1,741 @@
+// serve.cpp -- device block cache and continuous-batching contraction server.
+//
+// Block cache (the SIP's BlockManager / CachedBlockMap / LRUArrayPolicy
+// role, block_manager.cpp:106-206, cached_block_map.cpp:31-147,
+// lru_array_policy.h:37-96): blocks are named by 64-bit keys, live in one
+// HBM arena of a fixed budget, and are evicted least-recently-used first
+// when a request does not fit.  A block whose device copy is newer than its
+// host copy is written back first (spilled) to its host home, or to pinned
+// host memory the cache allocates for it.  Eviction is per block (the
+// reference evicts whole arrays), and never touches a block in use.
+//
+// Server: requests (one block contraction each, blocks named by cache
+// keys) are queued by any thread; a scheduler thread takes everything that
+// has arrived (up to a batch limit and half the cache budget of pinned
+// blocks, stopping at the first request that would read a block an earlier
+// one in the batch writes, or vice versa),
+// makes its blocks resident, and launches the batch as one grouped FP64
+// MFMA launch, its work list uploaded through a small ring of descriptor
+// slots (no allocation per batch).  A completion thread waits for each
+// batch's event and stamps its requests' completion times.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <list>
+#include <map>
+#include <mutex>
+#include <thread>
+#include <unordered_map>
+#include <unordered_set>
+#include <vector>
+
+#include "../../include/sipgpu.h"
+#include "internal.h"
+#include "kernels.h"
+
+using sipgpu::rt::fail;
+using sipgpu::rt::hip_fail;
+using sipgpu::rt::stream_of;
+
+namespace {
+
+using Clock = std::chrono::steady_clock;
+
+// First-fit sub-allocator over one device region, 256-byte granules.
+struct Arena {
+    char *base = nullptr;
+    size_t size = 0;
+    std::map<size_t, size_t> free_ranges; // offset -> length
+    size_t used = 0;
+
+    void reset(char *b, size_t n) {
+        base = b;
+        size = n;
+        free_ranges.clear();
+        if (n) free_ranges.emplace(0, n);
+        used = 0;
+    }
+    static size_t round(size_t b) { return (std::max<size_t>(b, 1) + 255) & ~(size_t)255; }
+    bool alloc(size_t bytes, size_t &off) {
+        const size_t need = round(bytes);
+        for (auto it = free_ranges.begin(); it != free_ranges.end(); ++it) {
+            if (it->second < need) continue;
+            off = it->first;
+            const size_t rest = it->second - need;
+            free_ranges.erase(it);
+            if (rest) free_ranges.emplace(off + need, rest);
+            used += need;
+            return true;
+        }
+        return false;
+    }
+    void release(size_t off, size_t bytes) {
+        const size_t len = round(bytes);
+        used -= len;
+        auto next = free_ranges.lower_bound(off);
+        size_t lo = off, hi = off + len;
+        if (next != free_ranges.begin()) {
+            auto prev = std::prev(next);
+            if (prev->first + prev->second == lo) {
+                lo = prev->first;
+                free_ranges.erase(prev);
+            }
+        }
+        next = free_ranges.lower_bound(hi);
+        if (next != free_ranges.end() && next->first == hi) {
+            hi += next->second;
+            free_ranges.erase(next);
+        }
+        free_ranges.emplace(lo, hi - lo);
+    }
+};
+
+constexpr size_t kNotResident = ~(size_t)0;
+
+struct CBlock {
+    size_t elems = 0;
+    size_t off = kNotResident;     // arena offset while resident
+    double *home = nullptr;        // host copy: the caller's, or a pinned spill buffer
+    bool home_owned = false;
+    bool home_valid = false;       // home holds the block's current value
+    int init = SIPGPU_CACHE_UNDEFINED;
+    unsigned long long seed = 0, stream = 0;
+    double a = 0.0, b = 0.0;
+    bool initialised = false;      // contents defined (by upload, fill or a write)
+    int pins = 0;
+    hipEvent_t last_use = nullptr; // completes when no launched work uses the device copy
+    bool used_on_device = false;
+    std::list<unsigned long long>::iterator lru_it;
+    bool in_lru = false;
+};
+
+} // namespace
+
+struct sipgpu_cache_s {
+    std::mutex mu;
+    Arena arena;
+    void *dev = nullptr;
+    hipStream_t copy = nullptr;                 // spill (device -> host) copies
+    std::unordered_map<unsigned long long, CBlock> blocks;
+    std::list<unsigned long long> lru;          // front: least recently released
+    std::vector<hipEvent_t> fences;             // spills whose copies may be in flight
+    sipgpu_cache_stats st{};
+};
+
+namespace {
+
+int make_event(hipEvent_t *e) {
+    SIPGPU_HIP_TRY(hipEventCreateWithFlags(e, hipEventDisableTiming));
+    return 0;
+}
+
+// Write back (if needed) and drop the device copy of an unpinned block.
+// The arena range is reusable once the returned fence (if any) completes.
+int evict_locked(sipgpu_cache_s &c, unsigned long long key, CBlock &b) {
+    if (b.in_lru) {
+        c.lru.erase(b.lru_it);
+        b.in_lru = false;
+    }
+    const size_t bytes = b.elems * sizeof(double);
+    if (b.initialised && !b.home_valid) {
+        if (!b.home) {
+            hipError_t e = hipHostMalloc((void **)&b.home, std::max<size_t>(bytes, 8),
+                                         hipHostMallocDefault);
+            if (e != hipSuccess) return fail(SIPGPU_ENOMEM, "cache: pinned spill buffer");
+            b.home_owned = true;
+        }
+        if (b.used_on_device) SIPGPU_HIP_TRY(hipStreamWaitEvent(c.copy, b.last_use, 0));
+        SIPGPU_HIP_TRY(hipMemcpyAsync(b.home, c.arena.base + b.off, bytes, hipMemcpyDeviceToHost,
+                                      c.copy));
+        hipEvent_t f;
+        if (int rc = make_event(&f)) return rc;
+        SIPGPU_HIP_TRY(hipEventRecord(f, c.copy));
+        c.fences.push_back(f);
+        b.home_valid = true;
+        c.st.spills += 1;
+        c.st.bytes_d2h += bytes;
+    } else if (b.used_on_device) {
+        // a later upload into this range must follow the last kernel reading it
+        hipEvent_t f;
+        if (int rc = make_event(&f)) return rc;
+        SIPGPU_HIP_TRY(hipStreamWaitEvent(c.copy, b.last_use, 0));
+        SIPGPU_HIP_TRY(hipEventRecord(f, c.copy));
+        c.fences.push_back(f);
+    }
+    c.arena.release(b.off, bytes);
+    b.off = kNotResident;
+    c.st.evictions += 1;
+    (void)key;
+    return 0;
+}
+
+// Make `stream` wait for every spill or last use that freed arena space.
+int fence_locked(sipgpu_cache_s &c, hipStream_t stream) {
+    std::vector<hipEvent_t> keep;
+    for (hipEvent_t f : c.fences) {
+        if (hipEventQuery(f) == hipSuccess) {
+            (void)hipEventDestroy(f);
+            continue;
+        }
+        SIPGPU_HIP_TRY(hipStreamWaitEvent(stream, f, 0));
+        keep.push_back(f);
+    }
+    c.fences.swap(keep);
+    return 0;
+}
+
+int reserve_locked(sipgpu_cache_s &c, size_t bytes, size_t &off) {
+    while (!c.arena.alloc(bytes, off)) {
+        if (c.lru.empty())
+            return fail(SIPGPU_EBUDGET, "cache: blocks in use exceed the HBM budget");
+        const unsigned long long victim = c.lru.front();
+        if (int rc = evict_locked(c, victim, c.blocks[victim])) return rc;
+    }
+    return 0;
+}
+
+int acquire_locked(sipgpu_cache_s &c, hipStream_t stream, unsigned long long key, int mode,
+                   double **dev) {
+    auto it = c.blocks.find(key);
+    if (it == c.blocks.end()) return fail(SIPGPU_ENOTFOUND, "cache: unknown block key");
+    CBlock &b = it->second;
+    const size_t bytes = b.elems * sizeof(double);
+    if (b.off == kNotResident) {
+        size_t off;
+        if (int rc = reserve_locked(c, bytes, off)) return rc;
+        b.off = off;
+        if (int rc = fence_locked(c, stream)) return rc;
+        double *d = reinterpret_cast<double *>(c.arena.base + off);
+        const bool need_value = mode != SIPGPU_CACHE_WRITE;
+        if (b.initialised && b.home_valid) {
+            if (need_value) {
+                SIPGPU_HIP_TRY(hipMemcpyAsync(d, b.home, bytes, hipMemcpyHostToDevice, stream));
+                c.st.bytes_h2d += bytes;
+            }
+        } else if (!b.initialised) {
+            if (b.init == SIPGPU_CACHE_ZERO) {
+                if (need_value) SIPGPU_HIP_TRY(hipMemsetAsync(d, 0, bytes, stream));
+            } else if (b.init == SIPGPU_CACHE_UNIFORM || b.init == SIPGPU_CACHE_NORMAL) {
+                SIPGPU_HIP_TRY(sipgpu::launch_fill_random(
+                    d, (long long)b.elems, b.seed, b.stream,
+                    b.init == SIPGPU_CACHE_UNIFORM ? 0 : 1, b.a, b.b, stream));
+            }
+            b.initialised = true;
+        }
+        c.st.misses += 1;
+    } else {
+        c.st.hits += 1;
+    }
+    if (b.in_lru) {
+        c.lru.erase(b.lru_it);
+        b.in_lru = false;
+    }
+    b.pins += 1;
+    if (mode != SIPGPU_CACHE_READ) {
+        b.home_valid = false;
+        b.initialised = true;
+    }
+    *dev = reinterpret_cast<double *>(c.arena.base + b.off);
+    return 0;
+}
+
+int release_locked(sipgpu_cache_s &c, hipStream_t stream, unsigned long long key) {
+    auto it = c.blocks.find(key);
+    if (it == c.blocks.end()) return fail(SIPGPU_ENOTFOUND, "cache: unknown block key");
+    CBlock &b = it->second;
+    if (b.pins <= 0) return fail(SIPGPU_EARG, "cache: release without acquire");
+    if (!b.last_use)
+        if (int rc = make_event(&b.last_use)) return rc;
+    SIPGPU_HIP_TRY(hipEventRecord(b.last_use, stream));
+    b.used_on_device = true;
+    if (--b.pins == 0) {
+        c.lru.push_back(key);
+        b.lru_it = std::prev(c.lru.end());
+        b.in_lru = true;
+    }
+    return 0;
+}
+
+} // namespace
+
+extern "C" {
+
+int sipgpu_cache_create(size_t budget_bytes, sipgpu_cache_t *cache) {
+    if (!cache || budget_bytes < 256) return fail(SIPGPU_EARG, "cache: bad arguments");
+    if (int rc = sipgpu::rt::ensure_ready()) return rc;
+    auto c = std::make_unique<sipgpu_cache_s>();
+    const size_t bytes = Arena::round(budget_bytes);
+    hipError_t e = hipMalloc(&c->dev, bytes);
+    if (e != hipSuccess) return fail(SIPGPU_ENOMEM, "cache: HBM budget not available");
+    c->arena.reset(static_cast<char *>(c->dev), bytes);
+    e = hipStreamCreateWithFlags(&c->copy, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        (void)hipFree(c->dev);
+        return hip_fail(e, "cache: copy stream");
+    }
+    c->st.budget_bytes = bytes;
+    *cache = c.release();
+    return 0;
+}
+
+int sipgpu_cache_destroy(sipgpu_cache_t c) {
+    if (!c) return fail(SIPGPU_EARG, "null cache");
+    SIPGPU_HIP_TRY(hipDeviceSynchronize());
+    for (auto &kv : c->blocks) {
+        if (kv.second.home_owned) (void)hipHostFree(kv.second.home);
+        if (kv.second.last_use) (void)hipEventDestroy(kv.second.last_use);
+    }
+    for (hipEvent_t f : c->fences) (void)hipEventDestroy(f);
+    (void)hipStreamDestroy(c->copy);
+    (void)hipFree(c->dev);
+    delete c;
+    return 0;
+}
+
+int sipgpu_cache_insert(sipgpu_cache_t c, unsigned long long key, size_t elems,
+                        const double *host, int init, unsigned long long seed,
+                        unsigned long long stream, double a, double b) {
+    if (!c || elems == 0) return fail(SIPGPU_EARG, "cache: bad arguments");
+    if (init < SIPGPU_CACHE_UNDEFINED || init > SIPGPU_CACHE_NORMAL)
+        return fail(SIPGPU_EARG, "cache: unknown initialisation");
+    std::lock_guard<std::mutex> g(c->mu);
+    if (c->blocks.count(key)) return fail(SIPGPU_EARG, "cache: key already present");
+    if (Arena::round(elems * sizeof(double)) > c->arena.size)
+        return fail(SIPGPU_EBUDGET, "cache: block larger than the HBM budget");
+    CBlock blk;
+    blk.elems = elems;
+    blk.home = const_cast<double *>(host);
+    blk.home_valid = host != nullptr;
+    blk.initialised = host != nullptr;
+    blk.init = init;
+    blk.seed = seed;
+    blk.stream = stream;
+    blk.a = a;
+    blk.b = b;
+    c->blocks.emplace(key, blk);
+    return 0;
+}
+
+int sipgpu_cache_acquire(sipgpu_cache_t c, sipgpu_stream_t s, unsigned long long key, int mode,
+                         double **dev) {
+    if (!c || !dev || mode < SIPGPU_CACHE_READ || mode > SIPGPU_CACHE_READWRITE)
+        return fail(SIPGPU_EARG, "cache: bad arguments");
+    std::lock_guard<std::mutex> g(c->mu);
+    return acquire_locked(*c, stream_of(s), key, mode, dev);
+}
+
+int sipgpu_cache_release(sipgpu_cache_t c, sipgpu_stream_t s, unsigned long long key) {
+    if (!c) return fail(SIPGPU_EARG, "null cache");
+    std::lock_guard<std::mutex> g(c->mu);
+    return release_locked(*c, stream_of(s), key);
+}
+
+int sipgpu_cache_read(sipgpu_cache_t c, unsigned long long key, double *out) {
+    if (!c || !out) return fail(SIPGPU_EARG, "cache: bad arguments");
+    std::lock_guard<std::mutex> g(c->mu);
+    auto it = c->blocks.find(key);
+    if (it == c->blocks.end()) return fail(SIPGPU_ENOTFOUND, "cache: unknown block key");
+    CBlock &b = it->second;
+    const size_t bytes = b.elems * sizeof(double);
+    SIPGPU_HIP_TRY(hipDeviceSynchronize());
+    if (b.off != kNotResident && !(b.initialised && b.home_valid)) {
+        SIPGPU_HIP_TRY(hipMemcpy(out, c->arena.base + b.off, bytes, hipMemcpyDeviceToHost));
+    } else if (b.initialised && b.home_valid) {
+        std::memcpy(out, b.home, bytes);
+    } else {
+        return fail(SIPGPU_EARG, "cache: block has no defined value yet");
+    }
+    return 0;
+}
+
+int sipgpu_cache_flush(sipgpu_cache_t c, unsigned long long key, int all) {
+    if (!c) return fail(SIPGPU_EARG, "null cache");
+    std::lock_guard<std::mutex> g(c->mu);
+    SIPGPU_HIP_TRY(hipDeviceSynchronize());
+    for (auto &kv : c->blocks) {
+        if (!all && kv.first != key) continue;
+        CBlock &b = kv.second;
+        if (b.off == kNotResident || !b.initialised || b.home_valid) continue;
+        const size_t bytes = b.elems * sizeof(double);
+        if (!b.home) {
+            hipError_t e = hipHostMalloc((void **)&b.home, std::max<size_t>(bytes, 8),
+                                         hipHostMallocDefault);
+            if (e != hipSuccess) return fail(SIPGPU_ENOMEM, "cache: pinned spill buffer");
+            b.home_owned = true;
+        }
+        SIPGPU_HIP_TRY(hipMemcpy(b.home, c->arena.base + b.off, bytes, hipMemcpyDeviceToHost));
+        b.home_valid = true;
+        c->st.bytes_d2h += bytes;
+    }
+    return 0;
+}
+
+int sipgpu_cache_evict(sipgpu_cache_t c, unsigned long long key) {
+    if (!c) return fail(SIPGPU_EARG, "null cache");
+    std::lock_guard<std::mutex> g(c->mu);
+    auto it = c->blocks.find(key);
+    if (it == c->blocks.end()) return fail(SIPGPU_ENOTFOUND, "cache: unknown block key");
+    if (it->second.pins) return fail(SIPGPU_EARG, "cache: block in use");
+    if (it->second.off == kNotResident) return 0;
+    return evict_locked(*c, key, it->second);
+}
+
+int sipgpu_cache_get_stats(sipgpu_cache_t c, sipgpu_cache_stats *out) {
+    if (!c || !out) return fail(SIPGPU_EARG, "cache: bad arguments");
+    std::lock_guard<std::mutex> g(c->mu);
+    *out = c->st;
+    out->resident_bytes = c->arena.used;
+    out->blocks = c->blocks.size();
+    return 0;
+}
+
+} // extern "C"
+
+// ---------------------------------------------------------------------------
+// Server
+// ---------------------------------------------------------------------------
+
+namespace {
+
+constexpr int kSlots = 4;
+constexpr size_t kSlotBytes = 32u << 20;
+
+struct Pending {
+    unsigned long long ticket;
+    sipgpu_request req;
+    Clock::time_point arrival;
+};
+
+struct Done {
+    Clock::time_point arrival, done;
+    int status = 0;
+};
+
+} // namespace
+
+struct sipgpu_server_s {
+    sipgpu_cache_t cache = nullptr;
+    int max_batch = 0;
+    hipStream_t stream = nullptr;
+    void *slot_dev[kSlots] = {};
+    void *slot_host[kSlots] = {};
+    hipEvent_t slot_free[kSlots] = {};
+    bool slot_used[kSlots] = {};
+    int next_slot = 0;
+
+    std::mutex mu;
+    std::condition_variable cv_queue, cv_done, cv_inflight;
+    std::deque<Pending> queue;
+    std::unordered_map<unsigned long long, Done> done;
+    unsigned long long next_ticket = 1;
+    bool stop = false;
+    bool sched_done = false; // the scheduler has pushed its last batch
+
+    struct Flight {
+        hipEvent_t ev;
+        std::vector<std::pair<unsigned long long, Clock::time_point>> reqs;
+        int status;
+    };
+    std::deque<Flight> inflight;
+    std::vector<hipEvent_t> event_pool;
+
+    std::thread sched, compl_;
+    sipgpu_server_stats st{};
+};
+
+namespace {
+
+int server_batch(sipgpu_server_s &S, std::vector<Pending> &batch) {
+    // blocks of the batch, each acquired once: destinations for write
+    // (read-write if their first entry accumulates), operands for read
+    std::unordered_map<unsigned long long, double *> dptr;
+    std::vector<unsigned long long> acquired;
+    std::unordered_map<unsigned long long, int> dmode;
+    for (const Pending &p : batch)
+        if (!dmode.count(p.req.dkey))
+            dmode[p.req.dkey] = p.req.accumulate ? SIPGPU_CACHE_READWRITE : SIPGPU_CACHE_WRITE;
+    int rc = 0;
+    sipgpu_stream_s sw;
+    sw.s = S.stream;
+    for (const Pending &p : batch) {
+        const unsigned long long keys[3] = {p.req.lkey, p.req.rkey, p.req.dkey};
+        for (int q = 0; q < 3 && !rc; ++q) {
+            if (dptr.count(keys[q])) continue;
+            const int mode = q == 2 ? dmode[keys[q]] : SIPGPU_CACHE_READ;
+            double *d = nullptr;
+            rc = sipgpu_cache_acquire(S.cache, &sw, keys[q], mode, &d);
+            if (!rc) {
+                dptr[keys[q]] = d;
+                acquired.push_back(keys[q]);
+            }
+        }
+        if (rc) break;
+    }
+    sipgpu::rt::BatchImage img;
+    if (!rc) {
+        std::vector<sipgpu_contraction> list(batch.size());
+        for (size_t i = 0; i < batch.size(); ++i) {
+            const sipgpu_request &r = batch[i].req;
+            sipgpu_contraction &e = list[i];
+            std::memcpy(e.ptrn, r.ptrn, sizeof(e.ptrn));
+            e.l = dptr[r.lkey];
+            e.r = dptr[r.rkey];
+            e.d = dptr[r.dkey];
+            e.lrank = r.lrank;
+            e.rrank = r.rrank;
+            e.drank = r.drank;
+            std::memcpy(e.ldims, r.ldims, sizeof(e.ldims));
+            std::memcpy(e.rdims, r.rdims, sizeof(e.rdims));
+            std::memcpy(e.ddims, r.ddims, sizeof(e.ddims));
+            e.accumulate = r.accumulate;
+        }
+        rc = sipgpu::rt::build_batch_image(list.data(), (int)list.size(), img);
+    }
+    if (!rc && img.host.size() > kSlotBytes)
+        rc = fail(SIPGPU_ESIZE, "server: batch work list exceeds its descriptor slot");
+    if (!rc && img.ntiles > 0) {
+        const int slot = S.next_slot;
+        S.next_slot = (S.next_slot + 1) % kSlots;
+        if (S.slot_used[slot]) {
+            hipError_t e = hipEventSynchronize(S.slot_free[slot]);
+            if (e != hipSuccess) rc = hip_fail(e, "server: slot");
+        }
+        if (!rc) {
+            std::memcpy(S.slot_host[slot], img.host.data(), img.host.size());
+            hipError_t e = hipMemcpyAsync(S.slot_dev[slot], S.slot_host[slot], img.host.size(),
+                                          hipMemcpyHostToDevice, S.stream);
+            if (e == hipSuccess)
+                e = sipgpu::launch_gemm(img.cfg, img.args_at(S.slot_dev[slot]), img.ntiles,
+                                        S.stream);
+            if (e == hipSuccess) e = hipEventRecord(S.slot_free[slot], S.stream);
+            if (e != hipSuccess) rc = hip_fail(e, "server: launch");
+            S.slot_used[slot] = true;
+        }
+    }
+    for (unsigned long long k : acquired) {
+        const int r2 = sipgpu_cache_release(S.cache, &sw, k);
+        if (!rc) rc = r2;
+    }
+    if (!rc) {
+        S.st.batches += 1;
+        S.st.requests += batch.size();
+        S.st.flops += img.flops;
+        S.st.max_batch = std::max<unsigned long long>(S.st.max_batch, batch.size());
+    }
+    return rc;
+}
+
+void scheduler_loop(sipgpu_server_s *S) {
+    (void)hipSetDevice(sipgpu::rt::device_id());
+    for (;;) {
+        std::vector<Pending> batch;
+        {
+            std::unique_lock<std::mutex> lk(S->mu);
+            S->cv_queue.wait(lk, [&] { return S->stop || !S->queue.empty(); });
+            if (S->queue.empty() && S->stop) return;
+            std::unordered_set<unsigned long long> reads, writes;
+            // the blocks a batch pins must fit beside the ones it evicts
+            const size_t pin_limit = S->cache->arena.size / 2;
+            size_t pinned = 0;
+            while (!S->queue.empty() && (int)batch.size() < S->max_batch) {
+                const sipgpu_request &r = S->queue.front().req;
+                // keep program order: no block both written and read in one batch
+                if (writes.count(r.lkey) || writes.count(r.rkey) || reads.count(r.dkey)) break;
+                size_t add = 0;
+                auto bytes_of = [](int rank, const int *dims) {
+                    size_t n = 8;
+                    for (int i = 0; i < rank; ++i) n *= (size_t)dims[i];
+                    return n;
+                };
+                if (!reads.count(r.lkey)) add += bytes_of(r.lrank, r.ldims);
+                if (!reads.count(r.rkey)) add += bytes_of(r.rrank, r.rdims);
+                if (!writes.count(r.dkey)) add += bytes_of(r.drank, r.ddims);
+                if (!batch.empty() && pinned + add > pin_limit) break;
+                pinned += add;
+                reads.insert(r.lkey);
+                reads.insert(r.rkey);
+                writes.insert(r.dkey);
+                batch.push_back(S->queue.front());
+                S->queue.pop_front();
+            }
+        }
+        const int rc = server_batch(*S, batch);
+        hipEvent_t ev = nullptr;
+        {
+            std::lock_guard<std::mutex> lk(S->mu);
+            if (!S->event_pool.empty()) {
+                ev = S->event_pool.back();
+                S->event_pool.pop_back();
+            }
+        }
+        if (!ev) (void)hipEventCreateWithFlags(&ev, hipEventDisableTiming);
+        (void)hipEventRecord(ev, S->stream);
+        {
+            std::lock_guard<std::mutex> lk(S->mu);
+            sipgpu_server_s::Flight f;
+            f.ev = ev;
+            f.status = rc;
+            for (const Pending &p : batch) f.reqs.emplace_back(p.ticket, p.arrival);
+            S->inflight.push_back(std::move(f));
+        }
+        S->cv_inflight.notify_one();
+    }
+}
+
+void completion_loop(sipgpu_server_s *S) {
+    (void)hipSetDevice(sipgpu::rt::device_id());
+    for (;;) {
+        sipgpu_server_s::Flight f;
+        {
+            std::unique_lock<std::mutex> lk(S->mu);
+            S->cv_inflight.wait(lk, [&] { return !S->inflight.empty() || S->sched_done; });
+            if (S->inflight.empty()) return; // scheduler finished and all batches stamped
+            f = std::move(S->inflight.front());
+            S->inflight.pop_front();
+        }
+        const hipError_t e = hipEventSynchronize(f.ev);
+        const Clock::time_point t = Clock::now();
+        std::lock_guard<std::mutex> lk(S->mu);
+        for (auto &r : f.reqs) {
+            Done d;
+            d.arrival = r.second;
+            d.done = t;
+            d.status = f.status ? f.status : (e == hipSuccess ? 0 : SIPGPU_EDEVICE);
+            S->done[r.first] = d;
+        }
+        S->event_pool.push_back(f.ev);
+        S->cv_done.notify_all();
+    }
+}
+
+int check_request(const sipgpu_request &r) {
+    if (r.lrank < 1 || r.lrank > SIPGPU_MAX_RANK || r.rrank < 1 || r.rrank > SIPGPU_MAX_RANK ||
+        r.drank < 1 || r.drank > SIPGPU_MAX_RANK)
+        return fail(SIPGPU_ERANK, "server: requests are block = block * block, rank 1..6");
+    if (r.dkey == r.lkey || r.dkey == r.rkey)
+        return fail(SIPGPU_EHAZARD, "server: a request writes a block it reads");
+    return 0;
+}
+
+int submit_at(sipgpu_server_s *S, const sipgpu_request *r, Clock::time_point arrival,
+              unsigned long long *ticket) {
+    if (int rc = check_request(*r)) return rc;
+    {
+        std::lock_guard<std::mutex> lk(S->mu);
+        if (S->stop) return fail(SIPGPU_EARG, "server: stopped");
+        const unsigned long long t = S->next_ticket++;
+        S->queue.push_back(Pending{t, *r, arrival});
+        if (ticket) *ticket = t;
+    }
+    S->cv_queue.notify_one();
+    return 0;
+}
+
+int wait_ticket(sipgpu_server_s *S, unsigned long long ticket, double *latency_ms) {
+    std::unique_lock<std::mutex> lk(S->mu);
+    S->cv_done.wait(lk, [&] { return S->done.count(ticket) != 0; });
+    const Done d = S->done[ticket];
+    S->done.erase(ticket);
+    if (latency_ms)
+        *latency_ms = std::chrono::duration<double, std::milli>(d.done - d.arrival).count();
+    if (d.status) return fail(d.status, "server: request failed (" + std::to_string(d.status) + ")");
+    return 0;
+}
+
+} // namespace
+
+extern "C" {
+
+int sipgpu_server_create(sipgpu_cache_t cache, int max_batch, sipgpu_server_t *server) {
+    if (!cache || !server || max_batch < 1) return fail(SIPGPU_EARG, "server: bad arguments");
+    if (int rc = sipgpu::rt::ensure_ready()) return rc;
+    auto S = std::make_unique<sipgpu_server_s>();
+    S->cache = cache;
+    S->max_batch = max_batch;
+    SIPGPU_HIP_TRY(hipStreamCreateWithFlags(&S->stream, hipStreamNonBlocking));
+    for (int i = 0; i < kSlots; ++i) {
+        SIPGPU_HIP_TRY(hipMalloc(&S->slot_dev[i], kSlotBytes));
+        SIPGPU_HIP_TRY(hipHostMalloc(&S->slot_host[i], kSlotBytes, hipHostMallocDefault));
+        SIPGPU_HIP_TRY(hipEventCreateWithFlags(&S->slot_free[i], hipEventDisableTiming));
+    }
+    S->sched = std::thread(scheduler_loop, S.get());
+    S->compl_ = std::thread(completion_loop, S.get());
+    *server = S.release();
+    return 0;
+}
+
+int sipgpu_server_submit(sipgpu_server_t S, const sipgpu_request *r, unsigned long long *ticket) {
+    if (!S || !r) return fail(SIPGPU_EARG, "server: bad arguments");
+    return submit_at(S, r, Clock::now(), ticket);
+}
+
+int sipgpu_server_wait(sipgpu_server_t S, unsigned long long ticket, double *latency_ms) {
+    if (!S) return fail(SIPGPU_EARG, "null server");
+    return wait_ticket(S, ticket, latency_ms);
+}
+
+int sipgpu_server_replay(sipgpu_server_t S, const sipgpu_request *reqs, const double *arrival_s,
+                         int n, double *latency_ms, double *wall_s) {
+    if (!S || n < 0 || (n > 0 && (!reqs || !arrival_s || !latency_ms)))
+        return fail(SIPGPU_EARG, "server: bad arguments");
+    std::vector<unsigned long long> tickets(n);
+    const Clock::time_point t0 = Clock::now();
+    int rc = 0;
+    for (int i = 0; i < n && !rc; ++i) {
+        const Clock::time_point at =
+            t0 + std::chrono::duration_cast<Clock::duration>(std::chrono::duration<double>(arrival_s[i]));
+        while (Clock::now() < at) {
+            // spin for the last 50 us, sleep before that
+            if (at - Clock::now() > std::chrono::microseconds(200))
+                std::this_thread::sleep_for(std::chrono::microseconds(100));
+        }
+        rc = submit_at(S, &reqs[i], at, &tickets[i]);
+    }
+    for (int i = 0; i < n; ++i) {
+        if (!tickets[i]) continue;
+        const int r2 = wait_ticket(S, tickets[i], &latency_ms[i]);
+        if (!rc) rc = r2;
+    }
+    if (wall_s) *wall_s = std::chrono::duration<double>(Clock::now() - t0).count();
+    return rc;
+}
+
+int sipgpu_server_get_stats(sipgpu_server_t S, sipgpu_server_stats *out) {
+    if (!S || !out) return fail(SIPGPU_EARG, "server: bad arguments");
+    std::lock_guard<std::mutex> lk(S->mu);
+    *out = S->st;
+    return 0;
+}
+
+int sipgpu_server_destroy(sipgpu_server_t S) {
+    if (!S) return fail(SIPGPU_EARG, "null server");
+    {
+        std::lock_guard<std::mutex> lk(S->mu);
+        S->stop = true;
+    }
+    S->cv_queue.notify_all();
+    S->sched.join();
+    {
+        std::lock_guard<std::mutex> lk(S->mu);
+        S->sched_done = true;
+    }
+    S->cv_inflight.notify_all();
+    S->compl_.join();
+    (void)hipStreamSynchronize(S->stream);
+    for (int i = 0; i < kSlots; ++i) {
+        (void)hipFree(S->slot_dev[i]);
+        (void)hipHostFree(S->slot_host[i]);
+        (void)hipEventDestroy(S->slot_free[i]);
+    }
+    for (hipEvent_t e : S->event_pool) (void)hipEventDestroy(e);
+    (void)hipStreamDestroy(S->stream);
+    delete S;
+    return 0;
+}
+
+} // extern "C"

@@ -42,6 +42,11 @@ EXPORTED = [
     "sipgpu_batch_info", "sipgpu_plan_tables", "sipgpu_plan_permute", "sipgpu_measure_peaks",
     "sipgpu_measured_clock_mhz", "sipgpu_batch_plan_split", "sipgpu_batch_split_info",
     "sipgpu_fill_random",
+    "sipgpu_cache_create", "sipgpu_cache_destroy", "sipgpu_cache_insert", "sipgpu_cache_acquire",
+    "sipgpu_cache_release", "sipgpu_cache_read", "sipgpu_cache_flush", "sipgpu_cache_evict",
+    "sipgpu_cache_get_stats", "sipgpu_server_create", "sipgpu_server_submit",
+    "sipgpu_server_wait", "sipgpu_server_replay", "sipgpu_server_get_stats",
+    "sipgpu_server_destroy",
 ]
 
 
@@ -58,6 +63,34 @@ class Contraction(ctypes.Structure):
                 ("ldims", ctypes.c_int * MAX_RANK), ("rdims", ctypes.c_int * MAX_RANK),
                 ("ddims", ctypes.c_int * MAX_RANK), ("ptrn", ctypes.c_int * (2 * MAX_RANK)),
                 ("accumulate", ctypes.c_int)]
+
+
+class Request(ctypes.Structure):
+    """sipgpu_request of include/sipgpu.h."""
+    _fields_ = [("ptrn", ctypes.c_int * (2 * MAX_RANK)), ("lrank", ctypes.c_int),
+                ("rrank", ctypes.c_int), ("drank", ctypes.c_int),
+                ("ldims", ctypes.c_int * MAX_RANK), ("rdims", ctypes.c_int * MAX_RANK),
+                ("ddims", ctypes.c_int * MAX_RANK), ("lkey", ctypes.c_ulonglong),
+                ("rkey", ctypes.c_ulonglong), ("dkey", ctypes.c_ulonglong),
+                ("accumulate", ctypes.c_int)]
+
+
+class CacheStats(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_ulonglong) for n in
+                ("hits", "misses", "evictions", "spills", "bytes_h2d", "bytes_d2h",
+                 "resident_bytes", "budget_bytes", "blocks")]
+
+    def as_dict(self):
+        return {n: int(getattr(self, n)) for n, _ in self._fields_}
+
+
+class ServerStats(ctypes.Structure):
+    _fields_ = [("batches", ctypes.c_ulonglong), ("requests", ctypes.c_ulonglong),
+                ("max_batch", ctypes.c_ulonglong), ("flops", ctypes.c_double)]
+
+    def as_dict(self):
+        return {n: (float(getattr(self, n)) if n == "flops" else int(getattr(self, n)))
+                for n, _ in self._fields_}
 
 
 _lib = None
@@ -107,6 +140,26 @@ def lib():
                                          _ip, _ip, _ip, _ip, _ip, _ip, _ip]
         L.sipgpu_plan_permute.argtypes = [ctypes.c_int, _ip, _ip, _ip]
         L.sipgpu_measured_clock_mhz.restype = ctypes.c_double
+        _u64 = ctypes.c_ulonglong
+        L.sipgpu_cache_create.argtypes = [ctypes.c_size_t, ctypes.POINTER(_vp)]
+        L.sipgpu_cache_destroy.argtypes = [_vp]
+        L.sipgpu_cache_insert.argtypes = [_vp, _u64, ctypes.c_size_t, _vp, ctypes.c_int, _u64,
+                                          _u64, ctypes.c_double, ctypes.c_double]
+        L.sipgpu_cache_acquire.argtypes = [_vp, _vp, _u64, ctypes.c_int, ctypes.POINTER(_vp)]
+        L.sipgpu_cache_release.argtypes = [_vp, _vp, _u64]
+        L.sipgpu_cache_read.argtypes = [_vp, _u64, _vp]
+        L.sipgpu_cache_flush.argtypes = [_vp, _u64, ctypes.c_int]
+        L.sipgpu_cache_evict.argtypes = [_vp, _u64]
+        L.sipgpu_cache_get_stats.argtypes = [_vp, ctypes.POINTER(CacheStats)]
+        L.sipgpu_server_create.argtypes = [_vp, ctypes.c_int, ctypes.POINTER(_vp)]
+        L.sipgpu_server_submit.argtypes = [_vp, ctypes.POINTER(Request), ctypes.POINTER(_u64)]
+        L.sipgpu_server_wait.argtypes = [_vp, _u64, ctypes.POINTER(ctypes.c_double)]
+        L.sipgpu_server_replay.argtypes = [_vp, ctypes.POINTER(Request),
+                                           ctypes.POINTER(ctypes.c_double), ctypes.c_int,
+                                           ctypes.POINTER(ctypes.c_double),
+                                           ctypes.POINTER(ctypes.c_double)]
+        L.sipgpu_server_get_stats.argtypes = [_vp, ctypes.POINTER(ServerStats)]
+        L.sipgpu_server_destroy.argtypes = [_vp]
         L.sipgpu_fill_random.argtypes = [_vp, _vp, ctypes.c_longlong, ctypes.c_ulonglong,
                                          ctypes.c_ulonglong, ctypes.c_int, ctypes.c_double,
                                          ctypes.c_double]
@@ -417,3 +470,115 @@ class Batch:
             self.destroy()
         except Exception:
             pass
+
+
+CACHE_INIT = {"undefined": 0, "zero": 1, "uniform": 2, "normal": 3}
+CACHE_MODE = {"r": 1, "w": 2, "rw": 3}
+
+
+class Cache:
+    """Device block cache: blocks named by integer keys in an HBM arena of
+    `budget_bytes`, evicted least-recently-released first, spilled to host."""
+
+    def __init__(self, budget_bytes):
+        p = _vp()
+        _check(lib().sipgpu_cache_create(int(budget_bytes), ctypes.byref(p)))
+        self.handle = p.value
+        self._homes = {}
+
+    def insert(self, key, elems, host=None, init="undefined", seed=0, stream=0, a=0.0, b=1.0):
+        ptr = None
+        if host is not None:
+            host = np.ascontiguousarray(host, dtype=np.float64)
+            assert host.size == elems
+            self._homes[key] = host  # the cache writes spills back into it
+            ptr = host.ctypes.data
+        _check(lib().sipgpu_cache_insert(self.handle, key, int(elems), ptr, CACHE_INIT[init],
+                                         int(seed), int(stream), float(a), float(b)))
+
+    def acquire(self, key, mode="r", s=None):
+        p = _vp()
+        _check(lib().sipgpu_cache_acquire(self.handle, None if s is None else s.handle, key,
+                                          CACHE_MODE[mode], ctypes.byref(p)))
+        return p.value
+
+    def release(self, key, s=None):
+        _check(lib().sipgpu_cache_release(self.handle, None if s is None else s.handle, key))
+
+    def read(self, key, elems):
+        out = np.empty(int(elems))
+        _check(lib().sipgpu_cache_read(self.handle, key, out.ctypes.data))
+        return out
+
+    def flush(self, key=None):
+        _check(lib().sipgpu_cache_flush(self.handle, 0 if key is None else key,
+                                        1 if key is None else 0))
+
+    def evict(self, key):
+        _check(lib().sipgpu_cache_evict(self.handle, key))
+
+    def stats(self):
+        st = CacheStats()
+        _check(lib().sipgpu_cache_get_stats(self.handle, ctypes.byref(st)))
+        return st.as_dict()
+
+    def destroy(self):
+        if self.handle:
+            _check(lib().sipgpu_cache_destroy(self.handle))
+            self.handle = None
+
+
+def make_request(ptrn, lkey, ldims, rkey, rdims, dkey, ddims, accumulate=False):
+    r = Request()
+    r.lkey, r.rkey, r.dkey = lkey, rkey, dkey
+    r.lrank, r.rrank, r.drank = len(ldims), len(rdims), len(ddims)
+    for i, v in enumerate(ldims):
+        r.ldims[i] = int(v)
+    for i, v in enumerate(rdims):
+        r.rdims[i] = int(v)
+    for i, v in enumerate(ddims):
+        r.ddims[i] = int(v)
+    for i, v in enumerate(ptrn):
+        r.ptrn[i] = int(v)
+    r.accumulate = 1 if accumulate else 0
+    return r
+
+
+class Server:
+    """Continuous-batching contraction server over a Cache."""
+
+    def __init__(self, cache, max_batch=256):
+        p = _vp()
+        _check(lib().sipgpu_server_create(cache.handle, int(max_batch), ctypes.byref(p)))
+        self.handle = p.value
+
+    def submit(self, req):
+        t = ctypes.c_ulonglong(0)
+        _check(lib().sipgpu_server_submit(self.handle, ctypes.byref(req), ctypes.byref(t)))
+        return t.value
+
+    def wait(self, ticket):
+        lat = ctypes.c_double(0)
+        _check(lib().sipgpu_server_wait(self.handle, ticket, ctypes.byref(lat)))
+        return lat.value
+
+    def replay(self, reqs, arrival_s):
+        n = len(reqs)
+        arr = (Request * n)(*reqs)
+        at = np.ascontiguousarray(arrival_s, dtype=np.float64)
+        lat = np.zeros(n)
+        wall = ctypes.c_double(0)
+        _check(lib().sipgpu_server_replay(
+            self.handle, arr, at.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), n,
+            lat.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), ctypes.byref(wall)))
+        return lat, wall.value
+
+    def stats(self):
+        st = ServerStats()
+        _check(lib().sipgpu_server_get_stats(self.handle, ctypes.byref(st)))
+        return st.as_dict()
+
+    def destroy(self):
+        if self.handle:
+            _check(lib().sipgpu_server_destroy(self.handle))
+            self.handle = None

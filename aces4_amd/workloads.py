@@ -298,3 +298,89 @@ def sparse_ladder(o=40, v=200, seg_lo=12, seg_hi=40, probs=(0.75, 1 / 12, 1 / 12
     return dict(name="sparse-ladder", D=R, L=V, R=T, entries=entries,
                 ptrn=[int(x) for x in ptrn], labels=labels, v_segs=vs, o_segs=os_,
                 v_irreps=vi, o_irreps=oi, dense_flops=dense)
+
+
+SERVE_TERMS = {
+    # name: (dest labels, left labels, right labels, left array, right array, dest array)
+    "ladder": ("abij", "abcd", "cdij", "V", "T", "R"),
+    "ring": ("aibj", "kbcj", "aick", "W", "U", "Q"),
+    "hole": ("abij", "abkl", "klij", "T", "H", "R"),
+}
+SERVE_ARRAYS = {
+    # name: (index kinds, initial value, a, b)
+    "V": ("vvvv", "normal", 0.1, 0.0),
+    "T": ("vvoo", "normal", 1e-2, 0.0),
+    "W": ("ovvo", "uniform", -0.5, 0.5),
+    "U": ("vovo", "normal", 1e-2, 0.0),
+    "H": ("oooo", "normal", 0.1, 0.0),
+    "R": ("vvoo", "zero", 0.0, 0.0),
+    "Q": ("vovo", "zero", 0.0, 0.0),
+}
+
+
+def serving_mix(n_requests=3000, rate=3000.0, seed=4, v_nseg=8, o_nseg=4, seg_lo=8, seg_hi=64,
+                mix=(("ladder", 0.4), ("ring", 0.4), ("hole", 0.2))):
+    """A stream of single block contractions for the serving path.
+
+    Segment extents are uniform in seg_lo..seg_hi (v_nseg virtual and o_nseg
+    occupied segments); each request picks a term (ladder / ring / hole-hole)
+    with the given probabilities and uniform segment indices, and accumulates
+    one block pair into its result block.  Arrivals are Poisson with `rate`
+    requests per second.  Blocks are named by 64-bit keys (array id << 48 |
+    block index) and generated in HBM on first use.
+    """
+    rng = np.random.default_rng(seed)
+    segs = {"v": [int(x) for x in rng.integers(seg_lo, seg_hi + 1, v_nseg)],
+            "o": [int(x) for x in rng.integers(seg_lo, seg_hi + 1, o_nseg)]}
+    names = sorted(SERVE_ARRAYS)
+    arr_id = {n: i + 1 for i, n in enumerate(names)}
+
+    def key(name, blk):
+        kinds = SERVE_ARRAYS[name][0]
+        lin = 0
+        for d in reversed(range(len(blk))):
+            lin = lin * len(segs[kinds[d]]) + blk[d]
+        return (arr_id[name] << 48) | lin
+
+    def dims(name, blk):
+        kinds = SERVE_ARRAYS[name][0]
+        return [segs[kinds[d]][blk[d]] for d in range(len(blk))]
+
+    ptrns = {}
+    for t, (dl, ll, rl, *_rest) in SERVE_TERMS.items():
+        labels = list(dict.fromkeys(dl + ll + rl))
+        code = {lab: 1 + i for i, lab in enumerate(labels)}
+        p, e = sg.get_contraction_ptrn(4, 4, 4, [code[x] for x in dl + ll + rl])
+        assert e == 0
+        ptrns[t] = [int(x) for x in p]
+    terms = [t for t, _ in mix]
+    probs = np.array([p for _, p in mix], dtype=np.float64)
+    probs /= probs.sum()
+    kind_of = {"a": "v", "b": "v", "c": "v", "d": "v", "i": "o", "j": "o", "k": "o", "l": "o"}
+    reqs, flops, blocks = [], [], {}
+    for _ in range(n_requests):
+        t = terms[int(rng.choice(len(terms), p=probs))]
+        dl, ll, rl, ln, rn, dn = SERVE_TERMS[t]
+        idx = {lab: int(rng.integers(0, len(segs[kind_of[lab]]))) for lab in set(dl + ll + rl)}
+        lb, rb, db = (tuple(idx[x] for x in ll), tuple(idx[x] for x in rl),
+                      tuple(idx[x] for x in dl))
+        for name, blk in ((ln, lb), (rn, rb), (dn, db)):
+            blocks[key(name, blk)] = (name, blk, int(np.prod(dims(name, blk))))
+        reqs.append(sg.make_request(ptrns[t], key(ln, lb), dims(ln, lb), key(rn, rb),
+                                    dims(rn, rb), key(dn, db), dims(dn, db), accumulate=True))
+        ext = {lab: segs[kind_of[lab]][idx[lab]] for lab in idx}
+        mn = float(np.prod([ext[x] for x in dl]))
+        kk = float(np.prod([ext[x] for x in ll if x not in dl]))
+        flops.append(2.0 * mn * kk)
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, n_requests))
+    return dict(requests=reqs, arrivals=arrivals, flops=np.array(flops), blocks=blocks,
+                segs=segs, seed=seed, rate=rate, terms=dict(mix))
+
+
+def populate_cache(cache, mix):
+    """Registers every block the stream touches (generated in HBM on first
+    use; seeded per block)."""
+    for k, (name, blk, n) in mix["blocks"].items():
+        _, init, a, b = SERVE_ARRAYS[name]
+        cache.insert(k, n, init=init, seed=mix["seed"], stream=k, a=a, b=b)
+    return sum(n for _, _, n in mix["blocks"].values()) * 8

@@ -43,6 +43,8 @@ def parse():
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--latency-samples", type=int, default=200)
     p.add_argument("--no-split", action="store_true")
+    p.add_argument("--no-sparse", action="store_true")
+    p.add_argument("--no-serve", action="store_true")
     return p.parse_args()
 
 
@@ -55,13 +57,16 @@ def cpu_info():
     return os.cpu_count(), model
 
 
-def cpu_baseline(w, budget_s):
+def cpu_baseline(w, budget_s, gpu_result, steps_run):
     """The reference's own kernels (oracle/_ref) when present, else the C
     restatement: per block pair TMP = W*T (tensor_block_contract, 8 OpenMP
     threads as MAX_OMP_THREADS) then R += TMP (tensor_block_add), over a
-    seeded random sample of the workload's block pairs until `budget_s`."""
+    seeded random sample of the workload's block pairs until `budget_s`.
+    Also checks one GPU result block (after `steps_run` accumulating steps)
+    against the C restatement."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     from checker import Oracle, RefLib
+    from cases import rel_fro
     ref = RefLib() if RefLib.available() else None
     kind = "reference" if ref else "port"
     o = Oracle()
@@ -90,7 +95,16 @@ def cpu_baseline(w, budget_s):
         if time.perf_counter() - t0 > budget_s:
             break
     dt = time.perf_counter() - t0
+    db0 = next(iter(D.block_ids()))
+    want = np.zeros(int(np.prod(D.dims(db0))))
+    for db, lb, rb in w["entries"]:
+        if db == db0:
+            tmp, _ = o.contract(ptrn, L.block(lb), L.dims(lb), R.block(rb), R.dims(rb),
+                                D.dims(db))
+            want += tmp
+    check = rel_fro(D.block(db0, gpu_result), steps_run * want)
     return dict(value=flops / dt / 1e12, unit="TFLOP/s", cores=8 if ref else 1, kind=kind,
+                check_rel_err=check,
                 sample="%d of %d block pairs (seeded random), %.2e flop in %.2f s; "
                        "TMP = W*T then R += TMP per pair; %s" % (
                            npairs, len(w["entries"]), flops, dt,
@@ -192,6 +206,79 @@ def split_bench(w, fp64_batch, reps=20, moduli=(14, 13)):
     return out
 
 
+def sparse_bench(reps=5, moduli=14):
+    """Block-sparse ladder (o=40, v=200, uneven segments 12..40, 4 irreps):
+    one grouped FP64 launch and the split path over all surviving block
+    pairs, data generated in HBM.  Split checked against FP64 on every
+    result block."""
+    from cases import rel_fro
+    w = wl.sparse_ladder()
+    for k in ("D", "L", "R"):
+        w[k].to_device()
+    D = w["D"]
+    out = {"pairs": len(w["entries"]), "result_blocks": len(D.offset),
+           "flop": wl.flops(w), "dense_flop": w["dense_flops"],
+           "surviving_fraction": wl.flops(w) / w["dense_flops"],
+           "v_segments": w["v_segs"], "o_segments": w["o_segs"]}
+    ref = None
+    for name, nm in (("fp64", None), ("split", moduli)):
+        b = wl.batch_for(w, moduli=nm)
+        D.dev.zero()
+        b.execute()
+        sg.synchronize()
+        got = D.dev.download()
+        if ref is None:
+            ref = got
+            err = None
+        else:
+            err = max(rel_fro(D.download_block(db, got), D.download_block(db, ref))
+                      for db in D.block_ids())
+        e0, e1 = sg.Event(), sg.Event()
+        e0.record()
+        for _ in range(reps):
+            b.execute()
+        e1.record()
+        e1.synchronize()
+        ms = e0.elapsed_ms(e1) / reps
+        out[name] = {"ms": ms, "tflops_fp64_equiv": b.flops / (ms * 1e-3) / 1e12,
+                     "moduli": b.moduli, "bits": b.bits}
+        if err is not None:
+            out[name]["max_block_rel_err_vs_fp64"] = err
+        b.destroy()
+    for k in ("D", "L", "R"):
+        w[k].dev.free()
+    return out
+
+
+def serving_bench(label, budget_gb, n, rate, seg_lo, seg_hi, max_batch=256):
+    """Continuous batching of a Poisson stream of single block contractions
+    (ladder / ring / hole-hole mix) through the device block cache: the
+    stream is replayed once to warm the cache, then timed against its
+    arrival schedule.  Latency = completion - scheduled arrival."""
+    mix = wl.serving_mix(n_requests=n, rate=rate, seg_lo=seg_lo, seg_hi=seg_hi)
+    cache = sg.Cache(int(budget_gb * (1 << 30)))
+    ws = wl.populate_cache(cache, mix)
+    srv = sg.Server(cache, max_batch=max_batch)
+    srv.replay(mix["requests"], np.zeros(n))
+    c0, s0 = cache.stats(), srv.stats()
+    lat, wall = srv.replay(mix["requests"], mix["arrivals"])
+    c1, s1 = cache.stats(), srv.stats()
+    srv.destroy()
+    cache.destroy()
+    batches = s1["batches"] - s0["batches"]
+    return {
+        "scenario": label, "requests": n, "offered_rate_per_s": rate,
+        "segments": mix["segs"], "terms": mix["terms"],
+        "working_set_gb": ws / (1 << 30), "budget_gb": budget_gb,
+        "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+        "mean_ms": float(lat.mean()), "max_ms": float(lat.max()),
+        "wall_s": wall, "tflops": float(mix["flops"].sum()) / wall / 1e12,
+        "batches": batches, "mean_batch": n / max(batches, 1),
+        "cache": {k: c1[k] - c0[k] for k in ("hits", "misses", "evictions", "spills",
+                                             "bytes_h2d", "bytes_d2h")},
+    }
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -220,31 +307,24 @@ def main():
     avg_launch_s = float(launch_ms.mean()) / 1e3
     achieved = flops / avg_launch_s / 1e12
 
-    # quick parity check of one result block against the oracle
-    sys.path.insert(0, os.path.join(ROOT, "tests"))
-    from checker import Oracle
-    from cases import rel_fro
-    o = Oracle()
-    D, L, R, ptrn = w["D"], w["L"], w["R"], w["ptrn"]
-    got = D.dev.download()
-    db0 = next(iter(D.block_ids()))
-    want = np.zeros(int(np.prod(D.dims(db0))))
-    for db, lb, rb in w["entries"]:
-        if db == db0:
-            tmp, _ = o.contract(ptrn, L.block(lb), L.dims(lb), R.block(rb), R.dims(rb),
-                                D.dims(db))
-            want += tmp
+    got = w["D"].dev.download()
     steps_run = a.warmup + a.steps
-    check = rel_fro(D.block(db0, got), steps_run * want)
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
 
     split = split_bench(w, batch) if not a.no_split else None
     if split:
         best = max(v["tflops_fp64_equiv"] for v in split.values())
         split["speedup_vs_plain"] = best / achieved
+    sparse = None if a.no_sparse else sparse_bench()
+    serving = None
+    if not a.no_serve:
+        serving = [serving_bench("resident (HBM budget 200 GB)", 200.0, 3000, 2000.0, 8, 64),
+                   serving_bench("LRU spill (budget 60% of working set)", 8.5, 1500, 4000.0,
+                                 8, 40)]
     lat = latency_sample(w, a.latency_samples) if a.latency_samples > 0 else None
     perm = permute_bench()
     cores, model = cpu_info()
-    cpu = None if a.no_cpu else cpu_baseline(w, a.cpu_seconds)
+    cpu = None if a.no_cpu else cpu_baseline(w, a.cpu_seconds, got, steps_run)
     if cpu:
         cpu["host"] = "%s, %d logical cores" % (model, cores)
 
@@ -278,9 +358,10 @@ def main():
                      "frac_of_measured": achieved / mfma_meas if mfma_meas else None},
         "permute": perm,
         "split": split,
+        "sparse_ladder": sparse,
         "latency": lat,
+        "serving": serving,
         "cpu_baseline": cpu,
-        "check_rel_err": check,
     }
     print(json.dumps(res))
 

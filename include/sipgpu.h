@@ -175,6 +175,95 @@ int sipgpu_batch_split_info(sipgpu_batch_t batch, int *nmoduli, int *bits);
 int sipgpu_batch_info(sipgpu_batch_t batch, double *flops, int *ntasks,
                       int *ntiles, int *npairs);
 
+/* ------------------------------------------------------------------------
+ * Device block cache.  Replaces the SIP's BlockManager / CachedBlockMap /
+ * LRUArrayPolicy residency logic (block_manager.cpp:106-206,
+ * cached_block_map.cpp:31-147, lru_array_policy.h:37-96) for blocks in
+ * HBM: blocks are named by 64-bit keys and held in one arena of a fixed
+ * budget; when a block does not fit, unpinned blocks are evicted
+ * least-recently-released first, and a block whose device copy is newer
+ * than its host copy is first written back (spilled) to its host home, or
+ * to pinned host memory the cache allocates.  The reference evicts whole
+ * arrays; this cache evicts single blocks.  Work that uses a block must be
+ * enqueued between sipgpu_cache_acquire and sipgpu_cache_release on the
+ * same stream; ordering between streams is the caller's. */
+typedef struct sipgpu_cache_s *sipgpu_cache_t;
+
+enum { /* initial value of a block with no host copy */
+    SIPGPU_CACHE_UNDEFINED = 0,
+    SIPGPU_CACHE_ZERO = 1,
+    SIPGPU_CACHE_UNIFORM = 2, /* sipgpu_fill_random(seed, stream, dist 0, a, b) */
+    SIPGPU_CACHE_NORMAL = 3   /* sipgpu_fill_random(seed, stream, dist 1, a) */
+};
+enum { SIPGPU_CACHE_READ = 1, SIPGPU_CACHE_WRITE = 2, SIPGPU_CACHE_READWRITE = 3 };
+
+typedef struct {
+    unsigned long long hits, misses, evictions, spills;
+    unsigned long long bytes_h2d, bytes_d2h;
+    unsigned long long resident_bytes, budget_bytes, blocks;
+} sipgpu_cache_stats;
+
+int sipgpu_cache_create(size_t budget_bytes, sipgpu_cache_t *cache);
+int sipgpu_cache_destroy(sipgpu_cache_t cache);
+/* Registers a block of `elems` doubles.  `host` (may be NULL) is its home
+ * copy, read on a miss and written by spills; it must outlive the block.
+ * Without a home, `init` defines its first value. */
+int sipgpu_cache_insert(sipgpu_cache_t cache, unsigned long long key, size_t elems,
+                        const double *host, int init, unsigned long long seed,
+                        unsigned long long stream, double a, double b);
+/* Makes the block resident (uploading on `s` on a miss, evicting as needed),
+ * pins it and returns its device address.  WRITE marks the device copy as
+ * the current value without uploading it. */
+int sipgpu_cache_acquire(sipgpu_cache_t cache, sipgpu_stream_t s, unsigned long long key,
+                         int mode, double **dev);
+/* Unpins the block after the work enqueued on `s` that uses it. */
+int sipgpu_cache_release(sipgpu_cache_t cache, sipgpu_stream_t s, unsigned long long key);
+/* Current value of a block into host memory (synchronises the device). */
+int sipgpu_cache_read(sipgpu_cache_t cache, unsigned long long key, double *out);
+/* Writes back dirty resident blocks (one key, or all when `all`) to their
+ * homes: the SIP's flush before super-instructions and checkpoints. */
+int sipgpu_cache_flush(sipgpu_cache_t cache, unsigned long long key, int all);
+int sipgpu_cache_evict(sipgpu_cache_t cache, unsigned long long key);
+int sipgpu_cache_get_stats(sipgpu_cache_t cache, sipgpu_cache_stats *stats);
+
+/* ------------------------------------------------------------------------
+ * Contraction server: continuous batching of single block contractions
+ * whose blocks live in a cache.  A scheduler thread takes every request that
+ * has arrived (up to `max_batch`, stopping at the first one that would read
+ * a block an earlier request of the batch writes, or the reverse), makes
+ * its blocks resident and launches the batch as one grouped FP64 launch; a
+ * completion thread stamps each request's completion.  A batch pins at most
+ * half the cache budget (one request beyond that alone).  Requests on the same
+ * blocks take effect in submission order.  No reference counterpart: the
+ * SIP's worker runs one contraction at a time (interpreter.cpp:1210-1262). */
+typedef struct sipgpu_server_s *sipgpu_server_t;
+
+typedef struct {
+    int ptrn[2 * SIPGPU_MAX_RANK];
+    int lrank, rrank, drank;
+    int ldims[SIPGPU_MAX_RANK], rdims[SIPGPU_MAX_RANK], ddims[SIPGPU_MAX_RANK];
+    unsigned long long lkey, rkey, dkey;
+    int accumulate;
+} sipgpu_request;
+
+typedef struct {
+    unsigned long long batches, requests, max_batch;
+    double flops;
+} sipgpu_server_stats;
+
+int sipgpu_server_create(sipgpu_cache_t cache, int max_batch, sipgpu_server_t *server);
+int sipgpu_server_submit(sipgpu_server_t server, const sipgpu_request *req,
+                         unsigned long long *ticket);
+/* Waits for a request; latency = completion - submission, milliseconds. */
+int sipgpu_server_wait(sipgpu_server_t server, unsigned long long ticket, double *latency_ms);
+/* Submits reqs[i] at arrival_s[i] seconds after the call (an arrival
+ * schedule), waits for all, and returns each latency (completion - arrival)
+ * and the wall time of the whole replay. */
+int sipgpu_server_replay(sipgpu_server_t server, const sipgpu_request *reqs,
+                         const double *arrival_s, int n, double *latency_ms, double *wall_s);
+int sipgpu_server_get_stats(sipgpu_server_t server, sipgpu_server_stats *stats);
+int sipgpu_server_destroy(sipgpu_server_t server);
+
 /* Seeded synthetic data written in HBM, enqueued on `s`: n doubles of the
  * counter-based SplitMix64 sequence (seed, stream).  dist 0: uniform in
  * [a, b) from the top 53 bits of each output; dist 1: normal with standard
