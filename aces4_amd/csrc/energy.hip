@@ -1,0 +1,104 @@
+// energy.hip -- orbital-energy denominators and energy reductions of
+// coupled-cluster blocks.
+//
+// For a block x of rank r (column-major, extents dims) with one orbital
+// energy vector per index (eps_d, the energies of that index's segment) and
+// a sign per index (+1 occupied, -1 virtual), the denominator of element
+// (i_0 .. i_{r-1}) is the sum of the occupied energies in index order minus
+// the virtual ones in index order (the order energy_denominator_rhf.F
+// do_rhf_den2/4/6 add them in).  Kernels: x /= D in place, and the fixed-
+// order reduction E += factor * sum (x / D) * y, whose per-workgroup
+// partials are summed by one thread in workgroup order (deterministic).
+#include "kernels.h"
+
+namespace sipgpu {
+
+namespace {
+
+constexpr int kEnergyThreads = 256;
+
+__device__ __forceinline__ double denominator(const EnergyArgs &a, long long e) {
+    double v[kEnergyMaxRank];
+#pragma unroll
+    for (int d = 0; d < kEnergyMaxRank; ++d) {
+        if (d < a.rank) {
+            const int i = (int)(e % a.dims[d]);
+            e /= a.dims[d];
+            v[d] = a.eps[d][i];
+        }
+    }
+    // occupied energies added in index order, then each virtual one
+    // subtracted in index order: eps = e_b + e_d + e_f - e_a - e_c - e_e
+    double D = 0.0;
+    bool first = true;
+#pragma unroll
+    for (int d = 0; d < kEnergyMaxRank; ++d)
+        if (d < a.rank && a.sign[d] > 0) {
+            D = first ? v[d] : D + v[d];
+            first = false;
+        }
+#pragma unroll
+    for (int d = 0; d < kEnergyMaxRank; ++d)
+        if (d < a.rank && a.sign[d] < 0) {
+            D = first ? -v[d] : D - v[d];
+            first = false;
+        }
+    return D;
+}
+
+__global__ __launch_bounds__(kEnergyThreads) void denominator_divide_kernel(EnergyArgs a,
+                                                                            double *x) {
+    for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < a.n;
+         e += (long long)gridDim.x * blockDim.x)
+        x[e] = x[e] / denominator(a, e);
+}
+
+__global__ __launch_bounds__(kEnergyThreads) void energy_partial_kernel(EnergyArgs a,
+                                                                        const double *x,
+                                                                        const double *y,
+                                                                        double *part) {
+    __shared__ double red[kEnergyThreads];
+    double v = 0.0;
+    for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < a.n;
+         e += (long long)gridDim.x * blockDim.x)
+        v += (x[e] / denominator(a, e)) * y[e];
+    red[threadIdx.x] = v;
+    __syncthreads();
+    for (int s = kEnergyThreads / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+__global__ void energy_final_kernel(double *out, const double *part, int nparts, double factor) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        double v = 0.0;
+        for (int i = 0; i < nparts; ++i) v += part[i];
+        out[0] += factor * v;
+    }
+}
+
+} // namespace
+
+hipError_t launch_denominator_divide(const EnergyArgs &a, double *x, hipStream_t s) {
+    if (a.n <= 0) return hipSuccess;
+    const long long blocks = (a.n + kEnergyThreads - 1) / kEnergyThreads;
+    hipLaunchKernelGGL(denominator_divide_kernel, dim3((unsigned)(blocks < 16384 ? blocks : 16384)),
+                       dim3(kEnergyThreads), 0, s, a, x);
+    return hipGetLastError();
+}
+
+hipError_t launch_energy_sum(const EnergyArgs &a, const double *x, const double *y,
+                             double factor, double *out, double *part, hipStream_t s) {
+    if (a.n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(energy_partial_kernel, dim3(kEnergyBlocks), dim3(kEnergyThreads), 0, s, a,
+                       x, y, part);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(energy_final_kernel, dim3(1), dim3(64), 0, s, out, part, kEnergyBlocks,
+                       factor);
+    return hipGetLastError();
+}
+
+} // namespace sipgpu

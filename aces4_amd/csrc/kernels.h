@@ -98,6 +98,21 @@ hipError_t launch_scalar_mul(double *D, const double *L, const double *R, hipStr
 // Calibration kernels (calib.hip).
 hipError_t launch_mfma_peak(double *out, long long *ticks, int blocks, int iters, hipStream_t s);
 hipError_t launch_copy(const double *in, double *out, size_t n, hipStream_t s);
+// Orbital-energy denominators: rank <= kEnergyMaxRank block, one energy
+// vector and one sign (+1 occupied, -1 virtual) per index.
+constexpr int kEnergyMaxRank = 6;
+constexpr int kEnergyBlocks = 512;
+struct EnergyArgs {
+    int rank;
+    int dims[kEnergyMaxRank];
+    const double *eps[kEnergyMaxRank];
+    int sign[kEnergyMaxRank];
+    long long n;
+};
+hipError_t launch_denominator_divide(const EnergyArgs &a, double *x, hipStream_t s);
+// out[0] += factor * sum_e (x[e] / D(e)) * y[e]; `part` holds kEnergyBlocks doubles.
+hipError_t launch_energy_sum(const EnergyArgs &a, const double *x, const double *y,
+                             double factor, double *out, double *part, hipStream_t s);
 hipError_t launch_fill_random(double *out, long long n, unsigned long long seed,
                               unsigned long long stream, int dist, double a, double b,
                               hipStream_t s);

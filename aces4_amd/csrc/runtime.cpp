@@ -110,7 +110,7 @@ Context &ctx() {
 
 int ensure_stream_scratch(sipgpu_stream_s &s) {
     if (s.scratch) return 0;
-    HIP_TRY(hipMalloc(&s.scratch, kDotBlocks * sizeof(double)));
+    HIP_TRY(hipMalloc(&s.scratch, std::max(kDotBlocks, kEnergyBlocks) * sizeof(double)));
     return 0;
 }
 
@@ -310,6 +310,53 @@ struct TablePool {
     }
 };
 
+// Static XCD schedule of a grouped launch.  Slot t of the launch runs on
+// XCD t % 8, so the tile list is laid out XCD-major.  Whole tasks go to one
+// XCD (their tiles then share its L2), heaviest first onto the least loaded
+// XCD; a task with more than 1/16 of all tiles is first cut into contiguous
+// tile ranges (column-major over its tiles, so a range shares operand
+// panels) of at most that size, so a batch of few large tasks still spans
+// the chip.  Empty slots are task -1.
+template <class Tile>
+std::vector<Tile> xcd_schedule(const std::vector<int> &tm, const std::vector<int> &tn,
+                               const std::vector<double> &cost, int &real_tiles) {
+    const int ntasks = (int)tm.size();
+    long long total = 0;
+    for (int t = 0; t < ntasks; ++t) total += (long long)tm[t] * tn[t];
+    const long long cap = std::max<long long>(1, (total + 2 * kXcds - 1) / (2 * kXcds));
+    struct Unit { int task; long long first, count; double cost; };
+    std::vector<Unit> units;
+    for (int t = 0; t < ntasks; ++t) {
+        const long long nt = (long long)tm[t] * tn[t];
+        const long long pieces = std::max<long long>(1, (nt + cap - 1) / cap);
+        const long long step = (nt + pieces - 1) / pieces;
+        for (long long f = 0; f < nt; f += step) {
+            const long long c = std::min(step, nt - f);
+            units.push_back(Unit{t, f, c, cost[t] * (double)c / (double)std::max<long long>(nt, 1)});
+        }
+    }
+    std::stable_sort(units.begin(), units.end(),
+                     [](const Unit &a, const Unit &b) { return a.cost > b.cost; });
+    std::vector<std::vector<Tile>> per_xcd(kXcds);
+    std::vector<double> load(kXcds, 0.0);
+    real_tiles = 0;
+    for (const Unit &u : units) {
+        const int x = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+        for (long long i = u.first; i < u.first + u.count; ++i) {
+            const int q = (int)(i / tm[u.task]), p = (int)(i % tm[u.task]);
+            per_xcd[x].push_back(Tile{u.task, p, q, 0});
+        }
+        load[x] += u.cost;
+        real_tiles += (int)u.count;
+    }
+    size_t per = 0;
+    for (auto &v : per_xcd) per = std::max(per, v.size());
+    std::vector<Tile> tiles(per * kXcds, Tile{-1, 0, 0, 0});
+    for (int x = 0; x < kXcds; ++x)
+        for (size_t j = 0; j < per_xcd[x].size(); ++j) tiles[j * kXcds + x] = per_xcd[x][j];
+    return tiles;
+}
+
 // No destination may overlap an operand or another destination (partially:
 // entries naming the same block accumulate into it).
 template <class ShapeVec>
@@ -459,28 +506,14 @@ int sipgpu::rt::build_batch_image(const sipgpu_contraction *list, int n, BatchIm
     b.cfg = pick_config(ntasks ? sumM / ntasks : 0, ntasks ? sumN / ntasks : 0);
     const int bm = gemm_tile_m(b.cfg), bn = gemm_tile_n(b.cfg);
 
-    // Static schedule: whole tasks to XCDs, heaviest first onto the least
-    // loaded XCD (their tiles then share that XCD's L2), heaviest tiles
-    // first within an XCD.  Slot t of the launch runs on XCD t % 8.
-    std::vector<int> order(ntasks);
-    for (int i = 0; i < ntasks; ++i) order[i] = i;
-    std::stable_sort(order.begin(), order.end(),
-                     [&](int a, int b2) { return task_cost[a] > task_cost[b2]; });
-    std::vector<std::vector<GemmTile>> per_xcd(kXcds);
-    std::vector<double> load(kXcds, 0.0);
-    for (int ti : order) {
-        const int x = (int)(std::min_element(load.begin(), load.end()) - load.begin());
-        const int tm = (tasks[ti].M + bm - 1) / bm, tn = (tasks[ti].N + bn - 1) / bn;
-        for (int q = 0; q < tn; ++q)
-            for (int p = 0; p < tm; ++p) per_xcd[x].push_back(GemmTile{ti, p, q, 0});
-        load[x] += task_cost[ti];
-        b.real_tiles += tm * tn;
+    std::vector<int> tms(ntasks), tns(ntasks);
+    for (int t = 0; t < ntasks; ++t) {
+        tms[t] = (tasks[t].M + bm - 1) / bm;
+        tns[t] = (tasks[t].N + bn - 1) / bn;
     }
-    size_t per = 0;
-    for (auto &v : per_xcd) per = std::max(per, v.size());
-    std::vector<GemmTile> tiles(per * kXcds, GemmTile{-1, 0, 0, 0});
-    for (int x = 0; x < kXcds; ++x)
-        for (size_t j = 0; j < per_xcd[x].size(); ++j) tiles[j * kXcds + x] = per_xcd[x][j];
+    int real_tiles = 0;
+    const std::vector<GemmTile> tiles = xcd_schedule<GemmTile>(tms, tns, task_cost, real_tiles);
+    b.real_tiles = real_tiles;
 
     // Upload: tiles | tasks | pairs | tables, each 256-byte aligned.
     auto align = [](size_t v) { return (v + 255) & ~(size_t)255; };
@@ -765,27 +798,14 @@ int plan_split_locked(Context &, const sipgpu_contraction *list, int n, int nmod
         for (long long e0 = 0; e0 < mn; e0 += kOzRecChunk) rec.push_back(OzRecItem{t, (int)e0});
         res_elems += mn;
     }
-    // tiles: whole tasks to XCDs, heaviest first (as for the FP64 batch)
-    std::vector<int> order(ntasks);
-    std::iota(order.begin(), order.end(), 0);
-    std::stable_sort(order.begin(), order.end(),
-                     [&](int x, int y) { return task_cost[x] > task_cost[y]; });
-    std::vector<std::vector<OzTile>> per_xcd(kXcds);
-    std::vector<double> load(kXcds, 0.0);
-    int real_tiles = 0;
-    for (int ti : order) {
-        const int x = (int)(std::min_element(load.begin(), load.end()) - load.begin());
-        const int tm = (tasks[ti].M + kOzBM - 1) / kOzBM, tn = (tasks[ti].N + kOzBN - 1) / kOzBN;
-        for (int q = 0; q < tn; ++q)
-            for (int p = 0; p < tm; ++p) per_xcd[x].push_back(OzTile{ti, p, q, 0});
-        load[x] += task_cost[ti];
-        real_tiles += tm * tn;
+    // tiles: the same XCD schedule as the FP64 batch
+    std::vector<int> tms(ntasks), tns(ntasks);
+    for (int t = 0; t < ntasks; ++t) {
+        tms[t] = (tasks[t].M + kOzBM - 1) / kOzBM;
+        tns[t] = (tasks[t].N + kOzBN - 1) / kOzBN;
     }
-    size_t per = 0;
-    for (auto &v : per_xcd) per = std::max(per, v.size());
-    std::vector<OzTile> tiles(per * kXcds, OzTile{-1, 0, 0, 0});
-    for (int x = 0; x < kXcds; ++x)
-        for (size_t j = 0; j < per_xcd[x].size(); ++j) tiles[j * kXcds + x] = per_xcd[x][j];
+    int real_tiles = 0;
+    const std::vector<OzTile> tiles = xcd_schedule<OzTile>(tms, tns, task_cost, real_tiles);
     if (tiles.size() >= (size_t)INT_MAX) return fail(SIPGPU_ESIZE, "split path: too many tiles");
 
     OzConst oc;
@@ -1229,6 +1249,56 @@ int sipgpu_batch_info(sipgpu_batch_t b, double *flops, int *ntasks, int *ntiles,
     if (ntasks) *ntasks = b->ntasks;
     if (ntiles) *ntiles = b->real_tiles;
     if (npairs) *npairs = b->npairs;
+    return 0;
+}
+
+namespace {
+int energy_args(int rank, const int *dims, const double *const *eps, const int *signs,
+                EnergyArgs &a) {
+    if (rank < 1 || rank > kEnergyMaxRank || !dims || !eps || !signs)
+        return fail(SIPGPU_EARG, "energy: rank must be 1..6 with dims, energies and signs");
+    a = EnergyArgs{};
+    a.rank = rank;
+    a.n = 1;
+    for (int d = 0; d < rank; ++d) {
+        if (dims[d] < 1) return fail(SIPGPU_EARG, "energy: extents must be positive");
+        if (signs[d] != 1 && signs[d] != -1) return fail(SIPGPU_EARG, "energy: signs are +1/-1");
+        if (!eps[d]) return fail(SIPGPU_EARG, "energy: null energy vector");
+        a.dims[d] = dims[d];
+        a.eps[d] = eps[d];
+        a.sign[d] = signs[d];
+        a.n *= dims[d];
+    }
+    return 0;
+}
+} // namespace
+
+int sipgpu_energy_denominator(sipgpu_stream_t s, int rank, const int *dims, double *x,
+                              const double *const *eps, const int *signs) {
+    if (!x) return fail(SIPGPU_EARG, "energy: null block");
+    Context &c = ctx();
+    std::lock_guard<std::mutex> g(c.mu);
+    int rc = ensure_init(c);
+    if (rc) return rc;
+    EnergyArgs a;
+    if ((rc = energy_args(rank, dims, eps, signs, a))) return rc;
+    HIP_TRY(launch_denominator_divide(a, x, resolve(c, s).s));
+    return 0;
+}
+
+int sipgpu_energy_sum(sipgpu_stream_t s, int rank, const int *dims, const double *x,
+                      const double *y, const double *const *eps, const int *signs,
+                      double factor, double *out) {
+    if (!x || !y || !out) return fail(SIPGPU_EARG, "energy: null block or result");
+    Context &c = ctx();
+    std::lock_guard<std::mutex> g(c.mu);
+    int rc = ensure_init(c);
+    if (rc) return rc;
+    EnergyArgs a;
+    if ((rc = energy_args(rank, dims, eps, signs, a))) return rc;
+    sipgpu_stream_s &st = resolve(c, s);
+    if ((rc = ensure_stream_scratch(st))) return rc;
+    HIP_TRY(launch_energy_sum(a, x, y, factor, out, st.scratch, st.s));
     return 0;
 }
 

@@ -41,7 +41,7 @@ EXPORTED = [
     "sipgpu_contract", "sipgpu_batch_plan", "sipgpu_batch_execute", "sipgpu_batch_destroy",
     "sipgpu_batch_info", "sipgpu_plan_tables", "sipgpu_plan_permute", "sipgpu_measure_peaks",
     "sipgpu_measured_clock_mhz", "sipgpu_batch_plan_split", "sipgpu_batch_split_info",
-    "sipgpu_fill_random",
+    "sipgpu_fill_random", "sipgpu_energy_denominator", "sipgpu_energy_sum",
     "sipgpu_cache_create", "sipgpu_cache_destroy", "sipgpu_cache_insert", "sipgpu_cache_acquire",
     "sipgpu_cache_release", "sipgpu_cache_read", "sipgpu_cache_flush", "sipgpu_cache_evict",
     "sipgpu_cache_get_stats", "sipgpu_server_create", "sipgpu_server_submit",
@@ -160,6 +160,10 @@ def lib():
                                            ctypes.POINTER(ctypes.c_double)]
         L.sipgpu_server_get_stats.argtypes = [_vp, ctypes.POINTER(ServerStats)]
         L.sipgpu_server_destroy.argtypes = [_vp]
+        _pp = ctypes.POINTER(ctypes.c_void_p)
+        L.sipgpu_energy_denominator.argtypes = [_vp, ctypes.c_int, _ip, _vp, _pp, _ip]
+        L.sipgpu_energy_sum.argtypes = [_vp, ctypes.c_int, _ip, _vp, _vp, _pp, _ip,
+                                        ctypes.c_double, _vp]
         L.sipgpu_fill_random.argtypes = [_vp, _vp, ctypes.c_longlong, ctypes.c_ulonglong,
                                          ctypes.c_ulonglong, ctypes.c_int, ctypes.c_double,
                                          ctypes.c_double]
@@ -295,6 +299,27 @@ def fill_random(ptr, n, seed, stream, dist="uniform", a=0.0, b=1.0, s=None):
     d = {"uniform": 0, "normal": 1}[dist]
     _check(lib().sipgpu_fill_random(None if s is None else s.handle, ptr, int(n), int(seed),
                                     int(stream), d, float(a), float(b)))
+
+
+def _energy_arrays(dims, eps, signs):
+    r = len(dims)
+    d = (ctypes.c_int * r)(*[int(x) for x in dims])
+    e = (ctypes.c_void_p * r)(*[int(x) for x in eps])
+    sg_ = (ctypes.c_int * r)(*[int(x) for x in signs])
+    return r, d, e, sg_
+
+
+def energy_denominator(dims, x, eps, signs, s=None):
+    """x /= D in place (device block x, device energy vectors eps[d])."""
+    r, d, e, sg_ = _energy_arrays(dims, eps, signs)
+    _check(lib().sipgpu_energy_denominator(None if s is None else s.handle, r, d, x, e, sg_))
+
+
+def energy_sum(dims, x, y, eps, signs, factor, out, s=None):
+    """*out += factor * sum (x / D) * y   (out: device address of one double)."""
+    r, d, e, sg_ = _energy_arrays(dims, eps, signs)
+    _check(lib().sipgpu_energy_sum(None if s is None else s.handle, r, d, x, y, e, sg_,
+                                   float(factor), out))
 
 
 def measured_clock_mhz():

@@ -1,0 +1,141 @@
+"""(T)-style triples: W blocks from twelve permuted contractions, and their
+energy through the orbital-energy denominator.
+
+For a block tuple (A, B, C, I, J, K) of segments, the W block (indices
+a, b, c, i, j, k, column-major in that order) is
+
+    W = sum over the 6 permutations P of the pairs (a,i), (b,j), (c,k) of
+        sum_d V(b,c,d,k) T(a,d,i,j)  +  sum_l V'(l,c,j,k) T(a,b,i,l)
+
+with the labels permuted by P (V' holds -V(l,c,j,k)), and the tuple's
+energy is  E = factor * sum W^2 / D,  D = e_i + e_j + e_k - e_a - e_b - e_c.
+This is the structure of the reference's rccsdpt_aaa.sialx: X blocks
+accumulated from contractions, energy_denominator_rhf applied to the block
+(energy_denominator_rhf.F:399-444), then contracted with itself to a scalar
+(rccsdpt_aaa.sialx:2164-2174).
+
+On the device every (permutation, term) pair is one planned batch over the
+d (or l) segments: the twelve batches accumulate into one W buffer in
+stream order (each has its own row / column grouping of W's six indices, so
+they cannot share a launch), then one fused pass divides by D, squares and
+reduces, without writing the divided block.
+"""
+import itertools
+
+import numpy as np
+
+from . import sipgpu as sg
+from .workloads import DeviceBlocks, uniform
+
+PAIRS = (("a", "i"), ("b", "j"), ("c", "k"))
+W_LABELS = "abcijk"
+W_SIGNS = (-1, -1, -1, 1, 1, 1)
+
+
+def term_labels():
+    """The twelve (left labels, right labels, left array, right array,
+    summed label) of W, in a fixed order."""
+    out = []
+    for perm in itertools.permutations(range(3)):
+        (x1, y1), (x2, y2), (x3, y3) = (PAIRS[p] for p in perm)
+        out.append((x2 + x3 + "d" + y3, x1 + "d" + y1 + y2, "V1", "T", "d"))
+        out.append(("l" + x3 + y2 + y3, x1 + x2 + y1 + "l", "V2", "T", "l"))
+    return out
+
+
+def triples(o_segs=(20, 10), v_segs=(20,) * 12, seed=5):
+    """Arrays of the triples term, generated in HBM per block: V1(b,c,d,k)
+    ~ N(0, 0.1^2), V2(l,c,j,k) ~ N(0, 0.1^2) (held as -V), T(a,d,i,j) ~
+    N(0, 1e-2^2); orbital energies e_occ ~ U(-2, -0.3), e_vir ~ U(0.1, 3)."""
+    o, v = sum(o_segs), sum(v_segs)
+    kinds = {"v": list(v_segs), "o": list(o_segs)}
+    zero = {"v": [0] * len(v_segs), "o": [0] * len(o_segs)}
+
+    def arr(name, kind, tag, sigma):
+        return DeviceBlocks(name, [kinds[x] for x in kind], [zero[x] for x in kind], seed, tag,
+                            "normal", sigma)
+
+    w = dict(V1=arr("V1", "vvvo", 11, 0.1), V2=arr("V2", "ovoo", 12, 0.1),
+             T=arr("T", "vvoo", 13, 1e-2), o_segs=list(o_segs), v_segs=list(v_segs), seed=seed)
+    w["e_occ"] = uniform(seed, o, stream=21, lo=-2.0, hi=-0.3)
+    w["e_vir"] = uniform(seed, v, stream=22, lo=0.1, hi=3.0)
+    w["kind"] = {"a": "v", "b": "v", "c": "v", "d": "v", "i": "o", "j": "o", "k": "o",
+                 "l": "o"}
+    return w
+
+
+def to_device(w):
+    for k in ("V1", "V2", "T"):
+        if w[k].dev is None:
+            w[k].to_device()
+    if "e_occ_dev" not in w:
+        w["e_occ_dev"] = sg.DeviceArray.from_host(w["e_occ"])
+        w["e_vir_dev"] = sg.DeviceArray.from_host(w["e_vir"])
+
+
+def seg_offset(segs, s):
+    return int(sum(segs[:s]))
+
+
+def tuple_dims(w, tup):
+    segs = {"v": w["v_segs"], "o": w["o_segs"]}
+    return [segs[w["kind"][lab]][s] for lab, s in zip(W_LABELS, tup)]
+
+
+def tuple_entries(w, tup):
+    """For W block `tup` = (A, B, C, I, J, K): per term, the contraction
+    pattern and the (left block, right block) pairs over the summed
+    segments."""
+    idx = dict(zip(W_LABELS, tup))
+    nseg = {"d": len(w["v_segs"]), "l": len(w["o_segs"])}
+    terms = []
+    for ll, rl, ln, rn, summed in term_labels():
+        labels = list(dict.fromkeys(W_LABELS + ll + rl))
+        code = {lab: 1 + i for i, lab in enumerate(labels)}
+        ptrn, e = sg.get_contraction_ptrn(6, 4, 4, [code[x] for x in W_LABELS + ll + rl])
+        assert e == 0
+        pairs = []
+        for s in range(nseg[summed]):
+            idx[summed] = s
+            pairs.append((ln, tuple(idx[x] for x in ll), rn, tuple(idx[x] for x in rl)))
+        terms.append(([int(x) for x in ptrn], pairs))
+    return terms
+
+
+class TupleEnergy:
+    """Planned device work for one W block: twelve batches into `wbuf` and
+    the fused denominator / energy reduction into `out` (device double)."""
+
+    def __init__(self, w, tup, wbuf, out, moduli=None, factor=1.0):
+        to_device(w)
+        self.dims = tuple_dims(w, tup)
+        self.batches = []
+        first = True
+        for ptrn, pairs in tuple_entries(w, tup):
+            cs = []
+            for ln, lb, rn, rb in pairs:
+                L, R = w[ln], w[rn]
+                cs.append(sg.make_contraction(ptrn, L.addr(lb), L.dims(lb), R.addr(rb),
+                                              R.dims(rb), wbuf, self.dims,
+                                              accumulate=not first))
+                first = False
+            self.batches.append(sg.Batch(cs, moduli=moduli))
+        segs = {"v": w["v_segs"], "o": w["o_segs"]}
+        self.eps = []
+        for lab, s in zip(W_LABELS, tup):
+            kind = w["kind"][lab]
+            base = w["e_vir_dev"] if kind == "v" else w["e_occ_dev"]
+            self.eps.append(base.at(seg_offset(segs[kind], s)))
+        self.wbuf, self.out, self.factor = wbuf, out, factor
+        self.flops = sum(b.flops for b in self.batches)
+
+    def execute(self, s=None):
+        for b in self.batches:
+            b.execute(s)
+        sg.energy_sum(self.dims, self.wbuf, self.wbuf, self.eps, W_SIGNS, self.factor, self.out,
+                      s=s)
+
+    def destroy(self):
+        for b in self.batches:
+            b.destroy()
+        self.batches = []

@@ -45,6 +45,7 @@ def parse():
     p.add_argument("--no-split", action="store_true")
     p.add_argument("--no-sparse", action="store_true")
     p.add_argument("--no-serve", action="store_true")
+    p.add_argument("--no-triples", action="store_true")
     return p.parse_args()
 
 
@@ -279,6 +280,59 @@ def serving_bench(label, budget_gb, n, rate, seg_lo, seg_hi, max_batch=256):
     }
 
 
+def triples_bench(tuples=((0, 1, 2, 0, 0, 0), (3, 4, 5, 0, 0, 0), (6, 7, 8, 0, 0, 0)),
+                  reps=2, moduli=(None, 14)):
+    """Triples (o = 30 in segments 20,10; v = 240 in 12 x 20): full-size W
+    blocks (20^6 elements, 512 MB) from the twelve permuted contractions,
+    then the fused denominator / energy reduction; W is never divided or
+    written back.  FP64-equivalent TFLOP/s over the sampled tuples, and the
+    time the whole unrestricted (T) sum would take at that rate."""
+    from aces4_amd import triples as tp
+    w = tp.triples()
+    tp.to_device(w)
+    nmax = max(int(np.prod(tp.tuple_dims(w, t))) for t in tuples)
+    wbuf = sg.DeviceArray(nmax)
+    out = sg.DeviceArray(1)
+    v, o = sum(w["v_segs"]), sum(w["o_segs"])
+    whole = 12.0 * (v ** 3) * (o ** 3) * (v + o)
+    res = {"tuples": [list(t) for t in tuples], "w_block_elems": nmax,
+           "whole_T_flop": whole}
+    energies = {}
+    for nm in moduli:
+        tes = [tp.TupleEnergy(w, t, wbuf.ptr, out.ptr, moduli=nm) for t in tuples]
+        flops = sum(te.flops for te in tes)
+        out.zero()
+        for te in tes:
+            te.execute()
+        sg.synchronize()
+        energies[nm] = out.download()[0]
+        e0, e1 = sg.Event(), sg.Event()
+        e0.record()
+        for _ in range(reps):
+            for te in tes:
+                te.execute()
+        e1.record()
+        e1.synchronize()
+        ms = e0.elapsed_ms(e1) / reps
+        key = "fp64" if nm is None else "split_%d" % nm
+        res[key] = {"ms_per_tuple": ms / len(tes), "tflops_fp64_equiv": flops / (ms * 1e-3) / 1e12,
+                    "flop_per_tuple": flops / len(tes),
+                    "whole_T_seconds_at_this_rate": whole / (flops / (ms * 1e-3)),
+                    "energy_sampled": energies[nm]}
+        for te in tes:
+            te.destroy()
+    if None in energies and len(energies) > 1:
+        for nm, e in energies.items():
+            if nm is not None:
+                res["split_%d" % nm]["energy_rel_err_vs_fp64"] = abs(e - energies[None]) / abs(
+                    energies[None])
+    wbuf.free()
+    out.free()
+    for k in ("V1", "V2", "T"):
+        w[k].dev.free()
+    return res
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -316,6 +370,7 @@ def main():
         best = max(v["tflops_fp64_equiv"] for v in split.values())
         split["speedup_vs_plain"] = best / achieved
     sparse = None if a.no_sparse else sparse_bench()
+    trip = None if a.no_triples else triples_bench()
     serving = None
     if not a.no_serve:
         serving = [serving_bench("resident (HBM budget 200 GB)", 200.0, 3000, 2000.0, 8, 64),
@@ -361,6 +416,7 @@ def main():
         "sparse_ladder": sparse,
         "latency": lat,
         "serving": serving,
+        "triples": trip,
         "cpu_baseline": cpu,
     }
     print(json.dumps(res))

@@ -176,6 +176,26 @@ int sipgpu_batch_info(sipgpu_batch_t batch, double *flops, int *ntasks,
                       int *ntiles, int *npairs);
 
 /* ------------------------------------------------------------------------
+ * Orbital-energy denominators (energy_denominator_rhf.F:15-352, the SIP's
+ * energy_denominator_rhf super-instruction, generalised to any rank <= 6).
+ * Block x (device, column-major, extents dims) has one orbital-energy vector
+ * per index, eps[d] (device, dims[d] entries: that index's segment), and a
+ * sign per index, +1 occupied / -1 virtual.  The denominator of an element
+ * is its occupied energies added in index order, then its virtual energies
+ * subtracted in index order (e_b + e_d + e_f - e_a - e_c - e_e for the
+ * reference's (a,i,b,j,c,k) blocks, do_rhf_den6 :399-444). */
+int sipgpu_energy_denominator(sipgpu_stream_t s, int rank, const int *dims, double *x,
+                              const double *const *eps, const int *signs);
+/* *out += factor * sum_e (x[e] / D(e)) * y[e], enqueued on `s`; `out` is a
+ * device double.  With y = x this is the (T) energy of a W block, i.e. the
+ * denominator division then the contract-to-scalar of rccsdpt_aaa.sialx
+ * :2166-2174 in one pass, without writing the divided block.  Fixed
+ * summation order (deterministic). */
+int sipgpu_energy_sum(sipgpu_stream_t s, int rank, const int *dims, const double *x,
+                      const double *y, const double *const *eps, const int *signs,
+                      double factor, double *out);
+
+/* ------------------------------------------------------------------------
  * Device block cache.  Replaces the SIP's BlockManager / CachedBlockMap /
  * LRUArrayPolicy residency logic (block_manager.cpp:106-206,
  * cached_block_map.cpp:31-147, lru_array_policy.h:37-96) for blocks in

@@ -290,3 +290,40 @@ void orc_tensor_block_contract(const int *ptrn, const double *l, int lr,
     free(rtmp);
     free(dtmp);
 }
+
+/* energy_denominator_rhf.F:399-444 (do_rhf_den6; do_rhf_den4 :310-352 and
+ * do_rhf_den2 :255-300 are the same rule at lower rank): loop over the
+ * block in memory order, add the occupied energies left to right, subtract
+ * the virtual ones left to right, divide. */
+void orc_energy_denominator(int rank, const int *dims, double *x, const double *const *eps,
+                            const int *signs) {
+    long long n = 1;
+    for (int d = 0; d < rank; ++d) n *= dims[d];
+    int idx[ORC_MAX_RANK] = {0};
+    for (long long e = 0; e < n; ++e) {
+        double den = 0.0;
+        int first = 1;
+        for (int d = 0; d < rank; ++d)
+            if (signs[d] > 0) {
+                den = first ? eps[d][idx[d]] : den + eps[d][idx[d]];
+                first = 0;
+            }
+        for (int d = 0; d < rank; ++d)
+            if (signs[d] < 0) {
+                den = first ? -eps[d][idx[d]] : den - eps[d][idx[d]];
+                first = 0;
+            }
+        x[e] = x[e] / den;
+        for (int d = 0; d < rank; ++d) { /* odometer, first index fastest */
+            if (++idx[d] < dims[d]) break;
+            idx[d] = 0;
+        }
+    }
+}
+
+/* tensor_dil_omp.F90:910-938 */
+double orc_fcontract(long long n, const double *l, const double *r, double d) {
+    double val = 0.0;
+    for (long long i = 0; i < n; ++i) val = val + l[i] * r[i];
+    return d + val;
+}

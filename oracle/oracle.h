@@ -76,6 +76,20 @@ void orc_tensor_block_contract(const int *ptrn, const double *l, int lrank,
 void orc_tensor_block_add(int rank, const int *dims, double *t0,
                           const double *t1, double scale, int *ierr);
 
+/* energy_denominator_rhf (src/sip/super_instructions/qm/qm-generic/
+ * energy_denominator_rhf.F:15-352): x /= eps in place, where for element
+ * (i_0 .. i_{rank-1}) eps adds eps_d[i_d] over the occupied indices
+ * (sign_d = +1) in index order, then subtracts eps_d[i_d] over the virtual
+ * ones (sign_d = -1) in index order (do_rhf_den6 :399-444: eps = epsb +
+ * epsd + epsf - epsa - epsc - epse). */
+void orc_energy_denominator(int rank, const int *dims, double *x, const double *const *eps,
+                            const int *signs);
+
+/* tensor_block_fcontract (tensor_dil_omp.F90:910-938): returns
+ * d + sum_l l[l] * r[l] over n elements, summed in ascending order (the
+ * reference reduces with OpenMP, whose order is unspecified). */
+double orc_fcontract(long long n, const double *l, const double *r, double d);
+
 #ifdef __cplusplus
 }
 #endif

@@ -85,6 +85,22 @@ class Oracle:
                                            len(ddims), _iptr(dd), ctypes.byref(ierr))
         return d, ierr.value
 
+    def energy_denominator(self, dims, x, eps, signs):
+        """x / eps (energy_denominator_rhf), eps per index from `eps[d]`."""
+        x = _f64(x).ravel().copy()
+        ev = [np.ascontiguousarray(e, dtype=np.float64) for e in eps]
+        ptrs = (ctypes.c_void_p * len(dims))(*[e.ctypes.data for e in ev])
+        d, sg_ = _ints(dims), _ints(signs)
+        self.lib.orc_energy_denominator(len(dims), _iptr(d), _dptr(x), ptrs, _iptr(sg_))
+        return x
+
+    def fcontract(self, l, r, d=0.0):
+        l, r = _f64(l).ravel(), _f64(r).ravel()
+        self.lib.orc_fcontract.restype = ctypes.c_double
+        self.lib.orc_fcontract.argtypes = [ctypes.c_longlong, ctypes.c_void_p,
+                                           ctypes.c_void_p, ctypes.c_double]
+        return self.lib.orc_fcontract(l.size, l.ctypes.data, r.ctypes.data, float(d))
+
     def perms(self, ptrn, ldims, rdims, ddims):
         """determine_index_permutations: (lo2n, ro2n, do2n, flags, M, N, K)."""
 
