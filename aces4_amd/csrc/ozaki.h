@@ -9,11 +9,12 @@ namespace sipgpu {
 
 constexpr int kOzMaxModuli = 16;
 constexpr int kOzBM = 128, kOzBN = 128; // INT8 GEMM workgroup tile
+constexpr int kOzBigTile = 256;         // the large-task GEMM's tile (both dims)
 constexpr int kOzChunk = 16;            // bytes of k per panel chunk
 constexpr int kOzStage = 128;           // bytes of k per GEMM pipeline stage
 constexpr int kOzStageChunks = kOzStage / kOzChunk;
 constexpr int kOzZero = 256;            // zero bytes at the start of every plane
-constexpr int kOzTileX = 64, kOzTileK = 64; // conversion tile (rows x k)
+constexpr int kOzTileX = 32, kOzTileK = 128; // conversion tile (rows x k)
 constexpr int kOzRecChunk = 2048;       // result elements per reconstruction workgroup
 
 // Moduli and reconstruction constants (host-computed).
@@ -22,6 +23,10 @@ struct OzConst {
     int P;                        // bits of the scaled integers (|A_int| <= 2^(P-1))
     int p[kOzMaxModuli];          // pairwise coprime, <= 255
     int u[kOzMaxModuli];          // (M / p_i)^-1 mod p_i
+    int c17[kOzMaxModuli];        // 2^17 mod p_i
+    int c34[kOzMaxModuli];        // 2^34 mod p_i
+    int off[kOzMaxModuli];        // p_i * 2^19: makes the limb sum positive
+    uint32_t bar[kOzMaxModuli];   // floor(2^32 / p_i) (Barrett reciprocal)
     double rp[kOzMaxModuli];      // 1 / p_i (rounded)
     uint64_t mi[kOzMaxModuli][2]; // M / p_i exactly, {low, high} 64-bit halves
     uint64_t m[2];                // M = prod p_i exactly (< 2^112)
@@ -63,7 +68,7 @@ struct OzTile {
     int task, mt, nt, pad;
 };
 
-// Conversion work item: rows [x0, x0 + 64) and k [k0, k0 + 64) of a block.
+// Conversion work item: rows [x0, x0 + kOzTileX) and k [k0, k0 + kOzTileK) of a block.
 struct OzConvTile {
     int block, x0, k0, pad;
 };
@@ -82,6 +87,7 @@ struct OzGemmArgs {
     uint8_t *res_planes;
     long long res_plane_bytes;
     int ntiles;
+    int tile; // kOzBM (= kOzBN) or kOzBigTile
     OzConst c;
 };
 
@@ -108,12 +114,13 @@ struct OzRecArgs {
 };
 
 hipError_t oz_fill_int(int *p, int n, int v, hipStream_t s);
-// Row maxima -> exponents, one workgroup per 64-row tile of a block
+// Row maxima -> exponents, one workgroup per kOzTileX-row tile of a block
 // (`row_items`, all with k0 = 0); each sweeps the block's whole K.
 hipError_t oz_row_exponents(const OzConvArgs &a, const OzConvTile *row_items, int nrow_items,
                             hipStream_t s);
 hipError_t oz_to_residues(const OzConvArgs &a, hipStream_t s);
 hipError_t oz_gemm(const OzGemmArgs &a, hipStream_t s);
+hipError_t oz_gemm_big(const OzGemmArgs &a, hipStream_t s);
 hipError_t oz_reconstruct(const OzRecArgs &a, hipStream_t s);
 
 } // namespace sipgpu

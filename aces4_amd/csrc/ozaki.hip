@@ -37,60 +37,87 @@ __global__ void fill_int_kernel(int *p, int n, int v) {
         p[i] = v;
 }
 
-// Rows [x0, x0 + 64) x k [k0, k0 + 64) of block b into S[x][k], reading
-// along the index that is adjacent in memory; entries outside the block
-// read as 0.
-__device__ __forceinline__ void load_subtile(const OzBlock &b, const int32_t *tab, int x0, int k0,
-                                             double (*S)[kOzTileK + 1], int tid) {
+// Rows [x0, x0 + 32) x k [k0, k0 + 128) of block b: each thread loads 16
+// values along the index that is adjacent in memory (32 lanes on 32
+// consecutive rows, or 128 on consecutive k); entries outside the block read
+// as 0.  Thread t holds (row, k) = (t % 32, t / 32 + 8 j) when rows are
+// adjacent, else (t / 128 + 2 j, t % 128).
+struct SubTile {
+    double v[16];
+};
+
+__device__ __forceinline__ void fetch_subtile(const OzBlock &b, const int32_t *tab, int x0, int k0,
+                                              int tid, SubTile &t) {
     if (b.xfast) {
-        const int x = x0 + (tid & 63);
+        const int x = x0 + (tid & 31);
         const bool xin = x < b.X;
         const int xo = xin ? tab[b.xtab + x] : 0;
-#pragma unroll 4
+#pragma unroll
         for (int j = 0; j < 16; ++j) {
-            const int kk = (tid >> 6) + 4 * j, k = k0 + kk;
-            double v = 0.0;
-            if (xin && k < b.K) v = b.data[tab[b.ktab + k] + xo];
-            S[tid & 63][kk] = v;
+            const int k = k0 + (tid >> 5) + 8 * j;
+            t.v[j] = (xin && k < b.K) ? b.data[tab[b.ktab + k] + xo] : 0.0;
         }
     } else {
-        const int k = k0 + (tid & 63);
+        const int k = k0 + (tid & 127);
         const bool kin = k < b.K;
         const int ko = kin ? tab[b.ktab + k] : 0;
-#pragma unroll 4
+#pragma unroll
         for (int j = 0; j < 16; ++j) {
-            const int xx = (tid >> 6) + 4 * j, x = x0 + xx;
-            double v = 0.0;
-            if (kin && x < b.X) v = b.data[ko + tab[b.xtab + x]];
-            S[xx][tid & 63] = v;
+            const int x = x0 + (tid >> 7) + 2 * j;
+            t.v[j] = (kin && x < b.X) ? b.data[ko + tab[b.xtab + x]] : 0.0;
         }
     }
 }
 
-// One workgroup per 64-row tile of a block: max |value| of each row over
+__device__ __forceinline__ void store_subtile(const OzBlock &b, int tid, const SubTile &t,
+                                              double (*S)[kOzTileK + 1]) {
+    if (b.xfast) {
+#pragma unroll
+        for (int j = 0; j < 16; ++j) S[tid & 31][(tid >> 5) + 8 * j] = t.v[j];
+    } else {
+#pragma unroll
+        for (int j = 0; j < 16; ++j) S[(tid >> 7) + 2 * j][tid & 127] = t.v[j];
+    }
+}
+
+// One workgroup per 32-row tile of a block: max |value| of each row over
 // the block's K, as a binary exponent folded into the row's component with
-// atomicMax (order-independent, so the result is deterministic).
+// atomicMax (order-independent, so the result is deterministic).  Maxima
+// stay in registers across the K sweep (two sub-tiles in flight per step);
+// one LDS reduction per tile folds them per row.
 __global__ __launch_bounds__(256) void row_exponent_kernel(OzConvArgs a, const OzConvTile *items,
                                                            int nitems) {
-    __shared__ double S[kOzTileX][kOzTileK + 1];
-    __shared__ double part[4][kOzTileX];
+    __shared__ double part[kOzTileX][129];
     const int tid = threadIdx.x;
     for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
         const OzConvTile w = items[it];
         const OzBlock b = a.blocks[w.block];
-        double mx = 0.0;
-        for (int k0 = 0; k0 < b.K; k0 += kOzTileK) {
-            __syncthreads();
-            load_subtile(b, a.tab, w.x0, k0, S, tid);
-            __syncthreads();
-            const int x = tid & 63, kg = (tid >> 6) * 16;
+        double mx[16];
 #pragma unroll
-            for (int j = 0; j < 16; ++j) mx = fmax(mx, fabs(S[x][kg + j]));
+        for (int j = 0; j < 16; ++j) mx[j] = 0.0;
+        for (int k0 = 0; k0 < b.K; k0 += 2 * kOzTileK) {
+            SubTile t0, t1;
+            fetch_subtile(b, a.tab, w.x0, k0, tid, t0);
+            fetch_subtile(b, a.tab, w.x0, k0 + kOzTileK, tid, t1);
+#pragma unroll
+            for (int j = 0; j < 16; ++j) mx[j] = fmax(mx[j], fmax(fabs(t0.v[j]), fabs(t1.v[j])));
         }
-        part[tid >> 6][tid & 63] = mx;
+        // rows adjacent: thread holds row tid%32 for 16 k; else 16 rows
+        __syncthreads();
+        if (b.xfast) {
+            double m = 0.0;
+#pragma unroll
+            for (int j = 0; j < 16; ++j) m = fmax(m, mx[j]);
+            part[tid & 31][tid >> 5] = m;
+        } else {
+#pragma unroll
+            for (int j = 0; j < 16; ++j) part[(tid >> 7) + 2 * j][tid & 127] = mx[j];
+        }
         __syncthreads();
         if (tid < kOzTileX) {
-            const double m = fmax(fmax(part[0][tid], part[1][tid]), fmax(part[2][tid], part[3][tid]));
+            const int cols = b.xfast ? 8 : 128;
+            double m = 0.0;
+            for (int c = 0; c < cols; ++c) m = fmax(m, part[tid][c]);
             const int x = w.x0 + tid;
             if (x < b.X && m > 0.0) {
                 int e;
@@ -98,13 +125,13 @@ __global__ __launch_bounds__(256) void row_exponent_kernel(OzConvArgs a, const O
                 atomicMax(&a.expo[b.exp_off + x], e);
             }
         }
-        __syncthreads();
     }
 }
 
 // Residue of an exact integer |v| < 2^52 held in a double, in [0, p):
 // q = floor(v / p) from the rounded reciprocal is off by at most one, and
-// v - p q is exact (fma of integers below 2^53).
+// v - p q is exact (fma of integers below 2^53).  (The residue panels use
+// the cheaper limb / Barrett form below; this serves the GEMM epilogue.)
 __device__ __forceinline__ int mod_exact(double v, double p, double rp) {
     const double q = floor(v * rp);
     double r = fma(-p, q, v);
@@ -113,36 +140,63 @@ __device__ __forceinline__ int mod_exact(double v, double p, double rp) {
     return (int)r;
 }
 
-// One workgroup per 64 x 64 conversion tile: scaled integers of the tile,
-// reduced to balanced residues (|r| <= 127) for every modulus, written as
-// 16-byte row pieces of the block's panel in each modulus plane.  Panel
-// bytes for k in [K, Kp) are written as zeros.
+// Persistent workgroups over 32 x 128 conversion tiles: the next tile's
+// values are loaded into registers while the current one (transposed
+// through LDS) is converted to balanced residues (|r| <= 127) for every
+// modulus and written as 16-byte row pieces of the block's panel in each
+// modulus plane.  Panel bytes for k in [K, Kp) are written as zeros.
 __global__ __launch_bounds__(256) void to_residues_kernel(OzConvArgs a) {
     __shared__ double S[kOzTileX][kOzTileK + 1];
     const int tid = threadIdx.x;
-    for (int it = blockIdx.x; it < a.nitems; it += gridDim.x) {
-        const OzConvTile w = a.items[it];
-        const OzBlock b = a.blocks[w.block];
+    int it = blockIdx.x;
+    if (it >= a.nitems) return;
+    SubTile next;
+    OzConvTile w = a.items[it];
+    OzBlock b = a.blocks[w.block];
+    fetch_subtile(b, a.tab, w.x0, w.k0, tid, next);
+    for (; it < a.nitems; it += gridDim.x) {
+        __syncthreads(); // the previous tile's LDS reads are done
+        store_subtile(b, tid, next, S);
         __syncthreads();
-        load_subtile(b, a.tab, w.x0, w.k0, S, tid);
-        __syncthreads();
-        const int xx = tid >> 2, kc = (tid & 3) * 16;
-        const int x = w.x0 + xx, k = w.k0 + kc;
-        if (x >= b.X || k >= b.Kp) continue;
-        const int e = a.expo[b.exp_off + x];
+        const OzConvTile cw = w;
+        const OzBlock cb = b;
+        const int nit = it + gridDim.x;
+        if (nit < a.nitems) {
+            w = a.items[nit];
+            b = a.blocks[w.block];
+            fetch_subtile(b, a.tab, w.x0, w.k0, tid, next);
+        }
+        const int xx = tid >> 3, kc = (tid & 7) * 16; // 8 lanes: one 128-byte row piece
+        const int x = cw.x0 + xx, k = cw.k0 + kc;
+        if (x >= cb.X || k >= cb.Kp) continue;
+        const int e = a.expo[cb.exp_off + x];
         const int shift = e == INT_MIN ? 0 : (a.c.P - 1) - e;
-        double v[16];
+        // each scaled integer v (|v| <= 2^51, exact in FP64) split once into
+        // v = h2 2^34 + h1 2^17 + h0 with h0, h1 in [0, 2^17), |h2| <= 2^17
+        int h0[16], h1[16], h2[16];
 #pragma unroll
-        for (int j = 0; j < 16; ++j) v[j] = e == INT_MIN ? 0.0 : rint(ldexp(S[xx][kc + j], shift));
-        int8_t *dst = a.planes + b.panel_off + (long long)x * b.Kp + k;
+        for (int j = 0; j < 16; ++j) {
+            const double v = e == INT_MIN ? 0.0 : rint(ldexp(S[xx][kc + j], shift));
+            const double d2 = floor(v * 0x1p-34);
+            const double r1 = fma(-d2, 0x1p34, v); // exact, in [0, 2^34)
+            const double d1 = floor(r1 * 0x1p-17);
+            h2[j] = (int)d2;
+            h1[j] = (int)d1;
+            h0[j] = (int)fma(-d1, 0x1p17, r1);
+        }
+        int8_t *dst = a.planes + cb.panel_off + (long long)x * cb.Kp + k;
         for (int i = 0; i < a.c.n; ++i) {
-            const int p = a.c.p[i];
-            const double pd = p, rp = a.c.rp[i];
+            const int p = a.c.p[i], c17 = a.c.c17[i], c34 = a.c.c34[i], off = a.c.off[i];
+            const uint32_t bar = a.c.bar[i];
             uint32_t word[4] = {0u, 0u, 0u, 0u};
 #pragma unroll
             for (int j = 0; j < 16; ++j) {
-                int r = mod_exact(v[j], pd, rp);
-                if (r > p / 2) r -= p;
+                // t = v mod p up to a multiple of p, in (0, 2^28): 24-bit products
+                const int t = __mul24(h2[j], c34) + __mul24(h1[j], c17) + h0[j] + off;
+                const uint32_t q = __umulhi((uint32_t)t, bar); // floor(t/p) or one less
+                int r = t - __mul24((int)q, p);                  // in [0, 2p)
+                if (r >= p) r -= p;
+                if (r > p / 2) r -= p; // balanced, |r| <= 127
                 word[j >> 2] |= (uint32_t)(uint8_t)(int8_t)r << (8 * (j & 3));
             }
             *reinterpret_cast<int32x4 *>(dst + (long long)i * a.plane_bytes) =
@@ -153,9 +207,10 @@ __global__ __launch_bounds__(256) void to_residues_kernel(OzConvArgs a) {
 
 // INT8 GEMM over residue panels, one modulus per blockIdx.y.  Workgroup tile
 // BM x BN (4 waves, 2 x 2, each (BM/2) x (BN/2)); the task's virtual K runs
-// in stages of 128 bytes = 8 chunks, double-buffered in LDS: the loads of
-// stage s + 1 (16 bytes per lane into registers) are in flight while stage
-// s is multiplied, then written to the other buffer.  LDS rows hold 8 chunk
+// in stages of 128 bytes = 8 chunks, double-buffered in LDS; loads run two
+// stages ahead through two register sets (16 bytes per lane each): the
+// loads of stage s + 2 are in flight while stage s is multiplied, and
+// stage s + 1 is written to the other buffer after it.  LDS rows hold 8 chunk
 // slots, chunk c of row r in slot c ^ (r & 7), so both the stores (one wave
 // = 8 whole rows) and the fragment reads (16 rows x 4 chunks) spread over
 // all banks.  Rows past the task's M / N read row M-1 /
@@ -171,8 +226,11 @@ __global__ __launch_bounds__(256, 2) void gemm_i8_kernel(OzGemmArgs a) {
     constexpr int IA = BM / 32, IB = BN / 32; // 8-row load instructions per wave per stage
     static_assert(CH == 8 && KS == 128, "a stage is 8 chunks of 16 bytes");
     static_assert(BM % 32 == 0 && BN % 32 == 0 && TM % 16 == 0 && TN % 16 == 0, "tile shape");
+    constexpr int kWin = 512; // chunk entries per LDS window (64 stages)
+    static_assert(kWin % (2 * CH) == 0, "a window holds whole pairs of stages");
     __shared__ __attribute__((aligned(1024))) int8_t As[2][BM * KS];
     __shared__ __attribute__((aligned(1024))) int8_t Bs[2][BN * KS];
+    __shared__ __attribute__((aligned(16))) OzChunkRef Es[kWin];
 
     const int tid = threadIdx.x, lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
@@ -202,20 +260,11 @@ __global__ __launch_bounds__(256, 2) void gemm_i8_kernel(OzGemmArgs a) {
             for (int j = 0; j < FN; ++j) acc[i][j] = int32x4{0, 0, 0, 0};
 
         const OzChunkRef *ch = a.chunks + task.chunk0;
-        const int nst = task.nchunks / CH;
-        int32x4 va[IA], vb[IB];
-        // stage registers: this lane's 16 bytes of 8-row groups of A and B
-        auto fetch = [&](const OzChunkRef &e) {
-#pragma unroll
-            for (int i = 0; i < IA; ++i)
-                va[i] = *reinterpret_cast<const __attribute__((address_space(1))) int32x4 *>(
-                    plane + e.a + ra[i] * e.stride);
-#pragma unroll
-            for (int i = 0; i < IB; ++i)
-                vb[i] = *reinterpret_cast<const __attribute__((address_space(1))) int32x4 *>(
-                    plane + e.b + rb[i] * e.stride);
-        };
-        auto put = [&](int buf) {
+        const int nst = task.nchunks / CH; // even: the planner pads to pairs of stages
+        // two register sets: the loads of stage s + 2 are issued while stage s
+        // is multiplied, and land in LDS after stage s + 1
+        int32x4 xa[IA], xb[IB], ya[IA], yb[IB];
+        auto put = [&](int buf, const int32x4 (&va)[IA], const int32x4 (&vb)[IB]) {
 #pragma unroll
             for (int i = 0; i < IA; ++i)
                 *reinterpret_cast<int32x4 *>(&As[buf][(wave * IA + i) * 1024 + lane * 16]) = va[i];
@@ -223,19 +272,7 @@ __global__ __launch_bounds__(256, 2) void gemm_i8_kernel(OzGemmArgs a) {
             for (int i = 0; i < IB; ++i)
                 *reinterpret_cast<int32x4 *>(&Bs[buf][(wave * IB + i) * 1024 + lane * 16]) = vb[i];
         };
-
-        // loads are unconditional (the last stage is fetched twice) so the
-        // prefetched registers need no merge copies, which would wait on them
-        OzChunkRef e = ch[g];
-        fetch(e);
-        e = ch[min(1, nst - 1) * CH + g];
-        put(0);
-        __syncthreads();
-        int buf = 0;
-        for (int s = 0; s < nst; ++s) {
-            fetch(e);
-            e = ch[min(s + 2, nst - 1) * CH + g];
-            __builtin_amdgcn_sched_barrier(0); // keep the loads ahead of the MFMAs
+        auto compute = [&](int buf) {
             const int8_t *sa = As[buf], *sb = Bs[buf];
 #pragma unroll
             for (int ks = 0; ks < KS / 64; ++ks) {
@@ -254,12 +291,62 @@ __global__ __launch_bounds__(256, 2) void gemm_i8_kernel(OzGemmArgs a) {
                         acc[i][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(fa[i], fb[j], acc[i][j],
                                                                           0, 0, 0);
             }
-            // stage s + 1 into the other buffer: every wave finished reading
-            // it before the barrier that ended stage s - 1 (after the last
-            // stage this writes a copy of it that nothing reads)
-            put(buf ^ 1);
+        };
+
+        // loads are unconditional (stage indices are clamped, so the last
+        // stages are fetched again) so the staged registers need no merge
+        // copies, which would wait on them
+        // Chunk entries are staged in LDS, a window of kWin entries at a time,
+        // and each lane reads its own with ds_read: waiting for an entry then
+        // waits on the LDS counter only, never on the operand loads in flight.
+        struct Sel {
+            uint32_t a, b;
+            int stride;
+        };
+        auto load_window = [&](int st0) {
+            __syncthreads(); // nobody still reads the previous window
+            const int c0 = st0 * CH, n = min(kWin, task.nchunks - c0);
+            for (int i = tid; i < n; i += 256)
+                *reinterpret_cast<int32x4 *>(&Es[i]) =
+                    *reinterpret_cast<const int32x4 *>(&ch[c0 + i]);
             __syncthreads();
-            buf ^= 1;
+        };
+        auto entry = [&](int st) {
+            const int idx = (min(st, nst - 1) * CH + g) % kWin;
+            const OzChunkRef e = Es[idx];
+            return Sel{e.a, e.b, e.stride};
+        };
+        auto fetch2 = [&](const Sel &e, int32x4 (&va)[IA], int32x4 (&vb)[IB]) {
+#pragma unroll
+            for (int i = 0; i < IA; ++i)
+                va[i] = *reinterpret_cast<const __attribute__((address_space(1))) int32x4 *>(
+                    plane + e.a + ra[i] * e.stride);
+#pragma unroll
+            for (int i = 0; i < IB; ++i)
+                vb[i] = *reinterpret_cast<const __attribute__((address_space(1))) int32x4 *>(
+                    plane + e.b + rb[i] * e.stride);
+        };
+        load_window(0);
+        fetch2(entry(0), xa, xb);
+        fetch2(entry(1), ya, yb);
+        put(0, xa, xb);
+        __syncthreads();
+        for (int s = 0; s < nst; s += 2) {
+            // entries of stages s + 2, s + 3 are in the next window (a window
+            // holds an even number of stages, so they share one)
+            if (s + 2 < nst && ((s + 2) * CH) % kWin == 0) load_window(s + 2);
+            // stage s from buffer 0; stage s + 1 waits in (ya, yb)
+            fetch2(entry(s + 2), xa, xb);
+            __builtin_amdgcn_sched_barrier(0); // keep the loads ahead of the MFMAs
+            compute(0);
+            put(1, ya, yb);
+            __syncthreads();
+            // stage s + 1 from buffer 1; stage s + 2 waits in (xa, xb)
+            fetch2(entry(s + 3), ya, yb);
+            __builtin_amdgcn_sched_barrier(0);
+            compute(1);
+            put(0, xa, xb); // after the last pair of stages this writes a copy nothing reads
+            __syncthreads();
         }
 
         // t = (C mod p) u mod p of the tile into this modulus' result plane,
@@ -334,35 +421,51 @@ __global__ __launch_bounds__(256) void reconstruct_kernel(OzRecArgs a) {
         const int total = task.M * task.N;
         const int e1 = min(w.e0 + kOzRecChunk, total);
         const uint8_t *res = a.res_planes + task.res_off;
-        for (int e = w.e0 + (int)threadIdx.x; e < e1; e += blockDim.x) {
-            int m, n;
-            if (task.nfast) {
-                m = e / task.N;
-                n = e - m * task.N;
-            } else {
-                n = e / task.M;
-                m = e - n * task.M;
-            }
-            U128 acc{0ull, 0ull};
-            double s = 0.0;
+        // four consecutive elements per thread: one 32-bit load per plane
+        // (task regions start 16-byte aligned and planes end 16 bytes past
+        // the last region, so the word never leaves the plane)
+        for (int e0 = w.e0 + 4 * (int)threadIdx.x; e0 < e1; e0 += 4 * (int)blockDim.x) {
+            U128 acc[4] = {{0ull, 0ull}, {0ull, 0ull}, {0ull, 0ull}, {0ull, 0ull}};
+            double s[4] = {0.0, 0.0, 0.0, 0.0};
             for (int i = 0; i < a.c.n; ++i) {
-                const unsigned t = res[(long long)i * a.res_plane_bytes + e];
-                s += (double)t * a.c.rp[i];
-                mad_small(acc, (unsigned long long)t, a.c.mi[i]);
+                const uint32_t t4 =
+                    *reinterpret_cast<const uint32_t *>(res + (long long)i * a.res_plane_bytes + e0);
+                const double rp = a.c.rp[i];
+                const uint64_t mi[2] = {a.c.mi[i][0], a.c.mi[i][1]};
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const unsigned t = (t4 >> (8 * q)) & 0xffu;
+                    s[q] += (double)t * rp;
+                    mad_small(acc[q], (unsigned long long)t, mi);
+                }
             }
-            U128 qm{0ull, 0ull};
-            mad_small(qm, (unsigned long long)rint(s), a.c.m);
-            const U128 C{acc.lo - qm.lo, acc.hi - qm.hi - (acc.lo < qm.lo ? 1ull : 0ull)};
-            const int ea = a.expo[task.expA + m], eb = a.expo[task.expB + n];
-            double v = 0.0;
-            if (ea != INT_MIN && eb != INT_MIN) {
-                const int sa = (a.c.P - 1) - ea, sb = (a.c.P - 1) - eb;
-                v = ldexp(to_double(C), -(sa + sb));
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int e = e0 + q;
+                if (e >= e1) break;
+                int m, n;
+                if (task.nfast) {
+                    m = e / task.N;
+                    n = e - m * task.N;
+                } else {
+                    n = e / task.M;
+                    m = e - n * task.M;
+                }
+                U128 qm{0ull, 0ull};
+                mad_small(qm, (unsigned long long)rint(s[q]), a.c.m);
+                const U128 C{acc[q].lo - qm.lo,
+                             acc[q].hi - qm.hi - (acc[q].lo < qm.lo ? 1ull : 0ull)};
+                const int ea = a.expo[task.expA + m], eb = a.expo[task.expB + n];
+                double v = 0.0;
+                if (ea != INT_MIN && eb != INT_MIN) {
+                    const int sa = (a.c.P - 1) - ea, sb = (a.c.P - 1) - eb;
+                    v = ldexp(to_double(C), -(sa + sb));
+                }
+                const int off = a.tab[task.mD + m] + a.tab[task.nD + n];
+                double *D = task.D;
+                if (task.beta) v += D[off];
+                D[off] = v;
             }
-            const int off = a.tab[task.mD + m] + a.tab[task.nD + n];
-            double *D = task.D;
-            if (task.beta) v += D[off];
-            D[off] = v;
         }
     }
 }
@@ -390,12 +493,13 @@ hipError_t oz_row_exponents(const OzConvArgs &a, const OzConvTile *row_items, in
 
 hipError_t oz_to_residues(const OzConvArgs &a, hipStream_t s) {
     if (a.nitems <= 0) return hipSuccess;
-    hipLaunchKernelGGL(to_residues_kernel, dim3(grid_for(a.nitems, 65535)), dim3(256), 0, s, a);
+    hipLaunchKernelGGL(to_residues_kernel, dim3(grid_for(a.nitems, 4 * 256)), dim3(256), 0, s, a);
     return hipGetLastError();
 }
 
 hipError_t oz_gemm(const OzGemmArgs &a, hipStream_t s) {
     if (a.ntiles <= 0) return hipSuccess;
+    if (a.tile == kOzBigTile) return oz_gemm_big(a, s);
     hipLaunchKernelGGL((gemm_i8_kernel<kOzBM, kOzBN>), dim3(a.ntiles, a.c.n), dim3(256), 0, s, a);
     return hipGetLastError();
 }

@@ -593,6 +593,10 @@ int oz_constants(int n, long long kmax, OzConst &c) {
         for (int j = 0; j < n; ++j)
             if (j != i) mi = (mi * (c.p[j] % p)) % p;
         c.u[i] = inv_mod((int)mi, p);
+        c.c17[i] = (int)((1LL << 17) % p);
+        c.c34[i] = (int)((1LL << 34) % p);
+        c.off[i] = p << 19;
+        c.bar[i] = (uint32_t)((1ULL << 32) / (unsigned long long)p);
         c.rp[i] = 1.0 / p;
         const unsigned __int128 Mi = M / (unsigned __int128)p;
         c.mi[i][0] = (uint64_t)Mi;
@@ -785,7 +789,8 @@ int plan_split_locked(Context &, const sipgpu_contraction *list, int n, int nmod
                 chunks.push_back(OzChunkRef{(uint32_t)(A.panel_off + k),
                                             (uint32_t)(B.panel_off + k), A.Kp, 0});
         }
-        while ((chunks.size() - tasks[t].chunk0) % kOzStageChunks)
+        // whole pairs of stages: the GEMM runs two stages per loop trip
+        while ((chunks.size() - tasks[t].chunk0) % (2 * kOzStageChunks))
             chunks.push_back(OzChunkRef{0u, 0u, 0, 0});
         tasks[t].nchunks = (int)chunks.size() - tasks[t].chunk0;
     }
@@ -793,16 +798,29 @@ int plan_split_locked(Context &, const sipgpu_contraction *list, int n, int nmod
     long long res_elems = 0;
     std::vector<OzRecItem> rec;
     for (int t = 0; t < ntasks; ++t) {
-        tasks[t].res_off = res_elems;
+        tasks[t].res_off = res_elems; // 16-byte aligned: the reconstruction reads words
         const long long mn = (long long)tasks[t].M * tasks[t].N;
         for (long long e0 = 0; e0 < mn; e0 += kOzRecChunk) rec.push_back(OzRecItem{t, (int)e0});
-        res_elems += mn;
+        res_elems += (mn + 15) / 16 * 16;
+    }
+    // GEMM tile: 256 x 256 when the tasks' virtual K averages at least 8 KB
+    // (half the LDS traffic per MFMA; its per-tile prologue and eight-round
+    // epilogue need a long K to amortize), else 128 x 128 (two workgroups
+    // per CU).  Measured: ring tasks (K = 4.7 KB) run 1.25x faster on 128,
+    // block-sparse ladder tasks (K = 20 KB) 1.18x faster on 256.
+    // SIPGPU_OZ_TILE=128|256 pins it.
+    long long sumK = 0;
+    for (const OzTask &t : tasks) sumK += (long long)t.nchunks * kOzChunk;
+    int tile = (ntasks && sumK / ntasks >= 8192) ? kOzBigTile : kOzBM;
+    if (const char *e = std::getenv("SIPGPU_OZ_TILE")) {
+        const int v = std::atoi(e);
+        if (v == kOzBM || v == kOzBigTile) tile = v;
     }
     // tiles: the same XCD schedule as the FP64 batch
     std::vector<int> tms(ntasks), tns(ntasks);
     for (int t = 0; t < ntasks; ++t) {
-        tms[t] = (tasks[t].M + kOzBM - 1) / kOzBM;
-        tns[t] = (tasks[t].N + kOzBN - 1) / kOzBN;
+        tms[t] = (tasks[t].M + tile - 1) / tile;
+        tns[t] = (tasks[t].N + tile - 1) / tile;
     }
     int real_tiles = 0;
     const std::vector<OzTile> tiles = xcd_schedule<OzTile>(tms, tns, task_cost, real_tiles);
@@ -843,7 +861,7 @@ int plan_split_locked(Context &, const sipgpu_contraction *list, int n, int nmod
     SplitData &sd = b.sd;
     sd.c = oc;
     sd.plane_bytes = (panel + 255) / 256 * 256;
-    sd.res_plane_bytes = std::max<long long>((res_elems + 255) / 256 * 256, 256);
+    sd.res_plane_bytes = (res_elems + 16 + 255) / 256 * 256; // a word past the last region
     HIP_TRY(hipMalloc(&sd.planes, (size_t)sd.plane_bytes * nmod));
     // the zero bytes at the start of each plane are never written again
     HIP_TRY(hipMemset(sd.planes, 0, (size_t)sd.plane_bytes * nmod));
@@ -873,6 +891,7 @@ int plan_split_locked(Context &, const sipgpu_contraction *list, int n, int nmod
     g.res_planes = sd.res_planes;
     g.res_plane_bytes = sd.res_plane_bytes;
     g.ntiles = (int)tiles.size();
+    g.tile = tile;
     g.c = oc;
     OzRecArgs &r = sd.rargs;
     r.tasks = dtasks;

@@ -163,3 +163,44 @@ def test_split_rejects_bad_moduli():
         with pytest.raises(sg.SipGpuError) as ei:
             wl.batch_for(w, moduli=m)
         assert ei.value.code == -107
+
+
+def test_split_long_k_crosses_entry_windows():
+    """One result block summing three pairs of K = 3000 (564 chunks of 16
+    bytes of virtual K, more than one staged window of chunk entries)."""
+    rng = np.random.default_rng(11)
+    M, N, K = 20, 30, 3000
+    ptrn, e = sg.get_contraction_ptrn(2, 2, 2, [1, 2, 3, 1, 3, 2])  # D(m,n) = L(k,m) R(k,n)
+    assert e == 0
+    Ls = [rng.standard_normal(K * M) for _ in range(3)]
+    Rs = [rng.standard_normal(K * N) for _ in range(3)]
+    dL = [sg.DeviceArray.from_host(x) for x in Ls]
+    dR = [sg.DeviceArray.from_host(x) for x in Rs]
+    D = sg.DeviceArray(M * N)
+    D.zero()
+    cs = [sg.make_contraction(ptrn, dL[i].ptr, [K, M], dR[i].ptr, [K, N], D.ptr, [M, N],
+                              accumulate=True) for i in range(3)]
+    want = sum(Ls[i].reshape(K, M, order="F").T @ Rs[i].reshape(K, N, order="F")
+               for i in range(3)).ravel(order="F")
+    for moduli in (14, None):
+        D.zero()
+        b = sg.Batch(cs, moduli=moduli)
+        b.execute()
+        sg.synchronize()
+        assert rel_fro(D.download(), want) <= (1e-12 if moduli else 1e-13)
+        b.destroy()
+
+
+@pytest.mark.parametrize("tile", ["128", "256"])
+@pytest.mark.parametrize("make", [
+    lambda: wl.ladder(),
+    lambda: wl.ring(o_segs=(15,), v_segs=(30, 15)),
+], ids=["ladder", "ring-uneven"])
+def test_split_both_gemm_tiles(make, tile, monkeypatch):
+    """The 128 x 128 and the 256 x 256 INT8 GEMMs (chosen per batch by task
+    size) give the same parity; SIPGPU_OZ_TILE pins one."""
+    monkeypatch.setenv("SIPGPU_OZ_TILE", tile)
+    w = make()
+    w["D"].host[:] = 0.0
+    _, got = run_split(w, 14)
+    assert per_block_err(w, got, oracle_batch(w)) <= TOL
