@@ -11,7 +11,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libsipgpu.so")
-SOURCES = ["plan.cpp", "permute.hip", "contract.hip", "calib.hip", "ozaki.hip", "ozaki_big.hip", "synth.hip", "energy.hip", "runtime.cpp", "serve.cpp"]
+SOURCES = ["plan.cpp", "permute.hip", "contract.hip", "calib.hip", "ozaki.hip", "ozaki_big.hip", "packed.hip", "synth.hip", "energy.hip", "runtime.cpp", "serve.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 FLAGS = [
@@ -27,7 +27,8 @@ FLAGS = [
 # unified VGPR file ("VGPR form"): the compiler then neither copies them
 # between register banks around loops nor splits the budget between banks.
 VGPR_FORM = ["-mllvm", "-amdgpu-mfma-vgpr-form=1"]
-PER_FILE = {"calib.hip": VGPR_FORM, "contract.hip": VGPR_FORM, "ozaki.hip": VGPR_FORM}
+PER_FILE = {"calib.hip": VGPR_FORM, "contract.hip": VGPR_FORM, "ozaki.hip": VGPR_FORM,
+            "packed.hip": VGPR_FORM}
 NOISE = "is not a recognized feature for this target"
 
 

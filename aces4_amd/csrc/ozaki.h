@@ -88,6 +88,7 @@ struct OzGemmArgs {
     long long res_plane_bytes;
     int ntiles;
     int tile; // kOzBM (= kOzBN) or kOzBigTile
+    const int32_t *tab; // destination tables (packed FP64 path)
     OzConst c;
 };
 
@@ -121,6 +122,9 @@ hipError_t oz_row_exponents(const OzConvArgs &a, const OzConvTile *row_items, in
 hipError_t oz_to_residues(const OzConvArgs &a, hipStream_t s);
 hipError_t oz_gemm(const OzGemmArgs &a, hipStream_t s);
 hipError_t oz_gemm_big(const OzGemmArgs &a, hipStream_t s);
+// Packed FP64 path: operand panels of doubles, then the GEMM over them.
+hipError_t oz_pack(const OzConvArgs &a, hipStream_t s);
+hipError_t oz_gemm_f64(const OzGemmArgs &a, hipStream_t s);
 hipError_t oz_reconstruct(const OzRecArgs &a, hipStream_t s);
 
 } // namespace sipgpu

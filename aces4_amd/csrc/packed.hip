@@ -1,0 +1,234 @@
+// packed.hip -- the FP64 contraction on packed operand panels.
+//
+// For large batches the operand permutations are applied once per block
+// instead of once per tile: every operand instance (block + its k / x
+// tables) is packed into X row-major rows of Kp doubles (K rounded up to
+// an even count, zero filled), and the grouped GEMM runs over each task's
+// virtual K -- its pairs' panels concatenated in 16-byte chunks through the
+// same chunk map as the split-precision path -- with no table lookups in
+// the K loop.  The product is scattered into the destination block through
+// its tables, accumulating when the task does.
+//
+// MFMA v_mfma_f64_16x16x4_f64: lane l holds A(i = l%16, k = l/16); register
+// r of lane l holds C(4r + l/16, l%16).  A 16-byte LDS read gives a lane two
+// consecutive k of its row; the pair feeds two MFMAs (the first double to the
+// first, the second to the second), so MFMA t of a 64-byte step sums
+// k = 2 (4 s + l/16) + t over l/16 = 0..3: the same permutation of k for both
+// operands, so the sum over all MFMAs is the full dot product.
+#include "kernels.h"
+#include "ozaki.h"
+
+namespace sipgpu {
+
+namespace {
+
+typedef int int32x4 __attribute__((ext_vector_type(4)));
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+typedef double f64x2 __attribute__((ext_vector_type(2)));
+typedef const __attribute__((address_space(1))) int8_t gi8_t;
+
+// Persistent workgroups over 32 x 128 tiles of operand blocks: read along
+// the index adjacent in memory, transposed through LDS, written as 128-byte
+// row pieces of the block's panel.
+__global__ __launch_bounds__(256) void pack_kernel(OzConvArgs a) {
+    __shared__ double S[kOzTileX][kOzTileK + 1];
+    const int tid = threadIdx.x;
+    for (int it = blockIdx.x; it < a.nitems; it += gridDim.x) {
+        const OzConvTile w = a.items[it];
+        const OzBlock b = a.blocks[w.block];
+        double v[16];
+        if (b.xfast) {
+            const int x = w.x0 + (tid & 31);
+            const bool xin = x < b.X;
+            const int xo = xin ? a.tab[b.xtab + x] : 0;
+#pragma unroll
+            for (int j = 0; j < 16; ++j) {
+                const int k = w.k0 + (tid >> 5) + 8 * j;
+                v[j] = (xin && k < b.K) ? b.data[a.tab[b.ktab + k] + xo] : 0.0;
+            }
+        } else {
+            const int k = w.k0 + (tid & 127);
+            const bool kin = k < b.K;
+            const int ko = kin ? a.tab[b.ktab + k] : 0;
+#pragma unroll
+            for (int j = 0; j < 16; ++j) {
+                const int x = w.x0 + (tid >> 7) + 2 * j;
+                v[j] = (kin && x < b.X) ? b.data[ko + a.tab[b.xtab + x]] : 0.0;
+            }
+        }
+        __syncthreads(); // the previous tile's LDS reads are done
+        if (b.xfast) {
+#pragma unroll
+            for (int j = 0; j < 16; ++j) S[tid & 31][(tid >> 5) + 8 * j] = v[j];
+        } else {
+#pragma unroll
+            for (int j = 0; j < 16; ++j) S[(tid >> 7) + 2 * j][tid & 127] = v[j];
+        }
+        __syncthreads();
+        const int xx = tid >> 3, kc = (tid & 7) * 16; // 8 lanes: one 1 KB row piece
+        const int x = w.x0 + xx, k = w.k0 + kc;
+        if (x >= b.X || k >= b.Kp) continue;
+        double *dst = reinterpret_cast<double *>(a.planes + b.panel_off) + (long long)x * b.Kp + k;
+        const int n = min(16, b.Kp - k); // Kp is even: whole 16-byte pairs
+        for (int j = 0; j < n; j += 2)
+            *reinterpret_cast<f64x2 *>(dst + j) = f64x2{S[xx][kc + j], S[xx][kc + j + 1]};
+    }
+}
+
+// Grouped FP64 GEMM over packed panels.  Workgroup tile 128 x 128 (4 waves,
+// 2 x 2, each 64 x 64: 16 accumulator fragments); stages of
+// 128 bytes (16 k) double-buffered in LDS with rows XOR-swizzled by 16-byte
+// chunk; loads run one stage ahead through registers; chunk entries come
+// from an LDS window.  Rows past the task's M / N read row M-1 / N-1 and
+// are never stored; chunks past the pairs' K read the plane's zero bytes.
+__global__ __launch_bounds__(256, 2) void gemm_f64_packed_kernel(OzGemmArgs a) {
+    constexpr int BM = 128, BN = 128, WM = 2, WN = 2;
+    constexpr int TM = BM / WM, TN = BN / WN, FM = TM / 16, FN = TN / 16;
+    constexpr int KS = kOzStage, CH = kOzStageChunks;
+    constexpr int IA = BM / 32, IB = BN / 32;
+    constexpr int kWin = 512;
+    static_assert(CH == 8 && KS == 128, "a stage is 8 chunks of 16 bytes");
+    __shared__ __attribute__((aligned(1024))) int8_t As[2][BM * KS];
+    __shared__ __attribute__((aligned(1024))) int8_t Bs[2][BN * KS];
+    __shared__ __attribute__((aligned(16))) OzChunkRef Es[kWin];
+
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int wm = wave % WM, wn = wave / WM;
+    const int li = lane & 15, lk = lane >> 4;
+    const int g = (lane & 7) ^ (lane >> 3);
+    gi8_t *plane = (gi8_t *)a.planes;
+
+    for (int t = blockIdx.x; t < a.ntiles; t += gridDim.x) {
+        const OzTile tile = a.tiles[t];
+        if (tile.task < 0) continue;
+        const OzTask task = a.tasks[tile.task];
+        const int m0 = tile.mt * BM, n0 = tile.nt * BN;
+        int ra[IA], rb[IB];
+#pragma unroll
+        for (int i = 0; i < IA; ++i) ra[i] = min(m0 + 8 * (wave * IA + i) + (lane >> 3), task.M - 1);
+#pragma unroll
+        for (int i = 0; i < IB; ++i) rb[i] = min(n0 + 8 * (wave * IB + i) + (lane >> 3), task.N - 1);
+
+        f64x4 acc[FM][FN];
+#pragma unroll
+        for (int i = 0; i < FM; ++i)
+#pragma unroll
+            for (int j = 0; j < FN; ++j) acc[i][j] = f64x4{0.0, 0.0, 0.0, 0.0};
+
+        const OzChunkRef *ch = a.chunks + task.chunk0;
+        const int nst = task.nchunks / CH; // even
+        auto load_window = [&](int st0) {
+            __syncthreads();
+            const int c0 = st0 * CH, n = min(kWin, task.nchunks - c0);
+            for (int i = tid; i < n; i += 256)
+                *reinterpret_cast<int32x4 *>(&Es[i]) = *reinterpret_cast<const int32x4 *>(&ch[c0 + i]);
+            __syncthreads();
+        };
+        struct Sel {
+            uint32_t a, b;
+            int stride;
+        };
+        auto entry = [&](int st) {
+            const OzChunkRef e = Es[(min(st, nst - 1) * CH + g) % kWin];
+            return Sel{e.a, e.b, e.stride};
+        };
+        int32x4 xa[IA], xb[IB];
+        auto fetch = [&](const Sel &e) {
+#pragma unroll
+            for (int i = 0; i < IA; ++i)
+                xa[i] = *reinterpret_cast<const __attribute__((address_space(1))) int32x4 *>(
+                    plane + e.a + (long long)ra[i] * e.stride);
+#pragma unroll
+            for (int i = 0; i < IB; ++i)
+                xb[i] = *reinterpret_cast<const __attribute__((address_space(1))) int32x4 *>(
+                    plane + e.b + (long long)rb[i] * e.stride);
+        };
+        auto put = [&](int buf) {
+#pragma unroll
+            for (int i = 0; i < IA; ++i)
+                *reinterpret_cast<int32x4 *>(&As[buf][(wave * IA + i) * 1024 + lane * 16]) = xa[i];
+#pragma unroll
+            for (int i = 0; i < IB; ++i)
+                *reinterpret_cast<int32x4 *>(&Bs[buf][(wave * IB + i) * 1024 + lane * 16]) = xb[i];
+        };
+        auto compute = [&](int buf) {
+            const int8_t *sa = As[buf], *sb = Bs[buf];
+#pragma unroll
+            for (int ks = 0; ks < KS / 64; ++ks) {
+                const int slot = ((ks * 4 + lk) ^ (li & 7)) * 16;
+                f64x2 fa[FM], fb[FN];
+#pragma unroll
+                for (int i = 0; i < FM; ++i)
+                    fa[i] = *reinterpret_cast<const f64x2 *>(sa + (wm * TM + i * 16 + li) * KS + slot);
+#pragma unroll
+                for (int j = 0; j < FN; ++j)
+                    fb[j] = *reinterpret_cast<const f64x2 *>(sb + (wn * TN + j * 16 + li) * KS + slot);
+#pragma unroll
+                for (int h = 0; h < 2; ++h)
+#pragma unroll
+                    for (int i = 0; i < FM; ++i)
+#pragma unroll
+                        for (int j = 0; j < FN; ++j)
+                            acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa[i][h], fb[j][h],
+                                                                             acc[i][j], 0, 0, 0);
+            }
+        };
+
+        // One stage of loads ahead is enough here: a stage is 64 MFMAs of
+        // 64 cycles per wave.  Loads are unconditional (the last stage is
+        // fetched twice) so the staged registers need no merge copies.
+        load_window(0);
+        fetch(entry(0));
+        put(0);
+        __syncthreads();
+        int buf = 0;
+        for (int s = 0; s < nst; ++s) {
+            if (s + 1 < nst && ((s + 1) * CH) % kWin == 0) load_window(s + 1);
+            fetch(entry(s + 1));
+            __builtin_amdgcn_sched_barrier(0);
+            compute(buf);
+            put(buf ^ 1); // after the last stage this writes a copy nothing reads
+            __syncthreads();
+            buf ^= 1;
+        }
+
+        // scatter into the destination block
+        __attribute__((address_space(1))) double *D =
+            (__attribute__((address_space(1))) double *)task.D;
+#pragma unroll
+        for (int j = 0; j < FN; ++j) {
+            const int n = n0 + wn * TN + j * 16 + li;
+            if (n >= task.N) continue;
+            const int no = a.tab[task.nD + n];
+#pragma unroll
+            for (int i = 0; i < FM; ++i)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int m = m0 + wm * TM + i * 16 + 4 * r + lk;
+                    if (m < task.M) {
+                        const int off = a.tab[task.mD + m] + no;
+                        double v = acc[i][j][r];
+                        if (task.beta) v += D[off];
+                        D[off] = v;
+                    }
+                }
+        }
+    }
+}
+
+} // namespace
+
+hipError_t oz_pack(const OzConvArgs &a, hipStream_t s) {
+    if (a.nitems <= 0) return hipSuccess;
+    hipLaunchKernelGGL(pack_kernel, dim3(a.nitems < 65535 ? a.nitems : 65535), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t oz_gemm_f64(const OzGemmArgs &a, hipStream_t s) {
+    if (a.ntiles <= 0) return hipSuccess;
+    hipLaunchKernelGGL(gemm_f64_packed_kernel, dim3(a.ntiles), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+} // namespace sipgpu

@@ -283,6 +283,7 @@ struct sipgpu_batch_s {
     void *dev = nullptr; // tiles | tasks | pairs | tab
     GemmArgs args{};
     bool split = false;
+    bool packed = false; // FP64 on packed operand panels
     SplitData sd;
 };
 
@@ -622,10 +623,13 @@ struct UnionFind {
     void unite(int a, int b) { parent[find(a)] = find(b); }
 };
 
-int plan_split_locked(Context &, const sipgpu_contraction *list, int n, int nmod,
-                      sipgpu_batch_s &b) {
-    if (nmod < kOzMinModuli || nmod > kOzMaxUsable)
+// Panel planner: nmod > 0 plans the split-precision path (1-byte residue
+// panels, nmod moduli); nmod == 0 the packed FP64 path (panels of doubles,
+// products scattered straight into the destinations).
+int plan_panels_locked(const sipgpu_contraction *list, int n, int nmod, sipgpu_batch_s &b) {
+    if (nmod != 0 && (nmod < kOzMinModuli || nmod > kOzMaxUsable))
         return fail(SIPGPU_EARG, "split path: nmoduli must be in 6..14");
+    const int eb = nmod ? 1 : 8; // bytes per panel element
     // shapes
     std::unordered_map<std::string, int> shape_index;
     std::vector<std::unique_ptr<ContractionShape>> shapes;
@@ -673,7 +677,8 @@ int plan_split_locked(Context &, const sipgpu_contraction *list, int n, int nmod
         ob.data = ptr;
         ob.K = (int)sh.K;
         ob.X = X;
-        ob.Kp = (int)((sh.K + kOzChunk - 1) / kOzChunk * kOzChunk);
+        const int kq = kOzChunk / eb; // elements per 16-byte chunk
+        ob.Kp = (int)((sh.K + kq - 1) / kq * kq);
         ob.ktab = ktab;
         ob.xtab = xtab;
         ob.xfast = unit_stride(xtab, X) ? 1 : 0;
@@ -727,7 +732,7 @@ int plan_split_locked(Context &, const sipgpu_contraction *list, int n, int nmod
             cost += 2.0 * (double)sh.M * (double)sh.N * (double)sh.K;
         }
         // exact int32 accumulation: sum of |a b| <= Kvirtual * 127^2 < 2^31
-        if (kptot + kOzStage > 131072)
+        if (nmod && kptot + kOzStage > 131072)
             return fail(SIPGPU_ESIZE, "split path: a result block sums more than 131072 k");
         if ((long long)t.M * t.N >= INT_MAX)
             return fail(SIPGPU_ESIZE, "split path: result block too large");
@@ -769,7 +774,7 @@ int plan_split_locked(Context &, const sipgpu_contraction *list, int n, int nmod
     std::vector<OzConvTile> conv, rows;
     for (int i = 0; i < nb; ++i) {
         blocks[i].panel_off = panel;
-        panel += ((long long)blocks[i].X * blocks[i].Kp + 255) / 256 * 256;
+        panel += ((long long)blocks[i].X * blocks[i].Kp * eb + 255) / 256 * 256;
         for (int x0 = 0; x0 < blocks[i].X; x0 += kOzTileX) {
             rows.push_back(OzConvTile{i, x0, 0, 0});
             for (int k0 = 0; k0 < blocks[i].Kp; k0 += kOzTileK)
@@ -785,9 +790,10 @@ int plan_split_locked(Context &, const sipgpu_contraction *list, int n, int nmod
         tasks[t].chunk0 = (int)chunks.size();
         for (const auto &ab : task_pairs[t].ab) {
             const OzBlock &A = blocks[ab.first], &B = blocks[ab.second];
-            for (int k = 0; k < A.Kp; k += kOzChunk)
-                chunks.push_back(OzChunkRef{(uint32_t)(A.panel_off + k),
-                                            (uint32_t)(B.panel_off + k), A.Kp, 0});
+            for (int k = 0; k < A.Kp; k += kOzChunk / eb)
+                chunks.push_back(OzChunkRef{(uint32_t)(A.panel_off + (long long)k * eb),
+                                            (uint32_t)(B.panel_off + (long long)k * eb),
+                                            A.Kp * eb, 0});
         }
         // whole pairs of stages: the GEMM runs two stages per loop trip
         while ((chunks.size() - tasks[t].chunk0) % (2 * kOzStageChunks))
@@ -797,7 +803,7 @@ int plan_split_locked(Context &, const sipgpu_contraction *list, int n, int nmod
     // result planes, reconstruction work
     long long res_elems = 0;
     std::vector<OzRecItem> rec;
-    for (int t = 0; t < ntasks; ++t) {
+    for (int t = 0; t < ntasks && nmod; ++t) {
         tasks[t].res_off = res_elems; // 16-byte aligned: the reconstruction reads words
         const long long mn = (long long)tasks[t].M * tasks[t].N;
         for (long long e0 = 0; e0 < mn; e0 += kOzRecChunk) rec.push_back(OzRecItem{t, (int)e0});
@@ -811,10 +817,10 @@ int plan_split_locked(Context &, const sipgpu_contraction *list, int n, int nmod
     // SIPGPU_OZ_TILE=128|256 pins it.
     long long sumK = 0;
     for (const OzTask &t : tasks) sumK += (long long)t.nchunks * kOzChunk;
-    int tile = (ntasks && sumK / ntasks >= 8192) ? kOzBigTile : kOzBM;
+    int tile = (nmod && ntasks && sumK / ntasks >= 8192) ? kOzBigTile : kOzBM;
     if (const char *e = std::getenv("SIPGPU_OZ_TILE")) {
         const int v = std::atoi(e);
-        if (v == kOzBM || v == kOzBigTile) tile = v;
+        if (nmod && (v == kOzBM || v == kOzBigTile)) tile = v;
     }
     // tiles: the same XCD schedule as the FP64 batch
     std::vector<int> tms(ntasks), tns(ntasks);
@@ -826,9 +832,11 @@ int plan_split_locked(Context &, const sipgpu_contraction *list, int n, int nmod
     const std::vector<OzTile> tiles = xcd_schedule<OzTile>(tms, tns, task_cost, real_tiles);
     if (tiles.size() >= (size_t)INT_MAX) return fail(SIPGPU_ESIZE, "split path: too many tiles");
 
-    OzConst oc;
-    const int P = oz_constants(nmod, kmax, oc);
-    if (P < 8) return fail(SIPGPU_EARG, "split path: too few moduli for this K");
+    OzConst oc{};
+    if (nmod) {
+        const int P = oz_constants(nmod, kmax, oc);
+        if (P < 8) return fail(SIPGPU_EARG, "split path: too few moduli for this K");
+    }
 
     // one descriptor allocation
     auto align = [](size_t v) { return (v + 255) & ~(size_t)255; };
@@ -862,11 +870,12 @@ int plan_split_locked(Context &, const sipgpu_contraction *list, int n, int nmod
     sd.c = oc;
     sd.plane_bytes = (panel + 255) / 256 * 256;
     sd.res_plane_bytes = (res_elems + 16 + 255) / 256 * 256; // a word past the last region
-    HIP_TRY(hipMalloc(&sd.planes, (size_t)sd.plane_bytes * nmod));
+    const int nplanes = nmod ? nmod : 1;
+    HIP_TRY(hipMalloc(&sd.planes, (size_t)sd.plane_bytes * nplanes));
     // the zero bytes at the start of each plane are never written again
-    HIP_TRY(hipMemset(sd.planes, 0, (size_t)sd.plane_bytes * nmod));
+    HIP_TRY(hipMemset(sd.planes, 0, (size_t)sd.plane_bytes * nplanes));
     HIP_TRY(hipDeviceSynchronize());
-    HIP_TRY(hipMalloc(&sd.res_planes, (size_t)sd.res_plane_bytes * nmod));
+    if (nmod) HIP_TRY(hipMalloc(&sd.res_planes, (size_t)sd.res_plane_bytes * nmod));
     sd.expo = reinterpret_cast<int *>(base + o_expo);
     sd.nexpo = nexpo;
     sd.row_items = reinterpret_cast<const OzConvTile *>(base + o_rows);
@@ -892,6 +901,7 @@ int plan_split_locked(Context &, const sipgpu_contraction *list, int n, int nmod
     g.res_plane_bytes = sd.res_plane_bytes;
     g.ntiles = (int)tiles.size();
     g.tile = tile;
+    g.tab = dtab;
     g.c = oc;
     OzRecArgs &r = sd.rargs;
     r.tasks = dtasks;
@@ -902,7 +912,8 @@ int plan_split_locked(Context &, const sipgpu_contraction *list, int n, int nmod
     r.res_planes = sd.res_planes;
     r.res_plane_bytes = sd.res_plane_bytes;
     r.c = oc;
-    b.split = true;
+    b.split = nmod > 0;
+    b.packed = nmod == 0;
     b.ntiles = (int)tiles.size();
     b.ntasks = ntasks;
     b.npairs = npairs;
@@ -910,13 +921,30 @@ int plan_split_locked(Context &, const sipgpu_contraction *list, int n, int nmod
     return 0;
 }
 
-int execute_split(sipgpu_batch_s &b, hipStream_t s) {
+int execute_packed(sipgpu_batch_s &b, int stages, hipStream_t s) {
+    if (stages & SIPGPU_STAGE_PREPARE) HIP_TRY(oz_pack(b.sd.cargs, s));
+    if (stages & SIPGPU_STAGE_PRODUCT) HIP_TRY(oz_gemm_f64(b.sd.gargs, s));
+    return 0;
+}
+
+int execute_split(sipgpu_batch_s &b, int stages, hipStream_t s) {
     SplitData &sd = b.sd;
-    HIP_TRY(oz_fill_int(sd.expo, sd.nexpo, INT_MIN, s));
-    HIP_TRY(oz_row_exponents(sd.cargs, sd.row_items, sd.nrow_items, s));
-    HIP_TRY(oz_to_residues(sd.cargs, s));
-    HIP_TRY(oz_gemm(sd.gargs, s));
-    HIP_TRY(oz_reconstruct(sd.rargs, s));
+    if (stages & SIPGPU_STAGE_PREPARE) {
+        HIP_TRY(oz_fill_int(sd.expo, sd.nexpo, INT_MIN, s));
+        HIP_TRY(oz_row_exponents(sd.cargs, sd.row_items, sd.nrow_items, s));
+        HIP_TRY(oz_to_residues(sd.cargs, s));
+    }
+    if (stages & SIPGPU_STAGE_PRODUCT) HIP_TRY(oz_gemm(sd.gargs, s));
+    if (stages & SIPGPU_STAGE_FINISH) HIP_TRY(oz_reconstruct(sd.rargs, s));
+    return 0;
+}
+
+int execute_batch(Context &c, sipgpu_batch_s &b, int stages, sipgpu_stream_t s) {
+    if (b.ntiles == 0) return 0;
+    const hipStream_t hs = resolve(c, s).s;
+    if (b.split) return execute_split(b, stages, hs);
+    if (b.packed) return execute_packed(b, stages, hs);
+    if (stages & SIPGPU_STAGE_PRODUCT) HIP_TRY(launch_gemm(b.cfg, b.args, b.ntiles, hs));
     return 0;
 }
 
@@ -1211,17 +1239,40 @@ int sipgpu_batch_plan(const sipgpu_contraction *list, int n, sipgpu_batch_t *bat
         if (b->dev) (void)hipFree(b->dev);
         return rc;
     }
+    // Large batches run on packed operand panels: the permutations are
+    // applied once per block (one streaming pass) instead of per tile, and
+    // the GEMM runs over whole 16-k stages of each task's virtual K.  Small
+    // batches (and single contractions) keep the table-driven kernel: no
+    // extra pass, no panel memory.  SIPGPU_FP64_PATH=tables|packed pins it.
+    bool packed = b->ntasks > 0 && b->flops >= 1e9 && b->flops / b->ntasks >= 5e7;
+    if (const char *e = std::getenv("SIPGPU_FP64_PATH")) {
+        if (!std::strcmp(e, "tables")) packed = false;
+        if (!std::strcmp(e, "packed")) packed = b->ntasks > 0;
+    }
+    if (packed) {
+        auto pb = std::make_unique<sipgpu_batch_s>();
+        rc = plan_panels_locked(list, n, 0, *pb);
+        if (rc) {
+            if (pb->dev) (void)hipFree(pb->dev);
+            if (pb->sd.planes) (void)hipFree(pb->sd.planes);
+        } else {
+            if (b->dev) (void)hipFree(b->dev);
+            b = std::move(pb);
+        }
+    }
     *batch = b.release();
     return 0;
 }
 
 int sipgpu_batch_execute(sipgpu_batch_t b, sipgpu_stream_t s) {
     if (!b) return fail(SIPGPU_EARG, "null batch");
-    Context &c = ctx();
-    if (b->ntiles == 0) return 0;
-    if (b->split) return execute_split(*b, resolve(c, s).s);
-    HIP_TRY(launch_gemm(b->cfg, b->args, b->ntiles, resolve(c, s).s));
-    return 0;
+    return execute_batch(ctx(), *b, SIPGPU_STAGE_ALL, s);
+}
+
+int sipgpu_batch_execute_stages(sipgpu_batch_t b, int stages, sipgpu_stream_t s) {
+    if (!b) return fail(SIPGPU_EARG, "null batch");
+    if (stages & ~SIPGPU_STAGE_ALL) return fail(SIPGPU_EARG, "unknown stage bits");
+    return execute_batch(ctx(), *b, stages, s);
 }
 
 int sipgpu_batch_destroy(sipgpu_batch_t b) {
@@ -1239,12 +1290,14 @@ int sipgpu_batch_destroy(sipgpu_batch_t b) {
 int sipgpu_batch_plan_split(const sipgpu_contraction *list, int n, int nmoduli,
                             sipgpu_batch_t *batch) {
     if (!batch || (n > 0 && !list) || n < 0) return fail(SIPGPU_EARG, "bad batch arguments");
+    if (nmoduli < kOzMinModuli || nmoduli > kOzMaxUsable)
+        return fail(SIPGPU_EARG, "split path: nmoduli must be in 6..14");
     Context &c = ctx();
     std::lock_guard<std::mutex> g(c.mu);
     int rc = ensure_init(c);
     if (rc) return rc;
     auto b = std::make_unique<sipgpu_batch_s>();
-    rc = plan_split_locked(c, list, n, nmoduli, *b);
+    rc = plan_panels_locked(list, n, nmoduli, *b);
     if (rc) {
         if (b->dev) (void)hipFree(b->dev);
         if (b->sd.planes) (void)hipFree(b->sd.planes);
@@ -1259,6 +1312,12 @@ int sipgpu_batch_split_info(sipgpu_batch_t b, int *nmoduli, int *bits) {
     if (!b) return fail(SIPGPU_EARG, "null batch");
     if (nmoduli) *nmoduli = b->split ? b->sd.c.n : 0;
     if (bits) *bits = b->split ? b->sd.c.P : 53;
+    return 0;
+}
+
+int sipgpu_batch_kind(sipgpu_batch_t b, int *kind) {
+    if (!b || !kind) return fail(SIPGPU_EARG, "null batch");
+    *kind = b->split ? SIPGPU_BATCH_SPLIT : b->packed ? SIPGPU_BATCH_PACKED : SIPGPU_BATCH_TABLES;
     return 0;
 }
 

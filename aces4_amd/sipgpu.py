@@ -41,7 +41,8 @@ EXPORTED = [
     "sipgpu_contract", "sipgpu_batch_plan", "sipgpu_batch_execute", "sipgpu_batch_destroy",
     "sipgpu_batch_info", "sipgpu_plan_tables", "sipgpu_plan_permute", "sipgpu_measure_peaks",
     "sipgpu_measured_clock_mhz", "sipgpu_batch_plan_split", "sipgpu_batch_split_info",
-    "sipgpu_fill_random", "sipgpu_energy_denominator", "sipgpu_energy_sum",
+    "sipgpu_fill_random", "sipgpu_energy_denominator", "sipgpu_energy_sum", "sipgpu_batch_kind",
+    "sipgpu_batch_execute_stages",
     "sipgpu_cache_create", "sipgpu_cache_destroy", "sipgpu_cache_insert", "sipgpu_cache_acquire",
     "sipgpu_cache_release", "sipgpu_cache_read", "sipgpu_cache_flush", "sipgpu_cache_evict",
     "sipgpu_cache_get_stats", "sipgpu_server_create", "sipgpu_server_submit",
@@ -130,6 +131,8 @@ def lib():
                                               ctypes.c_int, ctypes.POINTER(_vp)]
         L.sipgpu_batch_split_info.argtypes = [_vp, ctypes.POINTER(ctypes.c_int),
                                               ctypes.POINTER(ctypes.c_int)]
+        L.sipgpu_batch_kind.argtypes = [_vp, ctypes.POINTER(ctypes.c_int)]
+        L.sipgpu_batch_execute_stages.argtypes = [_vp, ctypes.c_int, _vp]
         L.sipgpu_batch_destroy.argtypes = [_vp]
         L.sipgpu_batch_info.argtypes = [_vp, ctypes.POINTER(ctypes.c_double),
                                         ctypes.POINTER(ctypes.c_int),
@@ -460,6 +463,9 @@ def make_contraction(ptrn, l, ldims, r, rdims, d, ddims, accumulate=False):
     return c
 
 
+STAGE_PREPARE, STAGE_PRODUCT, STAGE_FINISH, STAGE_ALL = 1, 2, 4, 7
+
+
 class Batch:
     """A planned batch of contractions: one grouped FP64-MFMA launch, or with
     `moduli` set, the split-precision (INT8 / CRT) path over that many moduli."""
@@ -476,6 +482,9 @@ class Batch:
         nm, bits = ctypes.c_int(0), ctypes.c_int(0)
         _check(lib().sipgpu_batch_split_info(self.handle, ctypes.byref(nm), ctypes.byref(bits)))
         self.moduli, self.bits = nm.value, bits.value
+        k = ctypes.c_int(0)
+        _check(lib().sipgpu_batch_kind(self.handle, ctypes.byref(k)))
+        self.kind = {0: "tables", 1: "packed", 2: "split"}[k.value]
         fl = ctypes.c_double(0)
         nt, ntl, npr = ctypes.c_int(0), ctypes.c_int(0), ctypes.c_int(0)
         _check(lib().sipgpu_batch_info(self.handle, ctypes.byref(fl), ctypes.byref(nt),
@@ -484,6 +493,12 @@ class Batch:
 
     def execute(self, stream=None):
         _check(lib().sipgpu_batch_execute(self.handle, stream.handle if stream else None))
+
+    def execute_stages(self, stages, stream=None):
+        """Runs the stages in the bit mask `stages` (STAGE_PREPARE, STAGE_PRODUCT,
+        STAGE_FINISH) on `stream`."""
+        _check(lib().sipgpu_batch_execute_stages(self.handle, int(stages),
+                                                 stream.handle if stream else None))
 
     def destroy(self):
         if self.handle:

@@ -171,6 +171,26 @@ int sipgpu_batch_plan_split(const sipgpu_contraction *list, int n, int nmoduli,
                             sipgpu_batch_t *batch);
 /* Moduli count and operand bits P of a split batch (0 and 53 for FP64 batches). */
 int sipgpu_batch_split_info(sipgpu_batch_t batch, int *nmoduli, int *bits);
+/* How a planned batch runs.  sipgpu_batch_plan picks the packed path for
+ * batches of at least 1e9 flop averaging 5e7 flop per result block: each
+ * operand block is first packed once into K-contiguous panels, then one
+ * grouped GEMM runs over whole 16-k stages of every task's concatenated K
+ * (SIPGPU_FP64_PATH=tables|packed in the environment pins the choice). */
+enum { SIPGPU_BATCH_TABLES = 0, SIPGPU_BATCH_PACKED = 1, SIPGPU_BATCH_SPLIT = 2 };
+int sipgpu_batch_kind(sipgpu_batch_t batch, int *kind);
+/* Runs only some stages of a planned batch on `s` (no reference
+ * counterpart: the SIP issues a contraction as one call).  PREPARE builds
+ * the operand panels (packed and split paths: packing, row / column scales
+ * and residues), PRODUCT runs the grouped GEMM, FINISH the CRT
+ * reconstruction of a split batch.  A stage absent from a batch's path is a
+ * no-op.  Running PREPARE, PRODUCT, FINISH in that order on one stream is
+ * sipgpu_batch_execute; callers use it to time the GEMM by itself or to
+ * overlap one batch's preparation with another's product on two streams
+ * (each stage reads what the previous stage of the same batch wrote, so the
+ * caller orders them with events). */
+enum { SIPGPU_STAGE_PREPARE = 1, SIPGPU_STAGE_PRODUCT = 2, SIPGPU_STAGE_FINISH = 4,
+       SIPGPU_STAGE_ALL = 7 };
+int sipgpu_batch_execute_stages(sipgpu_batch_t batch, int stages, sipgpu_stream_t s);
 /* Sum of 2*M*N*K over the batch, and its task / tile / pair counts. */
 int sipgpu_batch_info(sipgpu_batch_t batch, double *flops, int *ntasks,
                       int *ntiles, int *npairs);

@@ -131,3 +131,78 @@ def test_full_ring_config_properties():
     sg.synchronize()
     twice = w["D"].dev.download()
     assert rel_fro(twice, 2.0 * got) <= 1e-15
+
+
+@pytest.mark.parametrize("make", [
+    lambda: wl.ladder(),
+    lambda: wl.ring(o_segs=(5, 4), v_segs=(10, 10, 7)),
+    lambda: wl.ring(o_segs=(15,), v_segs=(30, 15)),
+], ids=["ladder", "ring-small", "ring-uneven"])
+def test_packed_path_vs_oracle(make, monkeypatch):
+    """The packed FP64 path (operands packed into panels, GEMM over virtual
+    K) against the oracle, forced on batches the planner would run on the
+    table kernel."""
+    monkeypatch.setenv("SIPGPU_FP64_PATH", "packed")
+    w = make()
+    rng = np.random.default_rng(1)
+    w["D"].host[:] = rng.standard_normal(w["D"].host.size)
+    b = wl.batch_for(w)
+    assert b.kind == "packed"
+    b.execute()
+    sg.synchronize()
+    assert per_block_err(w, w["D"].dev.download(), oracle_batch(w)) <= TOL
+
+
+def test_packed_path_assign_and_long_k(monkeypatch):
+    """Assign then accumulate on the packed path, and one block summing three
+    pairs of K = 3000 (4500 chunks of virtual K: several windows of chunk
+    entries)."""
+    monkeypatch.setenv("SIPGPU_FP64_PATH", "packed")
+    rng = np.random.default_rng(12)
+    M, N, K = 20, 30, 3000
+    ptrn, e = sg.get_contraction_ptrn(2, 2, 2, [1, 2, 3, 1, 3, 2])
+    Ls = [rng.standard_normal(K * M) for _ in range(3)]
+    Rs = [rng.standard_normal(K * N) for _ in range(3)]
+    dL = [sg.DeviceArray.from_host(x) for x in Ls]
+    dR = [sg.DeviceArray.from_host(x) for x in Rs]
+    D = sg.DeviceArray.from_host(np.full(M * N, 7.0))
+    cs = [sg.make_contraction(ptrn, dL[i].ptr, [K, M], dR[i].ptr, [K, N], D.ptr, [M, N],
+                              accumulate=(i > 0)) for i in range(3)]
+    b = sg.Batch(cs)
+    assert b.kind == "packed"
+    b.execute()
+    sg.synchronize()
+    want = sum(Ls[i].reshape(K, M, order="F").T @ Rs[i].reshape(K, N, order="F")
+               for i in range(3)).ravel(order="F")
+    assert rel_fro(D.download(), want) <= TOL
+
+
+def test_full_ring_runs_packed():
+    w = wl.ring()
+    b = wl.batch_for(w)
+    assert b.kind == "packed"
+
+
+@pytest.mark.parametrize("path", ["tables", "packed", "split"])
+def test_stages_equal_execute(path, monkeypatch):
+    """PREPARE, PRODUCT, FINISH run one at a time give bit for bit what one
+    sipgpu_batch_execute gives, on each of the three paths; a repeated
+    PRODUCT after one PREPARE repeats the same product."""
+    if path != "split":
+        monkeypatch.setenv("SIPGPU_FP64_PATH", path)
+    w = wl.ring(o_segs=(9, 6), v_segs=(14, 11, 8))
+    rng = np.random.default_rng(3)
+    w["D"].host[:] = rng.standard_normal(w["D"].host.size)
+    w["D"].to_device()
+    b = wl.batch_for(w, moduli=13 if path == "split" else None)
+    assert b.kind == path
+    b.execute()
+    sg.synchronize()
+    once = w["D"].dev.download()
+    w["D"].dev.upload(w["D"].host)
+    for st in (sg.STAGE_PREPARE, sg.STAGE_PRODUCT, sg.STAGE_FINISH):
+        b.execute_stages(st)
+    sg.synchronize()
+    assert np.array_equal(w["D"].dev.download(), once)
+    with pytest.raises(sg.SipGpuError):
+        b.execute_stages(8)
