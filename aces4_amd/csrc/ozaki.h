@@ -46,23 +46,52 @@ struct OzBlock {
 };
 
 // 16 bytes of a task's virtual K: where the left and right operand rows
-// hold them (plane byte offsets of row 0; row r adds r * stride).
+// hold them (plane offsets of row 0 -- bytes in a residue plane, 16-byte
+// units in a plane of doubles; row r adds r * stride bytes).
 struct OzChunkRef {
     uint32_t a, b;
     int32_t stride;
     int32_t pad;
 };
 
-struct OzTask {
+// One result block of a merged task: where its elements go.
+struct OzSub {
     double *D;
-    int M, N;
     int mD, nD;           // destination tables
+    int beta;             // 1: accumulate into D
+    int nfast;
+};
+
+// A GEMM task: one result block, or a grid of result blocks that share a
+// virtual K (rmap >= 0).  Row m of a grid belongs to block row
+// tab[rmap + 2m] at row tab[rmap + 2m + 1] inside it (columns likewise
+// through cmap); block (g, h) is subs[sub0 + g * ncp + h].
+struct OzTask {
+    double *D;            // single block: destination and its tables
+    int M, N;
+    int mD, nD;
     int chunk0, nchunks;  // virtual K: chunks [chunk0, chunk0 + nchunks), whole stages
     int beta;
-    int expA, expB;       // exponent offsets of its row / column components
+    int expA, expB;       // first row / column exponent
     int nfast;            // 1: result elements ordered with n fastest
     long long res_off;    // element offset inside each result plane
+    int rmap, cmap;       // grid: row / column maps in the table pool, else -1
+    int sub0, ncp;        // grid: first block record, block columns
 };
+
+// Destination offset of element (m, n) of a task.
+__device__ __forceinline__ double *oz_dest(const OzTask &t, const OzSub *subs, const int32_t *tab,
+                                           int m, int n, int &beta) {
+    if (t.rmap < 0) {
+        beta = t.beta;
+        return t.D + tab[t.mD + m] + tab[t.nD + n];
+    }
+    const int g = tab[t.rmap + 2 * m], lm = tab[t.rmap + 2 * m + 1];
+    const int h = tab[t.cmap + 2 * n], ln = tab[t.cmap + 2 * n + 1];
+    const OzSub s = subs[t.sub0 + g * t.ncp + h];
+    beta = s.beta;
+    return s.D + tab[s.mD + lm] + tab[s.nD + ln];
+}
 
 struct OzTile {
     int task, mt, nt, pad;
@@ -89,6 +118,7 @@ struct OzGemmArgs {
     int ntiles;
     int tile; // kOzBM (= kOzBN) or kOzBigTile
     const int32_t *tab; // destination tables (packed FP64 path)
+    const OzSub *subs;  // result blocks of merged tasks (packed FP64 path)
     OzConst c;
 };
 
@@ -108,6 +138,7 @@ struct OzRecArgs {
     const OzRecItem *items;
     int nitems;
     const int32_t *tab;
+    const OzSub *subs;
     const int *expo;
     const uint8_t *res_planes;
     long long res_plane_bytes;

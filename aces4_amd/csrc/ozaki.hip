@@ -461,10 +461,10 @@ __global__ __launch_bounds__(256) void reconstruct_kernel(OzRecArgs a) {
                     const int sa = (a.c.P - 1) - ea, sb = (a.c.P - 1) - eb;
                     v = ldexp(to_double(C), -(sa + sb));
                 }
-                const int off = a.tab[task.mD + m] + a.tab[task.nD + n];
-                double *D = task.D;
-                if (task.beta) v += D[off];
-                D[off] = v;
+                int beta;
+                double *p = oz_dest(task, a.subs, a.tab, m, n, beta);
+                if (beta) v += *p;
+                *p = v;
             }
         }
     }

@@ -79,8 +79,9 @@ __global__ __launch_bounds__(256) void pack_kernel(OzConvArgs a) {
 // 2 x 2, each 64 x 64: 16 accumulator fragments); stages of
 // 128 bytes (16 k) double-buffered in LDS with rows XOR-swizzled by 16-byte
 // chunk; loads run one stage ahead through registers; chunk entries come
-// from an LDS window.  Rows past the task's M / N read row M-1 / N-1 and
-// are never stored; chunks past the pairs' K read the plane's zero bytes.
+// from an LDS window (their plane offsets count 16-byte units, so panels may
+// reach 64 GiB).  Rows past the task's M / N read row M-1 / N-1 and are
+// never stored; chunks past the pairs' K read the plane's zero bytes.
 __global__ __launch_bounds__(256, 2) void gemm_f64_packed_kernel(OzGemmArgs a) {
     constexpr int BM = 128, BN = 128, WM = 2, WN = 2;
     constexpr int TM = BM / WM, TN = BN / WN, FM = TM / 16, FN = TN / 16;
@@ -138,11 +139,11 @@ __global__ __launch_bounds__(256, 2) void gemm_f64_packed_kernel(OzGemmArgs a) {
 #pragma unroll
             for (int i = 0; i < IA; ++i)
                 xa[i] = *reinterpret_cast<const __attribute__((address_space(1))) int32x4 *>(
-                    plane + e.a + (long long)ra[i] * e.stride);
+                    plane + (long long)e.a * kOzChunk + (long long)ra[i] * e.stride);
 #pragma unroll
             for (int i = 0; i < IB; ++i)
                 xb[i] = *reinterpret_cast<const __attribute__((address_space(1))) int32x4 *>(
-                    plane + e.b + (long long)rb[i] * e.stride);
+                    plane + (long long)e.b * kOzChunk + (long long)rb[i] * e.stride);
         };
         auto put = [&](int buf) {
 #pragma unroll
@@ -193,26 +194,48 @@ __global__ __launch_bounds__(256, 2) void gemm_f64_packed_kernel(OzGemmArgs a) {
             buf ^= 1;
         }
 
-        // scatter into the destination block
-        __attribute__((address_space(1))) double *D =
-            (__attribute__((address_space(1))) double *)task.D;
+        // scatter into the destination block(s)
+        if (task.rmap < 0) {
+            __attribute__((address_space(1))) double *D =
+                (__attribute__((address_space(1))) double *)task.D;
 #pragma unroll
-        for (int j = 0; j < FN; ++j) {
-            const int n = n0 + wn * TN + j * 16 + li;
-            if (n >= task.N) continue;
-            const int no = a.tab[task.nD + n];
+            for (int j = 0; j < FN; ++j) {
+                const int n = n0 + wn * TN + j * 16 + li;
+                if (n >= task.N) continue;
+                const int no = a.tab[task.nD + n];
 #pragma unroll
-            for (int i = 0; i < FM; ++i)
+                for (int i = 0; i < FM; ++i)
 #pragma unroll
-                for (int r = 0; r < 4; ++r) {
-                    const int m = m0 + wm * TM + i * 16 + 4 * r + lk;
-                    if (m < task.M) {
-                        const int off = a.tab[task.mD + m] + no;
-                        double v = acc[i][j][r];
-                        if (task.beta) v += D[off];
-                        D[off] = v;
+                    for (int r = 0; r < 4; ++r) {
+                        const int m = m0 + wm * TM + i * 16 + 4 * r + lk;
+                        if (m < task.M) {
+                            const int off = a.tab[task.mD + m] + no;
+                            double v = acc[i][j][r];
+                            if (task.beta) v += D[off];
+                            D[off] = v;
+                        }
                     }
-                }
+            }
+        } else {
+            // a grid of result blocks: each element finds its block
+#pragma unroll
+            for (int j = 0; j < FN; ++j) {
+                const int n = n0 + wn * TN + j * 16 + li;
+                if (n >= task.N) continue;
+#pragma unroll
+                for (int i = 0; i < FM; ++i)
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+                        const int m = m0 + wm * TM + i * 16 + 4 * r + lk;
+                        if (m < task.M) {
+                            int beta;
+                            double *p = oz_dest(task, a.subs, a.tab, m, n, beta);
+                            double v = acc[i][j][r];
+                            if (beta) v += *p;
+                            *p = v;
+                        }
+                    }
+            }
         }
     }
 }

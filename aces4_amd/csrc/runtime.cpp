@@ -15,10 +15,12 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <functional>
 #include <map>
 #include <memory>
 #include <tuple>
 #include <mutex>
+#include <set>
 #include <string>
 #include <unordered_map>
 #include <vector>
@@ -279,6 +281,7 @@ struct sipgpu_batch_s {
     int cfg = kGemm64x64;
     int ntiles = 0;     // including XCD padding slots
     int ntasks = 0, npairs = 0, real_tiles = 0;
+    int ngroups = 0;    // panel paths: GEMM tasks after merging result blocks
     double flops = 0.0;
     void *dev = nullptr; // tiles | tasks | pairs | tab
     GemmArgs args{};
@@ -613,16 +616,6 @@ int oz_constants(int n, long long kmax, OzConst &c) {
     return P;
 }
 
-struct UnionFind {
-    std::vector<int> parent;
-    explicit UnionFind(int n) : parent(n) { std::iota(parent.begin(), parent.end(), 0); }
-    int find(int x) {
-        while (parent[x] != x) x = parent[x] = parent[parent[x]];
-        return x;
-    }
-    void unite(int a, int b) { parent[find(a)] = find(b); }
-};
-
 // Panel planner: nmod > 0 plans the split-precision path (1-byte residue
 // panels, nmod moduli); nmod == 0 the packed FP64 path (panels of doubles,
 // products scattered straight into the destinations).
@@ -663,30 +656,32 @@ int plan_panels_locked(const sipgpu_contraction *list, int n, int nmod, sipgpu_b
     std::vector<std::array<int, 6>> tabs(shapes.size());
     for (size_t si = 0; si < shapes.size(); ++si) tabs[si] = tp.add_shape(*shapes[si]);
     auto unit_stride = [&](int off, int len) { return len > 1 && pool[off + 1] - pool[off] == 1; };
-    // operand instances: one residue panel per (block, k table, x table)
-    std::map<std::tuple<const double *, int, int, int, int>, int> inst;
-    std::vector<OzBlock> blocks;
-    auto operand = [&](const double *ptr, int si, int side) {
+    // operand instances: one per (block, k table, x table)
+    struct Inst { const double *ptr; int K, X, Kp, ktab, xtab, xfast; };
+    std::map<std::tuple<const double *, int, int, int, int>, int> inst_index;
+    std::vector<Inst> insts;
+    const int kq = kOzChunk / eb; // panel elements per 16-byte chunk
+    auto instance = [&](const double *ptr, int si, int side) {
         const ContractionShape &sh = *shapes[si];
         const int ktab = tabs[si][side == 0 ? 0 : 2], xtab = tabs[si][side == 0 ? 1 : 3];
         const int X = side == 0 ? (int)sh.M : (int)sh.N;
         const auto key = std::make_tuple(ptr, ktab, xtab, (int)sh.K, X);
-        auto it = inst.find(key);
-        if (it != inst.end()) return it->second;
-        OzBlock ob{};
-        ob.data = ptr;
-        ob.K = (int)sh.K;
-        ob.X = X;
-        const int kq = kOzChunk / eb; // elements per 16-byte chunk
-        ob.Kp = (int)((sh.K + kq - 1) / kq * kq);
-        ob.ktab = ktab;
-        ob.xtab = xtab;
-        ob.xfast = unit_stride(xtab, X) ? 1 : 0;
-        blocks.push_back(ob);
-        inst.emplace(key, (int)blocks.size() - 1);
-        return (int)blocks.size() - 1;
+        auto it = inst_index.find(key);
+        if (it != inst_index.end()) return it->second;
+        Inst in{ptr, (int)sh.K, X, (int)((sh.K + kq - 1) / kq * kq), ktab, xtab,
+                unit_stride(xtab, X) ? 1 : 0};
+        insts.push_back(in);
+        inst_index.emplace(key, (int)insts.size() - 1);
+        return (int)insts.size() - 1;
     };
-    // tasks grouped by destination (same rules as the FP64 batch)
+    // result blocks, one per destination (same rules as the FP64 batch):
+    // the pairs from its last assigning entry on, summed in list order
+    struct Dest {
+        OzSub s;
+        int M, N;
+        std::vector<int> a, b; // left / right instance of each pair
+        double cost;
+    };
     std::unordered_map<const double *, std::vector<int>> by_dest;
     std::vector<const double *> dest_order;
     for (int i = 0; i < n; ++i) {
@@ -694,10 +689,7 @@ int plan_panels_locked(const sipgpu_contraction *list, int n, int nmod, sipgpu_b
         if (v.empty()) dest_order.push_back(list[i].d);
         v.push_back(i);
     }
-    struct TaskPairs { std::vector<std::pair<int, int>> ab; };
-    std::vector<OzTask> tasks;
-    std::vector<TaskPairs> task_pairs;
-    std::vector<double> task_cost;
+    std::vector<Dest> dests;
     long long kmax = 0;
     int npairs = 0;
     b.flops = 0.0;
@@ -708,98 +700,269 @@ int plan_panels_locked(const sipgpu_contraction *list, int n, int nmod, sipgpu_b
             if (!list[idx[j]].accumulate) start = j;
         const int si0 = shape_of[idx[start]];
         const ContractionShape &s0 = *shapes[si0];
-        OzTask t{};
-        t.D = const_cast<double *>(dp);
-        t.M = (int)s0.M;
-        t.N = (int)s0.N;
-        t.mD = tabs[si0][4];
-        t.nD = tabs[si0][5];
-        t.beta = list[idx[start]].accumulate ? 1 : 0;
-        t.nfast = (unit_stride(t.nD, t.N) || t.M == 1) ? 1 : 0;
-        TaskPairs tpairs;
+        Dest d{};
+        d.M = (int)s0.M;
+        d.N = (int)s0.N;
+        d.s.D = const_cast<double *>(dp);
+        d.s.mD = tabs[si0][4];
+        d.s.nD = tabs[si0][5];
+        d.s.beta = list[idx[start]].accumulate ? 1 : 0;
+        d.s.nfast = (unit_stride(d.s.nD, d.N) || d.M == 1) ? 1 : 0;
         long long ktot = 0, kptot = 0;
-        double cost = 0.0;
         for (size_t j = start; j < idx.size(); ++j) {
             const int i = idx[j];
             const int si = shape_of[i];
             const ContractionShape &sh = *shapes[si];
             if (sh.M != s0.M || sh.N != s0.N || sh.moffD != s0.moffD || sh.noffD != s0.noffD)
                 return fail(SIPGPU_EARG, "entries writing one block disagree on its layout");
-            const int ia = operand(list[i].l, si, 0), ib = operand(list[i].r, si, 1);
-            tpairs.ab.emplace_back(ia, ib);
+            d.a.push_back(instance(list[i].l, si, 0));
+            d.b.push_back(instance(list[i].r, si, 1));
             ktot += sh.K;
-            kptot += blocks[ia].Kp;
-            cost += 2.0 * (double)sh.M * (double)sh.N * (double)sh.K;
+            kptot += insts[d.a.back()].Kp;
+            d.cost += 2.0 * (double)sh.M * (double)sh.N * (double)sh.K;
         }
         // exact int32 accumulation: sum of |a b| <= Kvirtual * 127^2 < 2^31
         if (nmod && kptot + kOzStage > 131072)
             return fail(SIPGPU_ESIZE, "split path: a result block sums more than 131072 k");
-        if ((long long)t.M * t.N >= INT_MAX)
+        if ((long long)d.M * d.N >= INT_MAX)
             return fail(SIPGPU_ESIZE, "split path: result block too large");
-        npairs += (int)tpairs.ab.size();
+        npairs += (int)d.a.size();
         kmax = std::max(kmax, ktot);
-        b.flops += cost;
-        tasks.push_back(t);
-        task_pairs.push_back(std::move(tpairs));
-        task_cost.push_back(cost);
+        b.flops += d.cost;
+        dests.push_back(std::move(d));
     }
+    const int ndest = (int)dests.size();
+
+    // Merged tasks.  A row key is a result block's list of left instances,
+    // a column key its list of right instances.  Result blocks whose column
+    // keys pair with the same set of row keys form a grid (rows x columns)
+    // over one virtual K: per pair, the grid's left instances are stacked
+    // row after row in one panel and its right ones likewise, and the grid
+    // runs as one GEMM whose tiles cross block boundaries (a 450-row block
+    // no longer wastes a quarter of its last 128-row tile).  Grids are cut to
+    // at most kMaxGrid rows and columns.
+    std::map<std::vector<int>, int> rkey, ckey;
+    std::vector<std::vector<int>> rvec, cvec; // instance lists by key
+    std::vector<int> d_r(ndest), d_c(ndest);
+    std::set<std::pair<int, int>> seen;
+    for (int i = 0; i < ndest; ++i) {
+        auto r = rkey.emplace(dests[i].a, (int)rvec.size());
+        if (r.second) rvec.push_back(dests[i].a);
+        auto c = ckey.emplace(dests[i].b, (int)cvec.size());
+        if (c.second) cvec.push_back(dests[i].b);
+        d_r[i] = r.first->second;
+        d_c[i] = c.first->second;
+        if (!seen.insert({d_r[i], d_c[i]}).second) { // the same product into two blocks
+            d_c[i] = (int)cvec.size();
+            cvec.push_back(dests[i].b);
+        }
+    }
+    auto rows_ext = [&](int key) { return insts[rvec[key][0]].X; };
+    auto cols_ext = [&](int key) { return insts[cvec[key][0]].X; };
+    std::vector<std::vector<int>> rows_of(cvec.size());
+    std::map<std::pair<int, int>, int> dest_at;
+    for (int i = 0; i < ndest; ++i) {
+        rows_of[d_c[i]].push_back(d_r[i]);
+        dest_at[{d_r[i], d_c[i]}] = i;
+    }
+    std::map<std::vector<int>, int> grid_of;
+    std::vector<std::pair<std::vector<int>, std::vector<int>>> grids; // (row keys, col keys)
+    const char *merge_env = std::getenv("SIPGPU_MERGE"); // "0": one task per result block
+    if (merge_env && !std::strcmp(merge_env, "0")) {
+        for (int i = 0; i < ndest; ++i) grids.push_back({{d_r[i]}, {d_c[i]}});
+    } else {
+        for (int c = 0; c < (int)cvec.size(); ++c) {
+            if (rows_of[c].empty()) continue;
+            std::sort(rows_of[c].begin(), rows_of[c].end());
+            auto it = grid_of.emplace(rows_of[c], (int)grids.size());
+            if (it.second) grids.push_back({rows_of[c], {}});
+            grids[it.first->second].second.push_back(c);
+        }
+    }
+    constexpr int kMaxGrid = 32768;
+    auto cut = [](const std::vector<int> &keys, const std::function<int(int)> &ext) {
+        std::vector<std::vector<int>> out(1);
+        long long s = 0;
+        for (int k : keys) {
+            if (!out.back().empty() && s + ext(k) > kMaxGrid) {
+                out.emplace_back();
+                s = 0;
+            }
+            out.back().push_back(k);
+            s += ext(k);
+        }
+        return out;
+    };
+    std::vector<OzTask> tasks;
+    std::vector<double> task_cost;
+    std::vector<std::vector<int>> task_rows, task_cols; // row / column keys of each task
+    std::vector<OzSub> subs;
+    for (const auto &gr : grids)
+        for (const auto &rs : cut(gr.first, rows_ext))
+            for (const auto &cs : cut(gr.second, cols_ext)) {
+                OzTask t{};
+                const Dest &d0 = dests[dest_at.at({rs[0], cs[0]})];
+                double cost = 0.0;
+                for (int r : rs) t.M += rows_ext(r);
+                for (int c : cs) t.N += cols_ext(c);
+                if ((long long)t.M * t.N >= INT_MAX)
+                    return fail(SIPGPU_ESIZE, "split path: result block too large");
+                t.nfast = d0.s.nfast;
+                t.rmap = t.cmap = -1;
+                if (rs.size() == 1 && cs.size() == 1) {
+                    t.D = d0.s.D;
+                    t.mD = d0.s.mD;
+                    t.nD = d0.s.nD;
+                    t.beta = d0.s.beta;
+                    cost = d0.cost;
+                } else {
+                    // row map: (row key ordinal, row inside its block) per row
+                    t.rmap = (int)pool.size();
+                    for (size_t g = 0; g < rs.size(); ++g)
+                        for (int m = 0; m < rows_ext(rs[g]); ++m) {
+                            pool.push_back((int32_t)g);
+                            pool.push_back(m);
+                        }
+                    t.cmap = (int)pool.size();
+                    for (size_t h = 0; h < cs.size(); ++h)
+                        for (int m = 0; m < cols_ext(cs[h]); ++m) {
+                            pool.push_back((int32_t)h);
+                            pool.push_back(m);
+                        }
+                    t.sub0 = (int)subs.size();
+                    t.ncp = (int)cs.size();
+                    for (int r : rs)
+                        for (int c : cs) {
+                            const Dest &d = dests[dest_at.at({r, c})];
+                            subs.push_back(d.s);
+                            cost += d.cost;
+                        }
+                }
+                tasks.push_back(t);
+                task_cost.push_back(cost);
+                task_rows.push_back(rs);
+                task_cols.push_back(cs);
+            }
     const int ntasks = (int)tasks.size();
-    // row / column components: operands of one task share its rows (columns)
-    const int nb = (int)blocks.size();
-    UnionFind uf(nb);
-    for (const TaskPairs &tpp : task_pairs)
-        for (size_t q = 1; q < tpp.ab.size(); ++q) {
-            uf.unite(tpp.ab[0].first, tpp.ab[q].first);
-            uf.unite(tpp.ab[0].second, tpp.ab[q].second);
-        }
-    std::unordered_map<int, int> comp_off;
-    int nexpo = 0;
-    for (int i = 0; i < nb; ++i) {
-        const int r = uf.find(i);
-        auto it = comp_off.find(r);
-        if (it == comp_off.end()) {
-            it = comp_off.emplace(r, nexpo).first;
-            nexpo += blocks[r].X;
-        }
-        if (blocks[i].X != blocks[r].X)
-            return fail(SIPGPU_EARG, "split path: operands sharing rows disagree on extents");
-        blocks[i].exp_off = it->second;
-    }
-    for (int t = 0; t < ntasks; ++t) {
-        tasks[t].expA = blocks[task_pairs[t].ab[0].first].exp_off;
-        tasks[t].expB = blocks[task_pairs[t].ab[0].second].exp_off;
-    }
-    // panels (after the plane's zero bytes), conversion work
+    if (pool.size() >= (size_t)INT_MAX) return fail(SIPGPU_ESIZE, "batch tables too large");
+
+    // Operand panels (after the plane's zero bytes).  A merged task's stack
+    // for one pair holds a copy of each of its keys' instances, row after
+    // row, with exponents of its own.  An unstacked operand is converted
+    // once and shared by every task that uses it unstacked; on the split
+    // path those tasks then share its row scales, so the instances of one
+    // task side (all scaled alike: its rows share one scale over all its
+    // pairs) are united into components with one exponent slot each.
+    std::vector<OzBlock> blocks;
+    std::map<std::pair<int, int>, int> shared_copy; // (instance, side) -> block
     long long panel = kOzZero;
+    int nexpo = 0;
+    std::vector<int> uf(2 * insts.size());
+    std::iota(uf.begin(), uf.end(), 0);
+    auto find = [&](int x) {
+        while (uf[x] != x) x = uf[x] = uf[uf[x]];
+        return x;
+    };
+    for (int t = 0; t < ntasks && nmod; ++t)
+        for (int side = 0; side < 2; ++side) {
+            const auto &keys = side == 0 ? task_rows[t] : task_cols[t];
+            if (keys.size() != 1) continue;
+            const auto &v = (side == 0 ? rvec : cvec)[keys[0]];
+            for (int ii : v) uf[find(2 * ii + side)] = find(2 * v[0] + side);
+        }
+    std::unordered_map<int, int> comp_slot;
+    auto slot_of = [&](int ii, int side) {
+        auto it = comp_slot.emplace(find(2 * ii + side), nexpo);
+        if (it.second) nexpo += insts[ii].X;
+        return it.first->second;
+    };
+    auto place = [&](int ii, int exp_off) {
+        const Inst &in = insts[ii];
+        OzBlock ob{};
+        ob.data = in.ptr;
+        ob.K = in.K;
+        ob.X = in.X;
+        ob.Kp = in.Kp;
+        ob.ktab = in.ktab;
+        ob.xtab = in.xtab;
+        ob.xfast = in.xfast;
+        ob.exp_off = exp_off;
+        ob.panel_off = panel;
+        panel += (long long)in.X * in.Kp * eb;
+        blocks.push_back(ob);
+        return (int)blocks.size() - 1;
+    };
+    auto align_panel = [&]() { panel = (panel + 255) / 256 * 256; };
+    // chunk offsets: bytes in a residue plane, 16-byte units in an FP64 one
+    const long long unit = nmod ? 1 : kOzChunk;
+    std::vector<OzChunkRef> chunks;
+    for (int t = 0; t < ntasks; ++t) {
+        const auto &rs = task_rows[t], &cs = task_cols[t];
+        if (nmod) {
+            for (int side = 0; side < 2; ++side) {
+                const auto &keys = side == 0 ? rs : cs;
+                int &e = side == 0 ? tasks[t].expA : tasks[t].expB;
+                if (keys.size() == 1) {
+                    e = slot_of((side == 0 ? rvec : cvec)[keys[0]][0], side);
+                } else {
+                    e = nexpo;
+                    nexpo += side == 0 ? tasks[t].M : tasks[t].N;
+                }
+            }
+        }
+        auto stack = [&](int side, int p) {
+            const auto &keys = side == 0 ? rs : cs;
+            const auto &vec = side == 0 ? rvec : cvec;
+            const int exp0 = side == 0 ? tasks[t].expA : tasks[t].expB;
+            if (keys.size() == 1) {
+                const int ii = vec[keys[0]][p];
+                // the FP64 panel of an instance is the same on either side
+                const auto key = std::make_pair(ii, nmod ? side : 0);
+                auto it = shared_copy.find(key);
+                if (it == shared_copy.end()) {
+                    it = shared_copy.emplace(key, place(ii, exp0)).first;
+                    align_panel();
+                }
+                return blocks[it->second].panel_off;
+            }
+            const long long base = panel;
+            int off = 0;
+            for (int key : keys) {
+                const int ii = vec[key][p];
+                place(ii, exp0 + off);
+                off += insts[ii].X;
+            }
+            align_panel();
+            return base;
+        };
+        // virtual K: the pairs' panel chunks, padded to whole pairs of
+        // stages (the GEMM runs two stages per loop trip) with chunks of the
+        // zero bytes
+        tasks[t].chunk0 = (int)chunks.size();
+        const int np = (int)rvec[rs[0]].size();
+        for (int p = 0; p < np; ++p) {
+            const long long la = stack(0, p), lb = stack(1, p);
+            const int Kp = insts[rvec[rs[0]][p]].Kp;
+            for (int k = 0; k < Kp; k += kq)
+                chunks.push_back(OzChunkRef{(uint32_t)((la + (long long)k * eb) / unit),
+                                            (uint32_t)((lb + (long long)k * eb) / unit), Kp * eb,
+                                            0});
+            if (panel >= (1LL << 32) * unit - 65536)
+                return fail(SIPGPU_ESIZE, nmod ? "split path: operand panels exceed 4 GiB per modulus"
+                                               : "packed path: operand panels exceed 64 GiB");
+        }
+        while ((chunks.size() - tasks[t].chunk0) % (2 * kOzStageChunks))
+            chunks.push_back(OzChunkRef{0u, 0u, 0, 0});
+        tasks[t].nchunks = (int)chunks.size() - tasks[t].chunk0;
+    }
+    // conversion work
     std::vector<OzConvTile> conv, rows;
-    for (int i = 0; i < nb; ++i) {
-        blocks[i].panel_off = panel;
-        panel += ((long long)blocks[i].X * blocks[i].Kp * eb + 255) / 256 * 256;
+    for (int i = 0; i < (int)blocks.size(); ++i)
         for (int x0 = 0; x0 < blocks[i].X; x0 += kOzTileX) {
             rows.push_back(OzConvTile{i, x0, 0, 0});
             for (int k0 = 0; k0 < blocks[i].Kp; k0 += kOzTileK)
                 conv.push_back(OzConvTile{i, x0, k0, 0});
         }
-    }
-    if (panel >= (1LL << 32) - 65536)
-        return fail(SIPGPU_ESIZE, "split path: operand panels exceed 4 GiB per modulus");
-    // virtual K of every task: its pairs' panel chunks, padded to whole
-    // stages with chunks of the zero bytes
-    std::vector<OzChunkRef> chunks;
-    for (int t = 0; t < ntasks; ++t) {
-        tasks[t].chunk0 = (int)chunks.size();
-        for (const auto &ab : task_pairs[t].ab) {
-            const OzBlock &A = blocks[ab.first], &B = blocks[ab.second];
-            for (int k = 0; k < A.Kp; k += kOzChunk / eb)
-                chunks.push_back(OzChunkRef{(uint32_t)(A.panel_off + (long long)k * eb),
-                                            (uint32_t)(B.panel_off + (long long)k * eb),
-                                            A.Kp * eb, 0});
-        }
-        // whole pairs of stages: the GEMM runs two stages per loop trip
-        while ((chunks.size() - tasks[t].chunk0) % (2 * kOzStageChunks))
-            chunks.push_back(OzChunkRef{0u, 0u, 0, 0});
-        tasks[t].nchunks = (int)chunks.size() - tasks[t].chunk0;
-    }
     // result planes, reconstruction work
     long long res_elems = 0;
     std::vector<OzRecItem> rec;
@@ -849,6 +1012,7 @@ int plan_panels_locked(const sipgpu_contraction *list, int n, int nmod, sipgpu_b
     const size_t o_conv = off; off = align(off + conv.size() * sizeof(OzConvTile));
     const size_t o_rows = off; off = align(off + rows.size() * sizeof(OzConvTile));
     const size_t o_rec = off; off = align(off + rec.size() * sizeof(OzRecItem));
+    const size_t o_subs = off; off = align(off + subs.size() * sizeof(OzSub));
     const size_t o_expo = off; off = align(off + (size_t)nexpo * sizeof(int));
     const size_t total = off + 256;
     std::vector<unsigned char> host(total, 0);
@@ -863,6 +1027,7 @@ int plan_panels_locked(const sipgpu_contraction *list, int n, int nmod, sipgpu_b
     put(o_conv, conv.data(), conv.size() * sizeof(OzConvTile));
     put(o_rows, rows.data(), rows.size() * sizeof(OzConvTile));
     put(o_rec, rec.data(), rec.size() * sizeof(OzRecItem));
+    put(o_subs, subs.data(), subs.size() * sizeof(OzSub));
     HIP_TRY(hipMalloc(&b.dev, total));
     HIP_TRY(hipMemcpy(b.dev, host.data(), total, hipMemcpyHostToDevice));
     unsigned char *base = static_cast<unsigned char *>(b.dev);
@@ -902,12 +1067,14 @@ int plan_panels_locked(const sipgpu_contraction *list, int n, int nmod, sipgpu_b
     g.ntiles = (int)tiles.size();
     g.tile = tile;
     g.tab = dtab;
+    g.subs = reinterpret_cast<const OzSub *>(base + o_subs);
     g.c = oc;
     OzRecArgs &r = sd.rargs;
     r.tasks = dtasks;
     r.items = reinterpret_cast<const OzRecItem *>(base + o_rec);
     r.nitems = (int)rec.size();
     r.tab = dtab;
+    r.subs = g.subs;
     r.expo = sd.expo;
     r.res_planes = sd.res_planes;
     r.res_plane_bytes = sd.res_plane_bytes;
@@ -915,7 +1082,8 @@ int plan_panels_locked(const sipgpu_contraction *list, int n, int nmod, sipgpu_b
     b.split = nmod > 0;
     b.packed = nmod == 0;
     b.ntiles = (int)tiles.size();
-    b.ntasks = ntasks;
+    b.ntasks = ndest;
+    b.ngroups = ntasks;
     b.npairs = npairs;
     b.real_tiles = real_tiles;
     return 0;

@@ -1,0 +1,121 @@
+"""GPU parity of merged tasks: result blocks that share a virtual K run as
+one grid-shaped GEMM (operand panels stacked per pair, each element routed
+back to its own block).  Every case is checked against the oracle (the SIP's
+own tensor_block_contract + block add sequence) on both panel paths, and
+against the same batch planned without merging (SIPGPU_MERGE=0)."""
+import numpy as np
+import pytest
+
+from cases import rel_fro
+from checker import Oracle
+from test_gpu_batch import oracle_batch, per_block_err
+
+from aces4_amd import sipgpu as sg
+from aces4_amd import workloads as wl
+
+pytestmark = pytest.mark.gpu
+o = Oracle()
+TOL = {"packed": 1e-13, "split": 1e-12}
+PATHS = ["packed", "split"]
+
+
+def plan(cs, path, monkeypatch):
+    if path == "packed":
+        monkeypatch.setenv("SIPGPU_FP64_PATH", "packed")
+        b = sg.Batch(cs)
+    else:
+        b = sg.Batch(cs, moduli=14)
+    assert b.kind == path
+    return b
+
+
+@pytest.mark.parametrize("path", PATHS)
+@pytest.mark.parametrize("make", [
+    lambda: wl.ring(o_segs=(9, 6), v_segs=(14, 11, 8)),
+    lambda: wl.ladder(),
+], ids=["ring", "ladder"])
+def test_merged_vs_unmerged(make, path, monkeypatch):
+    """Merging cuts the tile count of uneven blocks and changes no result
+    beyond the path's tolerance."""
+    w = make()
+    rng = np.random.default_rng(4)
+    w["D"].host[:] = rng.standard_normal(w["D"].host.size)
+    want = oracle_batch(w)
+    out = {}
+    for merge in ("1", "0"):
+        monkeypatch.setenv("SIPGPU_MERGE", merge)
+        w["D"].dev = None
+        if path == "packed":
+            monkeypatch.setenv("SIPGPU_FP64_PATH", "packed")
+        b = wl.batch_for(w, moduli=14 if path == "split" else None)
+        assert b.kind == path
+        b.execute()
+        sg.synchronize()
+        got = w["D"].dev.download()
+        print(path, merge, b.ntiles, per_block_err(w, got, want), flush=True)
+        assert per_block_err(w, got, want) <= TOL[path]
+        out[merge] = (b.ntiles, got)
+        w["D"].dev.free()
+        b.destroy()
+    assert out["1"][0] <= out["0"][0]
+    assert rel_fro(out["1"][1], out["0"][1]) <= 2 * TOL[path]
+
+
+def blocks(rng, shapes):
+    host = [rng.standard_normal(int(np.prod(s))) for s in shapes]
+    return host, [sg.DeviceArray.from_host(x) for x in host]
+
+
+@pytest.mark.parametrize("path", PATHS)
+def test_same_product_into_two_blocks(path, monkeypatch):
+    """Two result blocks with the same pair list (the same product) and a
+    third sharing their right operand: the duplicate is not merged into the
+    same grid cell, and every block gets its own (assign or accumulate)."""
+    rng = np.random.default_rng(5)
+    ptrn, e = sg.get_contraction_ptrn(2, 2, 2, [1, 2, 3, 1, 3, 2])  # D(m,n) = L(k,m) R(k,n)
+    assert e == 0
+    K, M, N = 40, 33, 27
+    (Lh, Rh), (Ld, Rd) = blocks(rng, [(K, M), (K, N)])
+    (L2h,), (L2d,) = blocks(rng, [(K, M)])
+    Dh = [rng.standard_normal(M * N) for _ in range(3)]
+    Dd = [sg.DeviceArray.from_host(x) for x in Dh]
+    cs = [sg.make_contraction(ptrn, Ld.ptr, [K, M], Rd.ptr, [K, N], Dd[0].ptr, [M, N],
+                              accumulate=True),
+          sg.make_contraction(ptrn, Ld.ptr, [K, M], Rd.ptr, [K, N], Dd[1].ptr, [M, N],
+                              accumulate=False),
+          sg.make_contraction(ptrn, L2d.ptr, [K, M], Rd.ptr, [K, N], Dd[2].ptr, [M, N],
+                              accumulate=True)]
+    b = plan(cs, path, monkeypatch)
+    b.execute()
+    sg.synchronize()
+    prod = lambda L: (L.reshape(K, M, order="F").T @ Rh.reshape(K, N, order="F")).ravel(order="F")
+    wants = [Dh[0] + prod(Lh), prod(Lh), Dh[2] + prod(L2h)]
+    for d, want in zip(Dd, wants):
+        assert rel_fro(d.download(), want) <= TOL[path]
+
+
+@pytest.mark.parametrize("path", PATHS)
+def test_grid_cut_at_row_limit(path, monkeypatch):
+    """130 result blocks of 300 rows sharing one right operand form a grid of
+    39000 rows, more than one task may hold (32768): it is cut in two, and
+    the rows on both sides of the cut land in their blocks."""
+    rng = np.random.default_rng(6)
+    ptrn, e = sg.get_contraction_ptrn(2, 2, 2, [1, 2, 3, 1, 3, 2])
+    assert e == 0
+    K, M, N, nb = 16, 300, 20, 130
+    Rh = rng.standard_normal(K * N)
+    Rd = sg.DeviceArray.from_host(Rh)
+    Lh = rng.standard_normal(nb * K * M)
+    Ld = sg.DeviceArray.from_host(Lh)
+    Dd = sg.DeviceArray(nb * M * N)
+    Dd.zero()
+    cs = [sg.make_contraction(ptrn, Ld.at(i * K * M), [K, M], Rd.ptr, [K, N],
+                              Dd.at(i * M * N), [M, N], accumulate=False) for i in range(nb)]
+    b = plan(cs, path, monkeypatch)
+    b.execute()
+    sg.synchronize()
+    got = Dd.download().reshape(nb, M * N)
+    R = Rh.reshape(K, N, order="F")
+    for i in range(nb):
+        want = (Lh[i * K * M:(i + 1) * K * M].reshape(K, M, order="F").T @ R).ravel(order="F")
+        assert rel_fro(got[i], want) <= TOL[path], i
