@@ -93,8 +93,14 @@ __device__ __forceinline__ double *oz_dest(const OzTask &t, const OzSub *subs, c
     return s.D + tab[s.mD + lm] + tab[s.nD + ln];
 }
 
+// A GEMM work unit: tile (mt, nt) of a task over stages [s0, s1) of its
+// virtual K (whole pairs of stages).  part < 0: the whole K, the product is
+// stored; part >= 0: one K segment of a split tile, its accumulators go to
+// workspace slot `part` and the fixup pass sums the tile's segments.  In the
+// fixup list, part is the tile's first slot and s0 its number of segments.
 struct OzTile {
-    int task, mt, nt, pad;
+    int task, mt, nt, part;
+    int s0, s1, pad0, pad1;
 };
 
 // Conversion work item: rows [x0, x0 + kOzTileX) and k [k0, k0 + kOzTileK) of a block.
@@ -119,6 +125,11 @@ struct OzGemmArgs {
     int tile; // kOzBM (= kOzBN) or kOzBigTile
     const int32_t *tab; // destination tables (packed FP64 path)
     const OzSub *subs;  // result blocks of merged tasks (packed FP64 path)
+    double *work;       // split-tile partial products, kOzBM * kOzBN per slot
+    const OzTile *fix;  // split tiles to sum and store
+    int nfix;
+    const int *wg_first; // persistent launch: units [wg_first[w], wg_first[w+1]) of
+    int nwg;             // workgroup w (nullptr: one tile per workgroup)
     OzConst c;
 };
 
@@ -156,6 +167,8 @@ hipError_t oz_gemm_big(const OzGemmArgs &a, hipStream_t s);
 // Packed FP64 path: operand panels of doubles, then the GEMM over them.
 hipError_t oz_pack(const OzConvArgs &a, hipStream_t s);
 hipError_t oz_gemm_f64(const OzGemmArgs &a, hipStream_t s);
+// Sums the K segments of split tiles (in segment order) and stores them.
+hipError_t oz_fixup_f64(const OzGemmArgs &a, hipStream_t s);
 hipError_t oz_reconstruct(const OzRecArgs &a, hipStream_t s);
 
 } // namespace sipgpu

@@ -100,7 +100,9 @@ __global__ __launch_bounds__(256, 2) void gemm_f64_packed_kernel(OzGemmArgs a) {
     const int g = (lane & 7) ^ (lane >> 3);
     gi8_t *plane = (gi8_t *)a.planes;
 
-    for (int t = blockIdx.x; t < a.ntiles; t += gridDim.x) {
+    const int u0 = a.wg_first ? a.wg_first[blockIdx.x] : (int)blockIdx.x;
+    const int u1 = a.wg_first ? a.wg_first[blockIdx.x + 1] : min((int)blockIdx.x + 1, a.ntiles);
+    for (int t = u0; t < u1; ++t) {
         const OzTile tile = a.tiles[t];
         if (tile.task < 0) continue;
         const OzTask task = a.tasks[tile.task];
@@ -118,10 +120,11 @@ __global__ __launch_bounds__(256, 2) void gemm_f64_packed_kernel(OzGemmArgs a) {
             for (int j = 0; j < FN; ++j) acc[i][j] = f64x4{0.0, 0.0, 0.0, 0.0};
 
         const OzChunkRef *ch = a.chunks + task.chunk0;
-        const int nst = task.nchunks / CH; // even
+        const int sb = tile.s0, se = tile.s1; // stages of this unit, se > sb
+        // windows of kWin chunk entries start at stage sb + 64 j
         auto load_window = [&](int st0) {
             __syncthreads();
-            const int c0 = st0 * CH, n = min(kWin, task.nchunks - c0);
+            const int c0 = st0 * CH, n = min(kWin, se * CH - c0);
             for (int i = tid; i < n; i += 256)
                 *reinterpret_cast<int32x4 *>(&Es[i]) = *reinterpret_cast<const int32x4 *>(&ch[c0 + i]);
             __syncthreads();
@@ -131,7 +134,7 @@ __global__ __launch_bounds__(256, 2) void gemm_f64_packed_kernel(OzGemmArgs a) {
             int stride;
         };
         auto entry = [&](int st) {
-            const OzChunkRef e = Es[(min(st, nst - 1) * CH + g) % kWin];
+            const OzChunkRef e = Es[((min(st, se - 1) - sb) * CH + g) % kWin];
             return Sel{e.a, e.b, e.stride};
         };
         int32x4 xa[IA], xb[IB];
@@ -179,13 +182,13 @@ __global__ __launch_bounds__(256, 2) void gemm_f64_packed_kernel(OzGemmArgs a) {
         // One stage of loads ahead is enough here: a stage is 64 MFMAs of
         // 64 cycles per wave.  Loads are unconditional (the last stage is
         // fetched twice) so the staged registers need no merge copies.
-        load_window(0);
-        fetch(entry(0));
+        load_window(sb);
+        fetch(entry(sb));
         put(0);
         __syncthreads();
         int buf = 0;
-        for (int s = 0; s < nst; ++s) {
-            if (s + 1 < nst && ((s + 1) * CH) % kWin == 0) load_window(s + 1);
+        for (int s = sb; s < se; ++s) {
+            if (s + 1 < se && ((s + 1 - sb) * CH) % kWin == 0) load_window(s + 1);
             fetch(entry(s + 1));
             __builtin_amdgcn_sched_barrier(0);
             compute(buf);
@@ -194,6 +197,18 @@ __global__ __launch_bounds__(256, 2) void gemm_f64_packed_kernel(OzGemmArgs a) {
             buf ^= 1;
         }
 
+        if (tile.part >= 0) {
+            // one K segment of a split tile: raw accumulators, register order
+            double *w = a.work + (long long)tile.part * (BM * BN);
+#pragma unroll
+            for (int i = 0; i < FM; ++i)
+#pragma unroll
+                for (int j = 0; j < FN; ++j)
+#pragma unroll
+                    for (int r = 0; r < 4; ++r)
+                        w[(((wave * FM + i) * FN + j) * 4 + r) * 64 + lane] = acc[i][j][r];
+            continue;
+        }
         // scatter into the destination block(s)
         if (task.rmap < 0) {
             __attribute__((address_space(1))) double *D =
@@ -240,7 +255,44 @@ __global__ __launch_bounds__(256, 2) void gemm_f64_packed_kernel(OzGemmArgs a) {
     }
 }
 
+// Split tiles: one workgroup per tile sums its segments' partial products
+// in segment order (the same layout the GEMM wrote: thread = wave * 64 +
+// lane holds fragment (i, j, r)) and stores the sum like the GEMM epilogue.
+__global__ __launch_bounds__(256) void fixup_f64_kernel(OzGemmArgs a) {
+    constexpr int BM = 128, BN = 128, WM = 2, TM = 64, TN = 64, FM = 4, FN = 4;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = wave % WM, wn = wave / WM, li = lane & 15, lk = lane >> 4;
+    for (int it = blockIdx.x; it < a.nfix; it += gridDim.x) {
+        const OzTile f = a.fix[it];
+        const OzTask task = a.tasks[f.task];
+        const int m0 = f.mt * BM, n0 = f.nt * BN;
+        const double *w0 = a.work + (long long)f.part * (BM * BN);
+        for (int j = 0; j < FN; ++j) {
+            const int n = n0 + wn * TN + j * 16 + li;
+            for (int i = 0; i < FM; ++i)
+                for (int r = 0; r < 4; ++r) {
+                    const int e = (((wave * FM + i) * FN + j) * 4 + r) * 64 + lane;
+                    double v = w0[e];
+                    for (int q = 1; q < f.s0; ++q) v += w0[(long long)q * (BM * BN) + e];
+                    const int m = m0 + wm * TM + i * 16 + 4 * r + lk;
+                    if (m < task.M && n < task.N) {
+                        int beta;
+                        double *p = oz_dest(task, a.subs, a.tab, m, n, beta);
+                        if (beta) v += *p;
+                        *p = v;
+                    }
+                }
+        }
+    }
+}
+
 } // namespace
+
+hipError_t oz_fixup_f64(const OzGemmArgs &a, hipStream_t s) {
+    if (a.nfix <= 0) return hipSuccess;
+    hipLaunchKernelGGL(fixup_f64_kernel, dim3(a.nfix < 65535 ? a.nfix : 65535), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
 
 hipError_t oz_pack(const OzConvArgs &a, hipStream_t s) {
     if (a.nitems <= 0) return hipSuccess;
@@ -250,7 +302,8 @@ hipError_t oz_pack(const OzConvArgs &a, hipStream_t s) {
 
 hipError_t oz_gemm_f64(const OzGemmArgs &a, hipStream_t s) {
     if (a.ntiles <= 0) return hipSuccess;
-    hipLaunchKernelGGL(gemm_f64_packed_kernel, dim3(a.ntiles), dim3(256), 0, s, a);
+    hipLaunchKernelGGL(gemm_f64_packed_kernel, dim3(a.wg_first ? a.nwg : a.ntiles), dim3(256), 0,
+                       s, a);
     return hipGetLastError();
 }
 

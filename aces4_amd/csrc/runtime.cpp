@@ -20,6 +20,7 @@
 #include <memory>
 #include <tuple>
 #include <mutex>
+#include <queue>
 #include <set>
 #include <string>
 #include <unordered_map>
@@ -270,6 +271,7 @@ struct SplitData {
     long long plane_bytes = 0;
     uint8_t *res_planes = nullptr;
     long long res_plane_bytes = 0;
+    double *work = nullptr; // split-tile partial products
     const OzConvTile *row_items = nullptr;
     int nrow_items = 0;
     OzConvArgs cargs{};
@@ -322,8 +324,8 @@ struct TablePool {
 // panels) of at most that size, so a batch of few large tasks still spans
 // the chip.  Empty slots are task -1.
 template <class Tile>
-std::vector<Tile> xcd_schedule(const std::vector<int> &tm, const std::vector<int> &tn,
-                               const std::vector<double> &cost, int &real_tiles) {
+std::vector<std::vector<Tile>> xcd_lists(const std::vector<int> &tm, const std::vector<int> &tn,
+                                         const std::vector<double> &cost, int &real_tiles) {
     const int ntasks = (int)tm.size();
     long long total = 0;
     for (int t = 0; t < ntasks; ++t) total += (long long)tm[t] * tn[t];
@@ -353,12 +355,147 @@ std::vector<Tile> xcd_schedule(const std::vector<int> &tm, const std::vector<int
         load[x] += u.cost;
         real_tiles += (int)u.count;
     }
+    return per_xcd;
+}
+
+// The launch order that sends list x to XCD x: slot j * kXcds + x.
+template <class Tile>
+std::vector<Tile> interleave(const std::vector<std::vector<Tile>> &per_xcd) {
     size_t per = 0;
     for (auto &v : per_xcd) per = std::max(per, v.size());
-    std::vector<Tile> tiles(per * kXcds, Tile{-1, 0, 0, 0});
+    Tile empty{};
+    empty.task = -1;
+    std::vector<Tile> tiles(per * kXcds, empty);
     for (int x = 0; x < kXcds; ++x)
         for (size_t j = 0; j < per_xcd[x].size(); ++j) tiles[j * kXcds + x] = per_xcd[x][j];
     return tiles;
+}
+
+template <class Tile>
+std::vector<Tile> xcd_schedule(const std::vector<int> &tm, const std::vector<int> &tn,
+                               const std::vector<double> &cost, int &real_tiles) {
+    return interleave(xcd_lists<Tile>(tm, tn, cost, real_tiles));
+}
+
+// Makespan of a list dispatched in order onto `slots` identical workgroup
+// slots (each unit starts on the slot that frees first).
+double list_makespan(const std::vector<double> &cost, int slots) {
+    std::priority_queue<double, std::vector<double>, std::greater<double>> q;
+    for (int i = 0; i < slots; ++i) q.push(0.0);
+    double end = 0.0;
+    for (double c : cost) {
+        const double t = q.top() + c;
+        q.pop();
+        q.push(t);
+        end = std::max(end, t);
+    }
+    return end;
+}
+
+// Persistent schedule of the packed FP64 GEMM: `slots` workgroups per XCD
+// (two per CU), each running its own list of work units.  Per XCD, the
+// tiles of whole rounds go in list order to the least loaded workgroup (the
+// order the hardware would dispatch them in, so concurrent tiles share
+// panels in L2); the last, partly filled round is water-filled: its stages,
+// concatenated in list order, are cut into one contiguous piece per
+// workgroup such that all workgroups finish together.  A tile cut into
+// several K segments takes consecutive workspace slots (one per segment, in
+// K order) and one fixup entry.  Units of workgroup j of XCD x go to
+// workgroup j * kXcds + x, so they run on XCD x.
+void persistent_schedule(const std::vector<std::vector<OzTile>> &lists,
+                         const std::vector<std::vector<double>> &costs, int slots,
+                         std::vector<OzTile> &units, std::vector<int> &first,
+                         std::vector<OzTile> &fix, int &nslots) {
+    std::vector<std::vector<OzTile>> wg((size_t)slots * kXcds);
+    constexpr int kMinSeg = 4; // stages
+    for (int x = 0; x < kXcds; ++x) {
+        const auto &L = lists[x];
+        const auto &c = costs[x];
+        const int n = (int)L.size();
+        const int r = n % slots;
+        const int full = n - r;
+        std::vector<double> load(slots, 0.0);
+        using Slot = std::pair<double, int>;
+        std::priority_queue<Slot, std::vector<Slot>, std::greater<Slot>> q;
+        for (int j = 0; j < slots; ++j) q.push({0.0, j});
+        for (int i = 0; i < full; ++i) {
+            const Slot sl = q.top();
+            q.pop();
+            wg[(size_t)sl.second * kXcds + x].push_back(L[i]);
+            load[sl.second] = sl.first + c[i];
+            q.push({load[sl.second], sl.second});
+        }
+        if (r == 0) continue;
+        // water level F: sum over workgroups of max(0, F - load) = tail work
+        double W = 0.0;
+        for (int i = full; i < n; ++i) W += c[i];
+        std::vector<double> sorted = load;
+        std::sort(sorted.begin(), sorted.end());
+        double F = sorted.back() + W / slots, acc = 0.0;
+        for (int m = 1; m <= slots; ++m) {
+            acc += sorted[m - 1];
+            const double f = (W + acc) / m;
+            if (m == slots || f <= sorted[m]) {
+                F = f;
+                break;
+            }
+        }
+        // cut the tail into pieces, workgroups in index order: piece j ends
+        // at the stage boundary (a multiple of kMinSeg into its tile)
+        // nearest the cumulative target, so rounding does not accumulate
+        struct Seg { int j, s0, s1; };
+        std::vector<std::vector<Seg>> segs(n - full);
+        int i = full, st = L[full].s0;
+        double pos = 0.0, target = 0.0;
+        for (int j = 0; j < slots && i < n; ++j) {
+            target += std::max(0.0, F - load[j]);
+            if (j == slots - 1) target = 1e300; // the last takes the rest
+            while (i < n) {
+                const int nst = L[i].s1 - L[i].s0;
+                const double cs = c[i] / nst; // cost per stage
+                const int rem = L[i].s1 - st;
+                if (pos + rem * cs <= target + 0.5 * kMinSeg * cs) {
+                    segs[i - full].push_back({j, st, L[i].s1});
+                    pos += rem * cs;
+                    if (++i < n) st = L[i].s0;
+                    continue;
+                }
+                int k = (int)std::lround((target - pos) / cs / kMinSeg) * kMinSeg;
+                if (k > 0 && L[i].s1 - (st + k) < kMinSeg) k = rem;
+                if (k > 0) {
+                    segs[i - full].push_back({j, st, st + k});
+                    st += k;
+                    pos += k * cs;
+                    if (st == L[i].s1 && ++i < n) st = L[i].s0;
+                }
+                break;
+            }
+        }
+        for (int t = 0; t < n - full; ++t) {
+            const OzTile &u = L[full + t];
+            const auto &sg = segs[t];
+            if (sg.size() == 1) { // whole tile
+                wg[(size_t)sg[0].j * kXcds + x].push_back(u);
+                continue;
+            }
+            OzTile f = u;
+            f.part = nslots;
+            f.s0 = (int)sg.size();
+            fix.push_back(f);
+            for (const Seg &e : sg) {
+                OzTile v = u;
+                v.s0 = e.s0;
+                v.s1 = e.s1;
+                v.part = nslots++;
+                wg[(size_t)e.j * kXcds + x].push_back(v);
+            }
+        }
+    }
+    first.assign(1, 0);
+    for (const auto &v : wg) {
+        units.insert(units.end(), v.begin(), v.end());
+        first.push_back((int)units.size());
+    }
 }
 
 // No destination may overlap an operand or another destination (partially:
@@ -986,13 +1123,31 @@ int plan_panels_locked(const sipgpu_contraction *list, int n, int nmod, sipgpu_b
         if (nmod && (v == kOzBM || v == kOzBigTile)) tile = v;
     }
     // tiles: the same XCD schedule as the FP64 batch
+    const char *tail_env = std::getenv("SIPGPU_SPLIT_TAIL"); // "0": one tile per workgroup
     std::vector<int> tms(ntasks), tns(ntasks);
     for (int t = 0; t < ntasks; ++t) {
         tms[t] = (tasks[t].M + tile - 1) / tile;
         tns[t] = (tasks[t].N + tile - 1) / tile;
     }
     int real_tiles = 0;
-    const std::vector<OzTile> tiles = xcd_schedule<OzTile>(tms, tns, task_cost, real_tiles);
+    std::vector<std::vector<OzTile>> lists = xcd_lists<OzTile>(tms, tns, task_cost, real_tiles);
+    std::vector<std::vector<double>> costs(kXcds);
+    for (int x = 0; x < kXcds; ++x)
+        for (OzTile &u : lists[x]) {
+            u.part = -1;
+            u.s0 = 0;
+            u.s1 = tasks[u.task].nchunks / kOzStageChunks;
+            costs[x].push_back(task_cost[u.task] / ((double)tms[u.task] * tns[u.task]));
+        }
+    std::vector<OzTile> tiles, fix;
+    std::vector<int> wg_first;
+    int nslots = 0;
+    if (!nmod && !(tail_env && !std::strcmp(tail_env, "0"))) {
+        // the packed FP64 GEMM runs two workgroups per CU, 32 CUs per XCD
+        persistent_schedule(lists, costs, 64, tiles, wg_first, fix, nslots);
+    } else {
+        tiles = interleave(lists);
+    }
     if (tiles.size() >= (size_t)INT_MAX) return fail(SIPGPU_ESIZE, "split path: too many tiles");
 
     OzConst oc{};
@@ -1013,6 +1168,8 @@ int plan_panels_locked(const sipgpu_contraction *list, int n, int nmod, sipgpu_b
     const size_t o_rows = off; off = align(off + rows.size() * sizeof(OzConvTile));
     const size_t o_rec = off; off = align(off + rec.size() * sizeof(OzRecItem));
     const size_t o_subs = off; off = align(off + subs.size() * sizeof(OzSub));
+    const size_t o_fix = off; off = align(off + fix.size() * sizeof(OzTile));
+    const size_t o_wg = off; off = align(off + wg_first.size() * sizeof(int));
     const size_t o_expo = off; off = align(off + (size_t)nexpo * sizeof(int));
     const size_t total = off + 256;
     std::vector<unsigned char> host(total, 0);
@@ -1028,6 +1185,8 @@ int plan_panels_locked(const sipgpu_contraction *list, int n, int nmod, sipgpu_b
     put(o_rows, rows.data(), rows.size() * sizeof(OzConvTile));
     put(o_rec, rec.data(), rec.size() * sizeof(OzRecItem));
     put(o_subs, subs.data(), subs.size() * sizeof(OzSub));
+    put(o_fix, fix.data(), fix.size() * sizeof(OzTile));
+    put(o_wg, wg_first.data(), wg_first.size() * sizeof(int));
     HIP_TRY(hipMalloc(&b.dev, total));
     HIP_TRY(hipMemcpy(b.dev, host.data(), total, hipMemcpyHostToDevice));
     unsigned char *base = static_cast<unsigned char *>(b.dev);
@@ -1068,6 +1227,12 @@ int plan_panels_locked(const sipgpu_contraction *list, int n, int nmod, sipgpu_b
     g.tile = tile;
     g.tab = dtab;
     g.subs = reinterpret_cast<const OzSub *>(base + o_subs);
+    if (nslots) HIP_TRY(hipMalloc(&sd.work, (size_t)nslots * kOzBM * kOzBN * sizeof(double)));
+    g.work = sd.work;
+    g.fix = reinterpret_cast<const OzTile *>(base + o_fix);
+    g.nfix = (int)fix.size();
+    g.wg_first = wg_first.empty() ? nullptr : reinterpret_cast<const int *>(base + o_wg);
+    g.nwg = wg_first.empty() ? 0 : (int)wg_first.size() - 1;
     g.c = oc;
     OzRecArgs &r = sd.rargs;
     r.tasks = dtasks;
@@ -1089,9 +1254,23 @@ int plan_panels_locked(const sipgpu_contraction *list, int n, int nmod, sipgpu_b
     return 0;
 }
 
+void free_batch_memory(sipgpu_batch_s &b) {
+    if (b.dev) (void)hipFree(b.dev);
+    if (b.sd.planes) (void)hipFree(b.sd.planes);
+    if (b.sd.res_planes) (void)hipFree(b.sd.res_planes);
+    if (b.sd.work) (void)hipFree(b.sd.work);
+    b.dev = nullptr;
+    b.sd.planes = nullptr;
+    b.sd.res_planes = nullptr;
+    b.sd.work = nullptr;
+}
+
 int execute_packed(sipgpu_batch_s &b, int stages, hipStream_t s) {
     if (stages & SIPGPU_STAGE_PREPARE) HIP_TRY(oz_pack(b.sd.cargs, s));
-    if (stages & SIPGPU_STAGE_PRODUCT) HIP_TRY(oz_gemm_f64(b.sd.gargs, s));
+    if (stages & SIPGPU_STAGE_PRODUCT) {
+        HIP_TRY(oz_gemm_f64(b.sd.gargs, s));
+        HIP_TRY(oz_fixup_f64(b.sd.gargs, s));
+    }
     return 0;
 }
 
@@ -1421,10 +1600,9 @@ int sipgpu_batch_plan(const sipgpu_contraction *list, int n, sipgpu_batch_t *bat
         auto pb = std::make_unique<sipgpu_batch_s>();
         rc = plan_panels_locked(list, n, 0, *pb);
         if (rc) {
-            if (pb->dev) (void)hipFree(pb->dev);
-            if (pb->sd.planes) (void)hipFree(pb->sd.planes);
+            free_batch_memory(*pb);
         } else {
-            if (b->dev) (void)hipFree(b->dev);
+            free_batch_memory(*b);
             b = std::move(pb);
         }
     }
@@ -1448,9 +1626,7 @@ int sipgpu_batch_destroy(sipgpu_batch_t b) {
     // the work list may still be read by a launch in flight on any stream
     hipError_t e = hipDeviceSynchronize();
     if (e != hipSuccess) return hip_fail(e, "sipgpu_batch_destroy");
-    if (b->dev) (void)hipFree(b->dev);
-    if (b->sd.planes) (void)hipFree(b->sd.planes);
-    if (b->sd.res_planes) (void)hipFree(b->sd.res_planes);
+    free_batch_memory(*b);
     delete b;
     return 0;
 }
@@ -1467,9 +1643,7 @@ int sipgpu_batch_plan_split(const sipgpu_contraction *list, int n, int nmoduli,
     auto b = std::make_unique<sipgpu_batch_s>();
     rc = plan_panels_locked(list, n, nmoduli, *b);
     if (rc) {
-        if (b->dev) (void)hipFree(b->dev);
-        if (b->sd.planes) (void)hipFree(b->sd.planes);
-        if (b->sd.res_planes) (void)hipFree(b->sd.res_planes);
+        free_batch_memory(*b);
         return rc;
     }
     *batch = b.release();

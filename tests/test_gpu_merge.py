@@ -119,3 +119,33 @@ def test_grid_cut_at_row_limit(path, monkeypatch):
     for i in range(nb):
         want = (Lh[i * K * M:(i + 1) * K * M].reshape(K, M, order="F").T @ R).ravel(order="F")
         assert rel_fro(got[i], want) <= TOL[path], i
+
+
+@pytest.mark.parametrize("make", [
+    lambda: wl.ring(o_segs=(9, 6), v_segs=(14, 11, 8)),
+    lambda: wl.ring(o_segs=(15, 15), v_segs=(30, 30, 15)),
+], ids=["ring-small", "ring-mid"])
+def test_persistent_split_tiles_vs_whole_tiles(make, monkeypatch):
+    """The packed GEMM's persistent schedule cuts its last round of tiles
+    into K segments summed by the fixup pass; the result matches the oracle
+    and the one-tile-per-workgroup launch (SIPGPU_SPLIT_TAIL=0)."""
+    monkeypatch.setenv("SIPGPU_FP64_PATH", "packed")
+    w = make()
+    rng = np.random.default_rng(8)
+    w["D"].host[:] = rng.standard_normal(w["D"].host.size)
+    want = oracle_batch(w)
+    out = {}
+    for tail in ("1", "0"):
+        monkeypatch.setenv("SIPGPU_SPLIT_TAIL", tail)
+        w["D"].dev = None
+        b = wl.batch_for(w)
+        for _ in range(2):  # a second run reuses the workspace
+            w["D"].dev.upload(w["D"].host)
+            b.execute()
+            sg.synchronize()
+        got = w["D"].dev.download()
+        assert per_block_err(w, got, want) <= TOL["packed"]
+        out[tail] = got
+        w["D"].dev.free()
+        b.destroy()
+    assert rel_fro(out["1"], out["0"]) <= 2 * TOL["packed"]
