@@ -8,6 +8,7 @@
 
 #include "../../include/sipgpu.h"
 #include "kernels.h"
+#include "ozaki.h"
 
 // A library stream (the HIP stream and its contract-to-scalar scratch).
 struct sipgpu_stream_s {
@@ -48,6 +49,49 @@ struct BatchImage {
 
 // Plans a list of block contractions (same rules as sipgpu_batch_plan).
 int build_batch_image(const sipgpu_contraction *list, int n, BatchImage &img);
+
+// Device state of a panel batch (packed FP64 or split precision).
+struct SplitData {
+    OzConst c{};
+    int *expo = nullptr;
+    int nexpo = 0;
+    int8_t *planes = nullptr;
+    long long plane_bytes = 0;
+    uint8_t *res_planes = nullptr;
+    long long res_plane_bytes = 0;
+    double *work = nullptr; // split-tile partial products
+    const OzConvTile *row_items = nullptr;
+    int nrow_items = 0;
+    OzConvArgs cargs{};
+    OzGemmArgs gargs{};
+    OzRecArgs rargs{};
+};
+
+// Host plan of a panel batch: its descriptor image (uploaded as is) and the
+// sizes of the device memory it runs in.  nmod > 0: split precision over
+// nmod moduli; 0: packed FP64.
+struct PanelPlan {
+    int nmod = 0;
+    std::vector<unsigned char> host;
+    size_t o_blocks = 0, o_tasks = 0, o_tiles = 0, o_chunks = 0, o_tab = 0, o_conv = 0,
+           o_rows = 0, o_rec = 0, o_subs = 0, o_fix = 0, o_wg = 0, o_expo = 0;
+    long long plane_bytes = 0, res_plane_bytes = 0;
+    int nslots = 0; // workspace slots of kOzBM * kOzBN doubles
+    OzConst c{};
+    int tile = 0, ntiles = 0, nconv = 0, nrows = 0, nrec = 0, nfix = 0, nwg = 0, nexpo = 0;
+    int ndest = 0, ngroups = 0, npairs = 0, real_tiles = 0;
+    double flops = 0.0;
+    size_t plane_total() const { return (size_t)plane_bytes * (nmod ? nmod : 1); }
+    size_t res_total() const { return (size_t)res_plane_bytes * nmod; }
+    size_t work_total() const { return (size_t)nslots * kOzBM * kOzBN * sizeof(double); }
+};
+
+int build_panel_plan(const sipgpu_contraction *list, int n, int nmod, PanelPlan &p);
+// Points the kernels' arguments at an uploaded image and its memory.
+void bind_panel_plan(const PanelPlan &p, void *desc, int8_t *planes, uint8_t *res_planes,
+                     double *work, SplitData &sd);
+// Enqueues the stages (SIPGPU_STAGE_* bits) of a bound panel batch.
+int execute_panels(SplitData &sd, bool split, int stages, hipStream_t s);
 
 } // namespace rt
 } // namespace sipgpu

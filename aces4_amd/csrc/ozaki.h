@@ -157,6 +157,8 @@ struct OzRecArgs {
 };
 
 hipError_t oz_fill_int(int *p, int n, int v, hipStream_t s);
+// Zeroes the kOzZero bytes at the start of each of `nplanes` planes.
+hipError_t oz_zero_heads(int8_t *planes, long long plane_bytes, int nplanes, hipStream_t s);
 // Row maxima -> exponents, one workgroup per kOzTileX-row tile of a block
 // (`row_items`, all with k0 = 0); each sweeps the block's whole K.
 hipError_t oz_row_exponents(const OzConvArgs &a, const OzConvTile *row_items, int nrow_items,

@@ -37,6 +37,15 @@ __global__ void fill_int_kernel(int *p, int n, int v) {
         p[i] = v;
 }
 
+// Zeroes the kOzZero bytes at the start of each plane (16 lanes x 16 bytes).
+__global__ void zero_heads_kernel(int8_t *planes, long long plane_bytes, int nplanes) {
+    static_assert(kOzZero == 256, "16 lanes of 16 bytes per plane");
+    const int p = blockIdx.x * 4 + (threadIdx.x >> 4);
+    if (p < nplanes)
+        *reinterpret_cast<int4 *>(planes + p * plane_bytes + (threadIdx.x & 15) * 16) =
+            int4{0, 0, 0, 0};
+}
+
 // Rows [x0, x0 + 32) x k [k0, k0 + 128) of block b: each thread loads 16
 // values along the index that is adjacent in memory (32 lanes on 32
 // consecutive rows, or 128 on consecutive k); entries outside the block read
@@ -480,6 +489,13 @@ hipError_t oz_fill_int(int *p, int n, int v, hipStream_t s) {
     if (n <= 0) return hipSuccess;
     hipLaunchKernelGGL(fill_int_kernel, dim3(grid_for((n + 255) / 256, 4096)), dim3(256), 0, s, p,
                        n, v);
+    return hipGetLastError();
+}
+
+hipError_t oz_zero_heads(int8_t *planes, long long plane_bytes, int nplanes, hipStream_t s) {
+    if (nplanes <= 0) return hipSuccess;
+    hipLaunchKernelGGL(zero_heads_kernel, dim3((nplanes + 3) / 4), dim3(64), 0, s, planes,
+                       plane_bytes, nplanes);
     return hipGetLastError();
 }
 

@@ -262,22 +262,8 @@ int64_t prod(int n, const int *dims) {
 // Batches
 // ---------------------------------------------------------------------------
 
-// Device state of a split-precision (INT8 / CRT) batch.
-struct SplitData {
-    OzConst c{};
-    int *expo = nullptr;
-    int nexpo = 0;
-    int8_t *planes = nullptr;
-    long long plane_bytes = 0;
-    uint8_t *res_planes = nullptr;
-    long long res_plane_bytes = 0;
-    double *work = nullptr; // split-tile partial products
-    const OzConvTile *row_items = nullptr;
-    int nrow_items = 0;
-    OzConvArgs cargs{};
-    OzGemmArgs gargs{};
-    OzRecArgs rargs{};
-};
+using sipgpu::rt::SplitData;
+using sipgpu::rt::PanelPlan;
 
 struct sipgpu_batch_s {
     int cfg = kGemm64x64;
@@ -756,7 +742,9 @@ int oz_constants(int n, long long kmax, OzConst &c) {
 // Panel planner: nmod > 0 plans the split-precision path (1-byte residue
 // panels, nmod moduli); nmod == 0 the packed FP64 path (panels of doubles,
 // products scattered straight into the destinations).
-int plan_panels_locked(const sipgpu_contraction *list, int n, int nmod, sipgpu_batch_s &b) {
+} // namespace
+
+int sipgpu::rt::build_panel_plan(const sipgpu_contraction *list, int n, int nmod, PanelPlan &P) {
     if (nmod != 0 && (nmod < kOzMinModuli || nmod > kOzMaxUsable))
         return fail(SIPGPU_EARG, "split path: nmoduli must be in 6..14");
     const int eb = nmod ? 1 : 8; // bytes per panel element
@@ -829,7 +817,7 @@ int plan_panels_locked(const sipgpu_contraction *list, int n, int nmod, sipgpu_b
     std::vector<Dest> dests;
     long long kmax = 0;
     int npairs = 0;
-    b.flops = 0.0;
+    P.flops = 0.0;
     for (const double *dp : dest_order) {
         const auto &idx = by_dest[dp];
         size_t start = 0;
@@ -865,7 +853,7 @@ int plan_panels_locked(const sipgpu_contraction *list, int n, int nmod, sipgpu_b
             return fail(SIPGPU_ESIZE, "split path: result block too large");
         npairs += (int)d.a.size();
         kmax = std::max(kmax, ktot);
-        b.flops += d.cost;
+        P.flops += d.cost;
         dests.push_back(std::move(d));
     }
     const int ndest = (int)dests.size();
@@ -1156,101 +1144,158 @@ int plan_panels_locked(const sipgpu_contraction *list, int n, int nmod, sipgpu_b
         if (P < 8) return fail(SIPGPU_EARG, "split path: too few moduli for this K");
     }
 
-    // one descriptor allocation
+    // the descriptor image
     auto align = [](size_t v) { return (v + 255) & ~(size_t)255; };
     size_t off = 0;
-    const size_t o_blocks = off; off = align(off + blocks.size() * sizeof(OzBlock));
-    const size_t o_tasks = off; off = align(off + tasks.size() * sizeof(OzTask));
-    const size_t o_tiles = off; off = align(off + tiles.size() * sizeof(OzTile));
-    const size_t o_chunks = off; off = align(off + chunks.size() * sizeof(OzChunkRef));
-    const size_t o_tab = off; off = align(off + pool.size() * sizeof(int32_t));
-    const size_t o_conv = off; off = align(off + conv.size() * sizeof(OzConvTile));
-    const size_t o_rows = off; off = align(off + rows.size() * sizeof(OzConvTile));
-    const size_t o_rec = off; off = align(off + rec.size() * sizeof(OzRecItem));
-    const size_t o_subs = off; off = align(off + subs.size() * sizeof(OzSub));
-    const size_t o_fix = off; off = align(off + fix.size() * sizeof(OzTile));
-    const size_t o_wg = off; off = align(off + wg_first.size() * sizeof(int));
-    const size_t o_expo = off; off = align(off + (size_t)nexpo * sizeof(int));
-    const size_t total = off + 256;
-    std::vector<unsigned char> host(total, 0);
+    P.o_blocks = off; off = align(off + blocks.size() * sizeof(OzBlock));
+    P.o_tasks = off; off = align(off + tasks.size() * sizeof(OzTask));
+    P.o_tiles = off; off = align(off + tiles.size() * sizeof(OzTile));
+    P.o_chunks = off; off = align(off + chunks.size() * sizeof(OzChunkRef));
+    P.o_tab = off; off = align(off + pool.size() * sizeof(int32_t));
+    P.o_conv = off; off = align(off + conv.size() * sizeof(OzConvTile));
+    P.o_rows = off; off = align(off + rows.size() * sizeof(OzConvTile));
+    P.o_rec = off; off = align(off + rec.size() * sizeof(OzRecItem));
+    P.o_subs = off; off = align(off + subs.size() * sizeof(OzSub));
+    P.o_fix = off; off = align(off + fix.size() * sizeof(OzTile));
+    P.o_wg = off; off = align(off + wg_first.size() * sizeof(int));
+    P.o_expo = off; off = align(off + (size_t)nexpo * sizeof(int));
+    P.host.assign(off + 256, 0);
     auto put = [&](size_t o, const void *src, size_t bytes) {
-        if (bytes) std::memcpy(host.data() + o, src, bytes);
+        if (bytes) std::memcpy(P.host.data() + o, src, bytes);
     };
-    put(o_blocks, blocks.data(), blocks.size() * sizeof(OzBlock));
-    put(o_tasks, tasks.data(), tasks.size() * sizeof(OzTask));
-    put(o_tiles, tiles.data(), tiles.size() * sizeof(OzTile));
-    put(o_chunks, chunks.data(), chunks.size() * sizeof(OzChunkRef));
-    put(o_tab, pool.data(), pool.size() * sizeof(int32_t));
-    put(o_conv, conv.data(), conv.size() * sizeof(OzConvTile));
-    put(o_rows, rows.data(), rows.size() * sizeof(OzConvTile));
-    put(o_rec, rec.data(), rec.size() * sizeof(OzRecItem));
-    put(o_subs, subs.data(), subs.size() * sizeof(OzSub));
-    put(o_fix, fix.data(), fix.size() * sizeof(OzTile));
-    put(o_wg, wg_first.data(), wg_first.size() * sizeof(int));
-    HIP_TRY(hipMalloc(&b.dev, total));
-    HIP_TRY(hipMemcpy(b.dev, host.data(), total, hipMemcpyHostToDevice));
-    unsigned char *base = static_cast<unsigned char *>(b.dev);
-    SplitData &sd = b.sd;
-    sd.c = oc;
-    sd.plane_bytes = (panel + 255) / 256 * 256;
-    sd.res_plane_bytes = (res_elems + 16 + 255) / 256 * 256; // a word past the last region
-    const int nplanes = nmod ? nmod : 1;
-    HIP_TRY(hipMalloc(&sd.planes, (size_t)sd.plane_bytes * nplanes));
-    // the zero bytes at the start of each plane are never written again
-    HIP_TRY(hipMemset(sd.planes, 0, (size_t)sd.plane_bytes * nplanes));
-    HIP_TRY(hipDeviceSynchronize());
-    if (nmod) HIP_TRY(hipMalloc(&sd.res_planes, (size_t)sd.res_plane_bytes * nmod));
-    sd.expo = reinterpret_cast<int *>(base + o_expo);
-    sd.nexpo = nexpo;
-    sd.row_items = reinterpret_cast<const OzConvTile *>(base + o_rows);
-    sd.nrow_items = (int)rows.size();
-    const int32_t *dtab = reinterpret_cast<const int32_t *>(base + o_tab);
-    const OzTask *dtasks = reinterpret_cast<const OzTask *>(base + o_tasks);
+    put(P.o_blocks, blocks.data(), blocks.size() * sizeof(OzBlock));
+    put(P.o_tasks, tasks.data(), tasks.size() * sizeof(OzTask));
+    put(P.o_tiles, tiles.data(), tiles.size() * sizeof(OzTile));
+    put(P.o_chunks, chunks.data(), chunks.size() * sizeof(OzChunkRef));
+    put(P.o_tab, pool.data(), pool.size() * sizeof(int32_t));
+    put(P.o_conv, conv.data(), conv.size() * sizeof(OzConvTile));
+    put(P.o_rows, rows.data(), rows.size() * sizeof(OzConvTile));
+    put(P.o_rec, rec.data(), rec.size() * sizeof(OzRecItem));
+    put(P.o_subs, subs.data(), subs.size() * sizeof(OzSub));
+    put(P.o_fix, fix.data(), fix.size() * sizeof(OzTile));
+    put(P.o_wg, wg_first.data(), wg_first.size() * sizeof(int));
+    P.nmod = nmod;
+    P.c = oc;
+    P.plane_bytes = (panel + 255) / 256 * 256;
+    P.res_plane_bytes = nmod ? (res_elems + 16 + 255) / 256 * 256 : 0; // a word past the last region
+    P.nslots = nslots;
+    P.tile = tile;
+    P.ntiles = (int)tiles.size();
+    P.nconv = (int)conv.size();
+    P.nrows = (int)rows.size();
+    P.nrec = (int)rec.size();
+    P.nfix = (int)fix.size();
+    P.nwg = wg_first.empty() ? 0 : (int)wg_first.size() - 1;
+    P.nexpo = nexpo;
+    P.ndest = ndest;
+    P.ngroups = ntasks;
+    P.npairs = npairs;
+    P.real_tiles = real_tiles;
+    return 0;
+}
+
+void sipgpu::rt::bind_panel_plan(const PanelPlan &P, void *desc, int8_t *planes,
+                                 uint8_t *res_planes, double *work, SplitData &sd) {
+    unsigned char *base = static_cast<unsigned char *>(desc);
+    sd.c = P.c;
+    sd.planes = planes;
+    sd.plane_bytes = P.plane_bytes;
+    sd.res_planes = res_planes;
+    sd.res_plane_bytes = P.res_plane_bytes;
+    sd.work = work;
+    sd.expo = reinterpret_cast<int *>(base + P.o_expo);
+    sd.nexpo = P.nexpo;
+    sd.row_items = reinterpret_cast<const OzConvTile *>(base + P.o_rows);
+    sd.nrow_items = P.nrows;
+    const int32_t *dtab = reinterpret_cast<const int32_t *>(base + P.o_tab);
+    const OzTask *dtasks = reinterpret_cast<const OzTask *>(base + P.o_tasks);
+    const OzSub *dsubs = reinterpret_cast<const OzSub *>(base + P.o_subs);
     OzConvArgs &ca = sd.cargs;
-    ca.blocks = reinterpret_cast<const OzBlock *>(base + o_blocks);
-    ca.items = reinterpret_cast<const OzConvTile *>(base + o_conv);
-    ca.nitems = (int)conv.size();
+    ca = OzConvArgs{};
+    ca.blocks = reinterpret_cast<const OzBlock *>(base + P.o_blocks);
+    ca.items = reinterpret_cast<const OzConvTile *>(base + P.o_conv);
+    ca.nitems = P.nconv;
     ca.tab = dtab;
     ca.expo = sd.expo;
-    ca.planes = sd.planes;
-    ca.plane_bytes = sd.plane_bytes;
-    ca.c = oc;
+    ca.planes = planes;
+    ca.plane_bytes = P.plane_bytes;
+    ca.c = P.c;
     OzGemmArgs &g = sd.gargs;
-    g.tiles = reinterpret_cast<const OzTile *>(base + o_tiles);
+    g = OzGemmArgs{};
+    g.tiles = reinterpret_cast<const OzTile *>(base + P.o_tiles);
     g.tasks = dtasks;
-    g.chunks = reinterpret_cast<const OzChunkRef *>(base + o_chunks);
-    g.planes = sd.planes;
-    g.plane_bytes = sd.plane_bytes;
-    g.res_planes = sd.res_planes;
-    g.res_plane_bytes = sd.res_plane_bytes;
-    g.ntiles = (int)tiles.size();
-    g.tile = tile;
+    g.chunks = reinterpret_cast<const OzChunkRef *>(base + P.o_chunks);
+    g.planes = planes;
+    g.plane_bytes = P.plane_bytes;
+    g.res_planes = res_planes;
+    g.res_plane_bytes = P.res_plane_bytes;
+    g.ntiles = P.ntiles;
+    g.tile = P.tile;
     g.tab = dtab;
-    g.subs = reinterpret_cast<const OzSub *>(base + o_subs);
-    if (nslots) HIP_TRY(hipMalloc(&sd.work, (size_t)nslots * kOzBM * kOzBN * sizeof(double)));
-    g.work = sd.work;
-    g.fix = reinterpret_cast<const OzTile *>(base + o_fix);
-    g.nfix = (int)fix.size();
-    g.wg_first = wg_first.empty() ? nullptr : reinterpret_cast<const int *>(base + o_wg);
-    g.nwg = wg_first.empty() ? 0 : (int)wg_first.size() - 1;
-    g.c = oc;
+    g.subs = dsubs;
+    g.work = work;
+    g.fix = reinterpret_cast<const OzTile *>(base + P.o_fix);
+    g.nfix = P.nfix;
+    g.wg_first = P.nwg ? reinterpret_cast<const int *>(base + P.o_wg) : nullptr;
+    g.nwg = P.nwg;
+    g.c = P.c;
     OzRecArgs &r = sd.rargs;
+    r = OzRecArgs{};
     r.tasks = dtasks;
-    r.items = reinterpret_cast<const OzRecItem *>(base + o_rec);
-    r.nitems = (int)rec.size();
+    r.items = reinterpret_cast<const OzRecItem *>(base + P.o_rec);
+    r.nitems = P.nrec;
     r.tab = dtab;
-    r.subs = g.subs;
+    r.subs = dsubs;
     r.expo = sd.expo;
-    r.res_planes = sd.res_planes;
-    r.res_plane_bytes = sd.res_plane_bytes;
-    r.c = oc;
+    r.res_planes = res_planes;
+    r.res_plane_bytes = P.res_plane_bytes;
+    r.c = P.c;
+}
+
+int sipgpu::rt::execute_panels(SplitData &sd, bool split, int stages, hipStream_t s) {
+    if (!split) {
+        if (stages & SIPGPU_STAGE_PREPARE) HIP_TRY(oz_pack(sd.cargs, s));
+        if (stages & SIPGPU_STAGE_PRODUCT) {
+            HIP_TRY(oz_gemm_f64(sd.gargs, s));
+            HIP_TRY(oz_fixup_f64(sd.gargs, s));
+        }
+        return 0;
+    }
+    if (stages & SIPGPU_STAGE_PREPARE) {
+        HIP_TRY(oz_fill_int(sd.expo, sd.nexpo, INT_MIN, s));
+        HIP_TRY(oz_row_exponents(sd.cargs, sd.row_items, sd.nrow_items, s));
+        HIP_TRY(oz_to_residues(sd.cargs, s));
+    }
+    if (stages & SIPGPU_STAGE_PRODUCT) HIP_TRY(oz_gemm(sd.gargs, s));
+    if (stages & SIPGPU_STAGE_FINISH) HIP_TRY(oz_reconstruct(sd.rargs, s));
+    return 0;
+}
+
+namespace {
+
+// A panel batch with memory of its own: descriptors, planes (zeroed once:
+// the zero bytes at the start of each plane are never written again),
+// result planes and the split-tile workspace.
+int plan_panels_locked(const sipgpu_contraction *list, int n, int nmod, sipgpu_batch_s &b) {
+    PanelPlan P;
+    if (int rc = sipgpu::rt::build_panel_plan(list, n, nmod, P)) return rc;
+    SplitData &sd = b.sd;
+    HIP_TRY(hipMalloc(&b.dev, P.host.size()));
+    HIP_TRY(hipMemcpy(b.dev, P.host.data(), P.host.size(), hipMemcpyHostToDevice));
+    HIP_TRY(hipMalloc(&sd.planes, P.plane_total()));
+    HIP_TRY(hipMemset(sd.planes, 0, P.plane_total()));
+    HIP_TRY(hipDeviceSynchronize());
+    if (P.res_total()) HIP_TRY(hipMalloc(&sd.res_planes, P.res_total()));
+    if (P.work_total()) HIP_TRY(hipMalloc(&sd.work, P.work_total()));
+    sipgpu::rt::bind_panel_plan(P, b.dev, sd.planes, sd.res_planes, sd.work, sd);
     b.split = nmod > 0;
     b.packed = nmod == 0;
-    b.ntiles = (int)tiles.size();
-    b.ntasks = ndest;
-    b.ngroups = ntasks;
-    b.npairs = npairs;
-    b.real_tiles = real_tiles;
+    b.ntiles = P.ntiles;
+    b.ntasks = P.ndest;
+    b.ngroups = P.ngroups;
+    b.npairs = P.npairs;
+    b.real_tiles = P.real_tiles;
+    b.flops = P.flops;
     return 0;
 }
 
@@ -1265,32 +1310,10 @@ void free_batch_memory(sipgpu_batch_s &b) {
     b.sd.work = nullptr;
 }
 
-int execute_packed(sipgpu_batch_s &b, int stages, hipStream_t s) {
-    if (stages & SIPGPU_STAGE_PREPARE) HIP_TRY(oz_pack(b.sd.cargs, s));
-    if (stages & SIPGPU_STAGE_PRODUCT) {
-        HIP_TRY(oz_gemm_f64(b.sd.gargs, s));
-        HIP_TRY(oz_fixup_f64(b.sd.gargs, s));
-    }
-    return 0;
-}
-
-int execute_split(sipgpu_batch_s &b, int stages, hipStream_t s) {
-    SplitData &sd = b.sd;
-    if (stages & SIPGPU_STAGE_PREPARE) {
-        HIP_TRY(oz_fill_int(sd.expo, sd.nexpo, INT_MIN, s));
-        HIP_TRY(oz_row_exponents(sd.cargs, sd.row_items, sd.nrow_items, s));
-        HIP_TRY(oz_to_residues(sd.cargs, s));
-    }
-    if (stages & SIPGPU_STAGE_PRODUCT) HIP_TRY(oz_gemm(sd.gargs, s));
-    if (stages & SIPGPU_STAGE_FINISH) HIP_TRY(oz_reconstruct(sd.rargs, s));
-    return 0;
-}
-
 int execute_batch(Context &c, sipgpu_batch_s &b, int stages, sipgpu_stream_t s) {
     if (b.ntiles == 0) return 0;
     const hipStream_t hs = resolve(c, s).s;
-    if (b.split) return execute_split(b, stages, hs);
-    if (b.packed) return execute_packed(b, stages, hs);
+    if (b.split || b.packed) return sipgpu::rt::execute_panels(b.sd, b.split, stages, hs);
     if (stages & SIPGPU_STAGE_PRODUCT) HIP_TRY(launch_gemm(b.cfg, b.args, b.ntiles, hs));
     return 0;
 }

@@ -446,9 +446,61 @@ struct sipgpu_server_s {
 
     std::thread sched, compl_;
     sipgpu_server_stats st{};
+
+    // split-precision batches: planes, result planes and workspace
+    void *scratch = nullptr;
+    size_t scratch_bytes = 0;
 };
 
 namespace {
+
+// A descriptor slot whose previous batch has finished reading it.
+int take_slot(sipgpu_server_s &S, int &slot) {
+    slot = S.next_slot;
+    S.next_slot = (S.next_slot + 1) % kSlots;
+    if (S.slot_used[slot]) {
+        hipError_t e = hipEventSynchronize(S.slot_free[slot]);
+        if (e != hipSuccess) return hip_fail(e, "server: slot");
+    }
+    return 0;
+}
+
+// A split-precision batch: its descriptors through a slot, its planes in the
+// scratch buffer.  Batches run in stream order, so one scratch buffer serves
+// them all; it grows (after the stream drains) when a batch needs more.
+int server_split(sipgpu_server_s &S, const sipgpu::rt::PanelPlan &plan,
+                 const std::vector<unsigned char> &desc) {
+    if (plan.ntiles == 0) return 0;
+    auto up = [](size_t v) { return (v + 255) & ~(size_t)255; };
+    const size_t need = up(plan.plane_total()) + up(plan.res_total()) + up(plan.work_total());
+    if (need > S.scratch_bytes) {
+        SIPGPU_HIP_TRY(hipStreamSynchronize(S.stream));
+        if (S.scratch) SIPGPU_HIP_TRY(hipFree(S.scratch));
+        S.scratch = nullptr;
+        S.scratch_bytes = 0;
+        const size_t bytes = std::max(need, (size_t)256 << 20);
+        SIPGPU_HIP_TRY(hipMalloc(&S.scratch, bytes));
+        S.scratch_bytes = bytes;
+    }
+    int slot = 0;
+    if (int rc = take_slot(S, slot)) return rc;
+    int8_t *planes = static_cast<int8_t *>(S.scratch);
+    uint8_t *res = reinterpret_cast<uint8_t *>(planes + up(plan.plane_total()));
+    double *work = plan.nslots ? reinterpret_cast<double *>(res + up(plan.res_total())) : nullptr;
+    std::memcpy(S.slot_host[slot], desc.data(), desc.size());
+    hipError_t e = hipMemcpyAsync(S.slot_dev[slot], S.slot_host[slot], desc.size(),
+                                  hipMemcpyHostToDevice, S.stream);
+    S.slot_used[slot] = true;
+    if (e == hipSuccess)
+        e = sipgpu::oz_zero_heads(planes, plan.plane_bytes, plan.nmod ? plan.nmod : 1, S.stream);
+    if (e != hipSuccess) return hip_fail(e, "server: split batch");
+    sipgpu::rt::SplitData sd;
+    sipgpu::rt::bind_panel_plan(plan, S.slot_dev[slot], planes, res, work, sd);
+    int rc = sipgpu::rt::execute_panels(sd, plan.nmod > 0, SIPGPU_STAGE_ALL, S.stream);
+    e = hipEventRecord(S.slot_free[slot], S.stream);
+    if (!rc && e != hipSuccess) rc = hip_fail(e, "server: split batch");
+    return rc;
+}
 
 int server_batch(sipgpu_server_s &S, std::vector<Pending> &batch) {
     // blocks of the batch, each acquired once: destinations for write
@@ -477,6 +529,8 @@ int server_batch(sipgpu_server_s &S, std::vector<Pending> &batch) {
         if (rc) break;
     }
     sipgpu::rt::BatchImage img;
+    sipgpu::rt::PanelPlan plan;
+    const int nmod = batch.empty() ? 0 : batch[0].req.moduli;
     if (!rc) {
         std::vector<sipgpu_contraction> list(batch.size());
         for (size_t i = 0; i < batch.size(); ++i) {
@@ -494,17 +548,18 @@ int server_batch(sipgpu_server_s &S, std::vector<Pending> &batch) {
             std::memcpy(e.ddims, r.ddims, sizeof(e.ddims));
             e.accumulate = r.accumulate;
         }
-        rc = sipgpu::rt::build_batch_image(list.data(), (int)list.size(), img);
+        if (nmod)
+            rc = sipgpu::rt::build_panel_plan(list.data(), (int)list.size(), nmod, plan);
+        else
+            rc = sipgpu::rt::build_batch_image(list.data(), (int)list.size(), img);
     }
-    if (!rc && img.host.size() > kSlotBytes)
+    const std::vector<unsigned char> &desc = nmod ? plan.host : img.host;
+    if (!rc && desc.size() > kSlotBytes)
         rc = fail(SIPGPU_ESIZE, "server: batch work list exceeds its descriptor slot");
-    if (!rc && img.ntiles > 0) {
-        const int slot = S.next_slot;
-        S.next_slot = (S.next_slot + 1) % kSlots;
-        if (S.slot_used[slot]) {
-            hipError_t e = hipEventSynchronize(S.slot_free[slot]);
-            if (e != hipSuccess) rc = hip_fail(e, "server: slot");
-        }
+    if (!rc && nmod) rc = server_split(S, plan, desc);
+    if (!rc && !nmod && img.ntiles > 0) {
+        int slot = 0;
+        rc = take_slot(S, slot);
         if (!rc) {
             std::memcpy(S.slot_host[slot], img.host.data(), img.host.size());
             hipError_t e = hipMemcpyAsync(S.slot_dev[slot], S.slot_host[slot], img.host.size(),
@@ -524,7 +579,7 @@ int server_batch(sipgpu_server_s &S, std::vector<Pending> &batch) {
     if (!rc) {
         S.st.batches += 1;
         S.st.requests += batch.size();
-        S.st.flops += img.flops;
+        S.st.flops += nmod ? plan.flops : img.flops;
         S.st.max_batch = std::max<unsigned long long>(S.st.max_batch, batch.size());
     }
     return rc;
@@ -546,6 +601,8 @@ void scheduler_loop(sipgpu_server_s *S) {
                 const sipgpu_request &r = S->queue.front().req;
                 // keep program order: no block both written and read in one batch
                 if (writes.count(r.lkey) || writes.count(r.rkey) || reads.count(r.dkey)) break;
+                // one precision per batch
+                if (!batch.empty() && r.moduli != batch[0].req.moduli) break;
                 size_t add = 0;
                 auto bytes_of = [](int rank, const int *dims) {
                     size_t n = 8;
@@ -619,6 +676,8 @@ int check_request(const sipgpu_request &r) {
         return fail(SIPGPU_ERANK, "server: requests are block = block * block, rank 1..6");
     if (r.dkey == r.lkey || r.dkey == r.rkey)
         return fail(SIPGPU_EHAZARD, "server: a request writes a block it reads");
+    if (r.moduli != 0 && (r.moduli < 6 || r.moduli > 14))
+        return fail(SIPGPU_EARG, "server: moduli must be 0 (FP64) or 6..14");
     return 0;
 }
 
@@ -733,6 +792,7 @@ int sipgpu_server_destroy(sipgpu_server_t S) {
         (void)hipEventDestroy(S->slot_free[i]);
     }
     for (hipEvent_t e : S->event_pool) (void)hipEventDestroy(e);
+    if (S->scratch) (void)hipFree(S->scratch);
     (void)hipStreamDestroy(S->stream);
     delete S;
     return 0;

@@ -73,7 +73,7 @@ class Request(ctypes.Structure):
                 ("ldims", ctypes.c_int * MAX_RANK), ("rdims", ctypes.c_int * MAX_RANK),
                 ("ddims", ctypes.c_int * MAX_RANK), ("lkey", ctypes.c_ulonglong),
                 ("rkey", ctypes.c_ulonglong), ("dkey", ctypes.c_ulonglong),
-                ("accumulate", ctypes.c_int)]
+                ("accumulate", ctypes.c_int), ("moduli", ctypes.c_int)]
 
 
 class CacheStats(ctypes.Structure):
@@ -568,8 +568,10 @@ class Cache:
             self.handle = None
 
 
-def make_request(ptrn, lkey, ldims, rkey, rdims, dkey, ddims, accumulate=False):
+def make_request(ptrn, lkey, ldims, rkey, rdims, dkey, ddims, accumulate=False, moduli=0):
+    """A server request; `moduli` > 0 runs it on the split-precision path."""
     r = Request()
+    r.moduli = int(moduli)
     r.lkey, r.rkey, r.dkey = lkey, rkey, dkey
     r.lrank, r.rrank, r.drank = len(ldims), len(rdims), len(ddims)
     for i, v in enumerate(ldims):

@@ -319,7 +319,7 @@ SERVE_ARRAYS = {
 
 
 def serving_mix(n_requests=3000, rate=3000.0, seed=4, v_nseg=8, o_nseg=4, seg_lo=8, seg_hi=64,
-                mix=(("ladder", 0.4), ("ring", 0.4), ("hole", 0.2))):
+                mix=(("ladder", 0.4), ("ring", 0.4), ("hole", 0.2)), moduli=0):
     """A stream of single block contractions for the serving path.
 
     Segment extents are uniform in seg_lo..seg_hi (v_nseg virtual and o_nseg
@@ -327,7 +327,8 @@ def serving_mix(n_requests=3000, rate=3000.0, seed=4, v_nseg=8, o_nseg=4, seg_lo
     with the given probabilities and uniform segment indices, and accumulates
     one block pair into its result block.  Arrivals are Poisson with `rate`
     requests per second.  Blocks are named by 64-bit keys (array id << 48 |
-    block index) and generated in HBM on first use.
+    block index) and generated in HBM on first use.  `moduli` > 0 runs every
+    request on the split-precision path.
     """
     rng = np.random.default_rng(seed)
     segs = {"v": [int(x) for x in rng.integers(seg_lo, seg_hi + 1, v_nseg)],
@@ -367,7 +368,8 @@ def serving_mix(n_requests=3000, rate=3000.0, seed=4, v_nseg=8, o_nseg=4, seg_lo
         for name, blk in ((ln, lb), (rn, rb), (dn, db)):
             blocks[key(name, blk)] = (name, blk, int(np.prod(dims(name, blk))))
         reqs.append(sg.make_request(ptrns[t], key(ln, lb), dims(ln, lb), key(rn, rb),
-                                    dims(rn, rb), key(dn, db), dims(dn, db), accumulate=True))
+                                    dims(rn, rb), key(dn, db), dims(dn, db), accumulate=True,
+                                    moduli=moduli))
         ext = {lab: segs[kind_of[lab]][idx[lab]] for lab in idx}
         mn = float(np.prod([ext[x] for x in dl]))
         kk = float(np.prod([ext[x] for x in ll if x not in dl]))

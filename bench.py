@@ -252,12 +252,13 @@ def sparse_bench(reps=5, moduli=14):
     return out
 
 
-def serving_bench(label, budget_gb, n, rate, seg_lo, seg_hi, max_batch=256):
+def serving_bench(label, budget_gb, n, rate, seg_lo, seg_hi, max_batch=256, moduli=0):
     """Continuous batching of a Poisson stream of single block contractions
     (ladder / ring / hole-hole mix) through the device block cache: the
     stream is replayed once to warm the cache, then timed against its
-    arrival schedule.  Latency = completion - scheduled arrival."""
-    mix = wl.serving_mix(n_requests=n, rate=rate, seg_lo=seg_lo, seg_hi=seg_hi)
+    arrival schedule.  Latency = completion - scheduled arrival.  `moduli`
+    > 0: every request on the split-precision path."""
+    mix = wl.serving_mix(n_requests=n, rate=rate, seg_lo=seg_lo, seg_hi=seg_hi, moduli=moduli)
     cache = sg.Cache(int(budget_gb * (1 << 30)))
     ws = wl.populate_cache(cache, mix)
     srv = sg.Server(cache, max_batch=max_batch)
@@ -270,6 +271,7 @@ def serving_bench(label, budget_gb, n, rate, seg_lo, seg_hi, max_batch=256):
     batches = s1["batches"] - s0["batches"]
     return {
         "scenario": label, "requests": n, "offered_rate_per_s": rate,
+        "precision": "split, %d moduli" % moduli if moduli else "fp64",
         "segments": mix["segs"], "terms": mix["terms"],
         "working_set_gb": ws / (1 << 30), "budget_gb": budget_gb,
         "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
@@ -394,6 +396,8 @@ def main():
     serving = None
     if not a.no_serve:
         serving = [serving_bench("resident (HBM budget 200 GB)", 200.0, 3000, 2000.0, 8, 64),
+                   serving_bench("resident (HBM budget 200 GB), split precision", 200.0, 3000,
+                                 2000.0, 8, 64, moduli=14),
                    serving_bench("LRU spill (budget 60% of working set)", 8.5, 1500, 4000.0,
                                  8, 40)]
     lat = latency_sample(w, a.latency_samples) if a.latency_samples > 0 else None

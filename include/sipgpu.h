@@ -270,9 +270,12 @@ int sipgpu_cache_get_stats(sipgpu_cache_t cache, sipgpu_cache_stats *stats);
  * Contraction server: continuous batching of single block contractions
  * whose blocks live in a cache.  A scheduler thread takes every request that
  * has arrived (up to `max_batch`, stopping at the first one that would read
- * a block an earlier request of the batch writes, or the reverse), makes
- * its blocks resident and launches the batch as one grouped FP64 launch; a
- * completion thread stamps each request's completion.  A batch pins at most
+ * a block an earlier request of the batch writes, or the reverse, and at a
+ * change of precision), makes its blocks resident and launches the batch:
+ * one grouped FP64 launch, or the split-precision stages with their planes
+ * in a server-owned scratch buffer (grown, after the device drains, to the
+ * largest batch's need); a completion thread stamps each request's
+ * completion.  A batch pins at most
  * half the cache budget (one request beyond that alone).  Requests on the same
  * blocks take effect in submission order.  No reference counterpart: the
  * SIP's worker runs one contraction at a time (interpreter.cpp:1210-1262). */
@@ -284,6 +287,7 @@ typedef struct {
     int ldims[SIPGPU_MAX_RANK], rdims[SIPGPU_MAX_RANK], ddims[SIPGPU_MAX_RANK];
     unsigned long long lkey, rkey, dkey;
     int accumulate;
+    int moduli; /* 0: plain FP64; 6..14: split precision over that many moduli */
 } sipgpu_request;
 
 typedef struct {

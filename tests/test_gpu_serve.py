@@ -98,8 +98,12 @@ def ring_keys(w):
     return keys
 
 
+@pytest.mark.parametrize("precision", ["fp64", "split", "mixed"])
 @pytest.mark.parametrize("budget_frac", [2.0, 0.4], ids=["fits", "evicts"])
-def test_server_matches_sequential_oracle(budget_frac):
+def test_server_matches_sequential_oracle(budget_frac, precision):
+    """Plain FP64 requests (1e-13), split-precision ones over 14 moduli
+    (1e-12), and a stream mixing both per request (batches break at each
+    change of precision), against the sequential oracle."""
     w = wl.ring(o_segs=(5, 4), v_segs=(10, 10, 7))
     D, L, R, ptrn = w["D"], w["L"], w["R"], w["ptrn"]
     total = 8 * (D.host.size + L.host.size + R.host.size)
@@ -114,11 +118,13 @@ def test_server_matches_sequential_oracle(budget_frac):
     srv = sg.Server(c, max_batch=64)
     rng = np.random.default_rng(2)
     order = rng.permutation(len(w["entries"]))
+    mods = {"fp64": [0], "split": [14], "mixed": [0, 14]}[precision]
     tickets = []
     for idx in order:
         db, lb, rb = w["entries"][idx]
         req = sg.make_request(ptrn, keys[(1, lb)], L.dims(lb), keys[(2, rb)], R.dims(rb),
-                              keys[(3, db)], D.dims(db), accumulate=True)
+                              keys[(3, db)], D.dims(db), accumulate=True,
+                              moduli=mods[int(rng.integers(len(mods)))])
         tickets.append(srv.submit(req))
     lat = [srv.wait(t) for t in tickets]
     assert all(x >= 0.0 for x in lat)
@@ -138,7 +144,54 @@ def test_server_matches_sequential_oracle(budget_frac):
     if budget_frac < 1.0:
         assert c.stats()["evictions"] > 0
     c.destroy()
-    assert worst <= 1e-13
+    assert worst <= (1e-13 if precision == "fp64" else 1e-12)
+
+
+@pytest.mark.parametrize("moduli", [0, 14])
+def test_server_result_same_alone_and_in_mix(moduli):
+    """Each request's result computed alone (one request in flight) equals
+    its result inside a stream of other requests batched with it, within the
+    path's tolerance."""
+    rng = np.random.default_rng(12)
+    ptrn = [1, -1, -2, 2]  # D(i,j) = L(i,k) R(k,j)
+    shapes = [(int(rng.integers(8, 64)), int(rng.integers(8, 64)), int(rng.integers(8, 64)))
+              for _ in range(24)]
+    c = sg.Cache(256 * MB)
+    key = 1
+    reqs = []
+    for m, k, n in shapes:
+        lk, rk, dk = key, key + 1, key + 2
+        key += 3
+        c.insert(lk, m * k, host=rng.standard_normal(m * k))
+        c.insert(rk, k * n, host=rng.standard_normal(k * n))
+        c.insert(dk, m * n, init="zero")
+        reqs.append(sg.make_request(ptrn, lk, [m, k], rk, [k, n], dk, [m, n], moduli=moduli))
+    srv = sg.Server(c, max_batch=64)
+    alone = []
+    for r in reqs:
+        srv.wait(srv.submit(r))
+        alone.append(c.read(r.dkey, r.ddims[0] * r.ddims[1]))
+    ts = [srv.submit(r) for r in reqs]
+    for t in ts:
+        srv.wait(t)
+    st = srv.stats()
+    assert st["max_batch"] > 1
+    tol = 1e-13 if moduli == 0 else 1e-12
+    for r, a in zip(reqs, alone):
+        assert rel_fro(c.read(r.dkey, r.ddims[0] * r.ddims[1]), a) <= tol
+    srv.destroy()
+    c.destroy()
+
+
+def test_server_rejects_bad_moduli():
+    c = sg.Cache(16 * MB)
+    for k in (1, 2, 3):
+        c.insert(k, 16, init="zero")
+    srv = sg.Server(c)
+    with pytest.raises(sg.SipGpuError):
+        srv.submit(sg.make_request([1, -1, -2, 2], 1, [4, 4], 2, [4, 4], 3, [4, 4], moduli=5))
+    srv.destroy()
+    c.destroy()
 
 
 def test_server_dependent_requests_keep_order():
