@@ -17,16 +17,38 @@ namespace {
 
 constexpr int kEnergyThreads = 256;
 
-__device__ __forceinline__ double denominator(const EnergyArgs &a, long long e) {
-    double v[kEnergyMaxRank];
+// Mixed-radix digits (index values) of element e, first index fastest.
+__device__ __forceinline__ void decode(const EnergyArgs &a, long long e, int (&idx)[kEnergyMaxRank]) {
 #pragma unroll
     for (int d = 0; d < kEnergyMaxRank; ++d) {
+        idx[d] = 0;
         if (d < a.rank) {
-            const int i = (int)(e % a.dims[d]);
+            idx[d] = (int)(e % a.dims[d]);
             e /= a.dims[d];
-            v[d] = a.eps[d][i];
         }
     }
+}
+
+// idx += step in mixed radix (both digit vectors are in range, so one
+// subtraction per digit is enough); a carry out of the last digit only
+// happens past the end of the block, where the loops stop.
+__device__ __forceinline__ void advance(const EnergyArgs &a, int (&idx)[kEnergyMaxRank],
+                                        const int (&step)[kEnergyMaxRank]) {
+    int carry = 0;
+#pragma unroll
+    for (int d = 0; d < kEnergyMaxRank; ++d)
+        if (d < a.rank) {
+            int v = idx[d] + step[d] + carry;
+            carry = v >= a.dims[d];
+            idx[d] = carry ? v - a.dims[d] : v;
+        }
+}
+
+__device__ __forceinline__ double denominator(const EnergyArgs &a,
+                                              const int (&idx)[kEnergyMaxRank]) {
+    double v[kEnergyMaxRank];
+#pragma unroll
+    for (int d = 0; d < kEnergyMaxRank; ++d) v[d] = d < a.rank ? a.eps[d][idx[d]] : 0.0;
     // occupied energies added in index order, then each virtual one
     // subtracted in index order: eps = e_b + e_d + e_f - e_a - e_c - e_e
     double D = 0.0;
@@ -46,11 +68,19 @@ __device__ __forceinline__ double denominator(const EnergyArgs &a, long long e) 
     return D;
 }
 
+// Grid-stride loops keep each thread's element index as digits: decoded
+// once, then advanced by the grid stride (no divisions per element).
 __global__ __launch_bounds__(kEnergyThreads) void denominator_divide_kernel(EnergyArgs a,
                                                                             double *x) {
-    for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < a.n;
-         e += (long long)gridDim.x * blockDim.x)
-        x[e] = x[e] / denominator(a, e);
+    const long long e0 = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long S = (long long)gridDim.x * blockDim.x;
+    int idx[kEnergyMaxRank], step[kEnergyMaxRank];
+    decode(a, e0, idx);
+    decode(a, S, step);
+    for (long long e = e0; e < a.n; e += S) {
+        x[e] = x[e] / denominator(a, idx);
+        advance(a, idx, step);
+    }
 }
 
 __global__ __launch_bounds__(kEnergyThreads) void energy_partial_kernel(EnergyArgs a,
@@ -58,10 +88,16 @@ __global__ __launch_bounds__(kEnergyThreads) void energy_partial_kernel(EnergyAr
                                                                         const double *y,
                                                                         double *part) {
     __shared__ double red[kEnergyThreads];
+    const long long e0 = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long S = (long long)gridDim.x * blockDim.x;
+    int idx[kEnergyMaxRank], step[kEnergyMaxRank];
+    decode(a, e0, idx);
+    decode(a, S, step);
     double v = 0.0;
-    for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < a.n;
-         e += (long long)gridDim.x * blockDim.x)
-        v += (x[e] / denominator(a, e)) * y[e];
+    for (long long e = e0; e < a.n; e += S) {
+        v += (x[e] / denominator(a, idx)) * y[e];
+        advance(a, idx, step);
+    }
     red[threadIdx.x] = v;
     __syncthreads();
     for (int s = kEnergyThreads / 2; s > 0; s >>= 1) {
@@ -71,12 +107,21 @@ __global__ __launch_bounds__(kEnergyThreads) void energy_partial_kernel(EnergyAr
     if (threadIdx.x == 0) part[blockIdx.x] = red[0];
 }
 
-__global__ void energy_final_kernel(double *out, const double *part, int nparts, double factor) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) {
-        double v = 0.0;
-        for (int i = 0; i < nparts; ++i) v += part[i];
-        out[0] += factor * v;
+// One workgroup: thread t sums partials t, t + 256, ... in that order, then
+// a fixed tree over the threads (the same order on every run).
+__global__ __launch_bounds__(kEnergyThreads) void energy_final_kernel(double *out,
+                                                                      const double *part,
+                                                                      int nparts, double factor) {
+    __shared__ double red[kEnergyThreads];
+    double v = 0.0;
+    for (int i = threadIdx.x; i < nparts; i += kEnergyThreads) v += part[i];
+    red[threadIdx.x] = v;
+    __syncthreads();
+    for (int s = kEnergyThreads / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+        __syncthreads();
     }
+    if (threadIdx.x == 0) out[0] += factor * red[0];
 }
 
 } // namespace
@@ -96,8 +141,8 @@ hipError_t launch_energy_sum(const EnergyArgs &a, const double *x, const double 
                        x, y, part);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(energy_final_kernel, dim3(1), dim3(64), 0, s, out, part, kEnergyBlocks,
-                       factor);
+    hipLaunchKernelGGL(energy_final_kernel, dim3(1), dim3(kEnergyThreads), 0, s, out, part,
+                       kEnergyBlocks, factor);
     return hipGetLastError();
 }
 

@@ -101,7 +101,7 @@ hipError_t launch_copy(const double *in, double *out, size_t n, hipStream_t s);
 // Orbital-energy denominators: rank <= kEnergyMaxRank block, one energy
 // vector and one sign (+1 occupied, -1 virtual) per index.
 constexpr int kEnergyMaxRank = 6;
-constexpr int kEnergyBlocks = 512;
+constexpr int kEnergyBlocks = 4096; // 16 waves per CU: the reduction's gathers need them
 struct EnergyArgs {
     int rank;
     int dims[kEnergyMaxRank];

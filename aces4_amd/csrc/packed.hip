@@ -209,80 +209,73 @@ __global__ __launch_bounds__(256, 2) void gemm_f64_packed_kernel(OzGemmArgs a) {
                         w[(((wave * FM + i) * FN + j) * 4 + r) * 64 + lane] = acc[i][j][r];
             continue;
         }
-        // scatter into the destination block(s)
-        if (task.rmap < 0) {
-            __attribute__((address_space(1))) double *D =
-                (__attribute__((address_space(1))) double *)task.D;
+        // scatter into the destination block(s), 16 elements at a time:
+        // their destinations first, then all loads (beta), then the stores,
+        // so the loads overlap instead of waiting behind each other's stores
 #pragma unroll
-            for (int j = 0; j < FN; ++j) {
-                const int n = n0 + wn * TN + j * 16 + li;
-                if (n >= task.N) continue;
-                const int no = a.tab[task.nD + n];
+        for (int j = 0; j < FN; ++j) {
+            const int n = n0 + wn * TN + j * 16 + li;
+            const bool nin = n < task.N;
+            const int no = (task.rmap < 0 && nin) ? a.tab[task.nD + n] : 0;
+            double *p[FM * 4];
+            int bt[FM * 4];
 #pragma unroll
-                for (int i = 0; i < FM; ++i)
+            for (int i = 0; i < FM; ++i)
 #pragma unroll
-                    for (int r = 0; r < 4; ++r) {
-                        const int m = m0 + wm * TM + i * 16 + 4 * r + lk;
-                        if (m < task.M) {
-                            const int off = a.tab[task.mD + m] + no;
-                            double v = acc[i][j][r];
-                            if (task.beta) v += D[off];
-                            D[off] = v;
+                for (int r = 0; r < 4; ++r) {
+                    const int m = m0 + wm * TM + i * 16 + 4 * r + lk;
+                    const int q = i * 4 + r;
+                    p[q] = nullptr;
+                    bt[q] = 0;
+                    if (nin && m < task.M) {
+                        if (task.rmap < 0) {
+                            p[q] = task.D + a.tab[task.mD + m] + no;
+                            bt[q] = task.beta;
+                        } else {
+                            p[q] = oz_dest(task, a.subs, a.tab, m, n, bt[q]);
                         }
                     }
-            }
-        } else {
-            // a grid of result blocks: each element finds its block
+                }
 #pragma unroll
-            for (int j = 0; j < FN; ++j) {
-                const int n = n0 + wn * TN + j * 16 + li;
-                if (n >= task.N) continue;
+            for (int i = 0; i < FM; ++i)
 #pragma unroll
-                for (int i = 0; i < FM; ++i)
+                for (int r = 0; r < 4; ++r)
+                    if (p[i * 4 + r] && bt[i * 4 + r]) acc[i][j][r] += *p[i * 4 + r];
 #pragma unroll
-                    for (int r = 0; r < 4; ++r) {
-                        const int m = m0 + wm * TM + i * 16 + 4 * r + lk;
-                        if (m < task.M) {
-                            int beta;
-                            double *p = oz_dest(task, a.subs, a.tab, m, n, beta);
-                            double v = acc[i][j][r];
-                            if (beta) v += *p;
-                            *p = v;
-                        }
-                    }
-            }
+            for (int i = 0; i < FM; ++i)
+#pragma unroll
+                for (int r = 0; r < 4; ++r)
+                    if (p[i * 4 + r]) *p[i * 4 + r] = acc[i][j][r];
         }
     }
 }
 
-// Split tiles: one workgroup per tile sums its segments' partial products
-// in segment order (the same layout the GEMM wrote: thread = wave * 64 +
-// lane holds fragment (i, j, r)) and stores the sum like the GEMM epilogue.
+// Split tiles: one thread per element of a split tile (64 workgroups of 256
+// per tile) sums the tile's segments in segment order -- the layout the GEMM
+// wrote: element e = (((wave * 4 + i) * 4 + j) * 4 + r) * 64 + lane holds
+// fragment (i, j, r) of that wave and lane -- and stores the sum like the
+// GEMM epilogue.  One element per thread keeps thousands of the dependent
+// destination lookups in flight.
 __global__ __launch_bounds__(256) void fixup_f64_kernel(OzGemmArgs a) {
-    constexpr int BM = 128, BN = 128, WM = 2, TM = 64, TN = 64, FM = 4, FN = 4;
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int wm = wave % WM, wn = wave / WM, li = lane & 15, lk = lane >> 4;
-    for (int it = blockIdx.x; it < a.nfix; it += gridDim.x) {
-        const OzTile f = a.fix[it];
+    constexpr int BM = 128, BN = 128, TM = 64, TN = 64;
+    constexpr int kPer = BM * BN / 256; // workgroups per tile
+    const long long nwork = (long long)a.nfix * kPer;
+    for (long long w = blockIdx.x; w < nwork; w += gridDim.x) {
+        const OzTile f = a.fix[w / kPer];
+        const int e = (int)(w % kPer) * 256 + threadIdx.x;
+        const int lane = e & 63, r = (e >> 6) & 3, j = (e >> 8) & 3, i = (e >> 10) & 3;
+        const int wave = e >> 12, wm = wave & 1, wn = wave >> 1;
         const OzTask task = a.tasks[f.task];
-        const int m0 = f.mt * BM, n0 = f.nt * BN;
-        const double *w0 = a.work + (long long)f.part * (BM * BN);
-        for (int j = 0; j < FN; ++j) {
-            const int n = n0 + wn * TN + j * 16 + li;
-            for (int i = 0; i < FM; ++i)
-                for (int r = 0; r < 4; ++r) {
-                    const int e = (((wave * FM + i) * FN + j) * 4 + r) * 64 + lane;
-                    double v = w0[e];
-                    for (int q = 1; q < f.s0; ++q) v += w0[(long long)q * (BM * BN) + e];
-                    const int m = m0 + wm * TM + i * 16 + 4 * r + lk;
-                    if (m < task.M && n < task.N) {
-                        int beta;
-                        double *p = oz_dest(task, a.subs, a.tab, m, n, beta);
-                        if (beta) v += *p;
-                        *p = v;
-                    }
-                }
-        }
+        const int m = f.mt * BM + wm * TM + i * 16 + 4 * r + (lane >> 4);
+        const int n = f.nt * BN + wn * TN + j * 16 + (lane & 15);
+        if (m >= task.M || n >= task.N) continue;
+        const double *w0 = a.work + (long long)f.part * (BM * BN) + e;
+        double v = w0[0];
+        for (int q = 1; q < f.s0; ++q) v += w0[(long long)q * (BM * BN)];
+        int beta;
+        double *p = oz_dest(task, a.subs, a.tab, m, n, beta);
+        if (beta) v += *p;
+        *p = v;
     }
 }
 
@@ -290,7 +283,9 @@ __global__ __launch_bounds__(256) void fixup_f64_kernel(OzGemmArgs a) {
 
 hipError_t oz_fixup_f64(const OzGemmArgs &a, hipStream_t s) {
     if (a.nfix <= 0) return hipSuccess;
-    hipLaunchKernelGGL(fixup_f64_kernel, dim3(a.nfix < 65535 ? a.nfix : 65535), dim3(256), 0, s, a);
+    const long long work = (long long)a.nfix * (kOzBM * kOzBN / 256);
+    hipLaunchKernelGGL(fixup_f64_kernel, dim3((unsigned)(work < 1048576 ? work : 1048576)),
+                       dim3(256), 0, s, a);
     return hipGetLastError();
 }
 

@@ -300,6 +300,26 @@ struct TablePool {
         return {add(s.koffL), add(s.moffL), add(s.koffR), add(s.noffR), add(s.moffD),
                 add(s.noffD)};
     }
+    // The same tables with rows (columns) renumbered in order of their
+    // destination offsets: a relabeling of the product's rows that changes no
+    // sum, after which the destination's unit-stride index, wherever it sits
+    // among the row (column) indices, runs along consecutive rows (columns).
+    std::array<int, 6> add_shape_dest_ordered(const ContractionShape &s) {
+        auto order = [](const std::vector<int32_t> &d) {
+            std::vector<int> p(d.size());
+            std::iota(p.begin(), p.end(), 0);
+            std::stable_sort(p.begin(), p.end(), [&](int a, int b) { return d[a] < d[b]; });
+            return p;
+        };
+        auto take = [](const std::vector<int32_t> &t, const std::vector<int> &p) {
+            std::vector<int32_t> out(p.size());
+            for (size_t i = 0; i < p.size(); ++i) out[i] = t[p[i]];
+            return out;
+        };
+        const std::vector<int> pm = order(s.moffD), pn = order(s.noffD);
+        return {add(s.koffL), add(take(s.moffL, pm)), add(s.koffR), add(take(s.noffR, pn)),
+                add(take(s.moffD, pm)), add(take(s.noffD, pn))};
+    }
 };
 
 // Static XCD schedule of a grouped launch.  Slot t of the launch runs on
@@ -779,7 +799,7 @@ int sipgpu::rt::build_panel_plan(const sipgpu_contraction *list, int n, int nmod
     TablePool tp;
     std::vector<int32_t> &pool = tp.pool;
     std::vector<std::array<int, 6>> tabs(shapes.size());
-    for (size_t si = 0; si < shapes.size(); ++si) tabs[si] = tp.add_shape(*shapes[si]);
+    for (size_t si = 0; si < shapes.size(); ++si) tabs[si] = tp.add_shape_dest_ordered(*shapes[si]);
     auto unit_stride = [&](int off, int len) { return len > 1 && pool[off + 1] - pool[off] == 1; };
     // operand instances: one per (block, k table, x table)
     struct Inst { const double *ptr; int K, X, Kp, ktab, xtab, xfast; };
@@ -857,6 +877,20 @@ int sipgpu::rt::build_panel_plan(const sipgpu_contraction *list, int n, int nmod
         dests.push_back(std::move(d));
     }
     const int ndest = (int)dests.size();
+    // Orientation: the GEMM's output lanes run along its columns, so a block
+    // whose unit-stride index is among its rows is computed transposed
+    // (D^T = R^T L: the same sums with the operands' roles swapped) and its
+    // stores stay contiguous.
+    for (Dest &d : dests) {
+        const bool mfast = d.M > 1 && unit_stride(d.s.mD, d.M);
+        const bool nfast = d.N > 1 && unit_stride(d.s.nD, d.N);
+        if (mfast && !nfast) {
+            std::swap(d.a, d.b);
+            std::swap(d.M, d.N);
+            std::swap(d.s.mD, d.s.nD);
+            d.s.nfast = 1;
+        }
+    }
 
     // Merged tasks.  A row key is a result block's list of left instances,
     // a column key its list of right instances.  Result blocks whose column
