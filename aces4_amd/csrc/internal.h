@@ -24,6 +24,13 @@ int fail(int code, const std::string &msg);
 int hip_fail(hipError_t e, const char *what);
 // The library is initialised on its device (first use initialises device 0).
 int ensure_ready();
+// True while `s` is being captured into a graph.
+bool capturing(hipStream_t s);
+// Streams the calling thread is capturing (planning, allocation, host copies
+// and synchronisation would invalidate their capture).
+int &thread_captures();
+// SIPGPU_ECAPTURE when the calling thread is capturing, else 0.
+int refuse_in_capture(const char *what);
 // The library's device, and the HIP stream of `s` (null: the library's own).
 int device_id();
 hipStream_t stream_of(sipgpu_stream_t s);
@@ -47,8 +54,17 @@ struct BatchImage {
     }
 };
 
-// Plans a list of block contractions (same rules as sipgpu_batch_plan).
-int build_batch_image(const sipgpu_contraction *list, int n, BatchImage &img);
+// Plans a list of block contractions (same rules as sipgpu_batch_plan) for
+// the table kernel; cfg >= 0 pins its tile configuration.
+int build_batch_image(const sipgpu_contraction *list, int n, BatchImage &img, int cfg = -1);
+
+// Choices of the panel planner; -1 takes the default (environment, then the
+// built-in rule).
+struct PanelOptions {
+    int merge = -1; // 1: merge result blocks sharing a virtual K into grids
+    int tail = -1;  // 1: persistent schedule with a water-filled last round (FP64)
+    int tile = -1;  // split path GEMM tile: 128 or 256
+};
 
 // Device state of a panel batch (packed FP64 or split precision).
 struct SplitData {
@@ -86,7 +102,8 @@ struct PanelPlan {
     size_t work_total() const { return (size_t)nslots * kOzBM * kOzBN * sizeof(double); }
 };
 
-int build_panel_plan(const sipgpu_contraction *list, int n, int nmod, PanelPlan &p);
+int build_panel_plan(const sipgpu_contraction *list, int n, int nmod, PanelPlan &p,
+                     const PanelOptions &opt = PanelOptions());
 // Points the kernels' arguments at an uploaded image and its memory.
 void bind_panel_plan(const PanelPlan &p, void *desc, int8_t *planes, uint8_t *res_planes,
                      double *work, SplitData &sd);

@@ -154,6 +154,8 @@ int get_shape(Context &c, const int *ptrn, int lr, const int *ld, int rr, const 
     const std::string key = shape_key(ptrn, lr, ld, rr, rd, dr, dd);
     auto it = c.shapes.find(key);
     if (it != c.shapes.end()) { *out = it->second.get(); return 0; }
+    // a new shape uploads its tables: run it once before capturing
+    if (int rc = sipgpu::rt::refuse_in_capture("a contraction of a new shape")) return rc;
     auto ds = std::make_unique<DeviceShape>();
     int rc = plan_contraction(ptrn, lr, ld, rr, rd, dr, dd, ds->shape, true);
     if (rc) return rc;
@@ -550,7 +552,8 @@ int check_hazards(const sipgpu_contraction *list, int n, const ShapeVec &shapes,
 
 } // namespace
 
-int sipgpu::rt::build_batch_image(const sipgpu_contraction *list, int n, BatchImage &b) {
+int sipgpu::rt::build_batch_image(const sipgpu_contraction *list, int n, BatchImage &b,
+                                  int cfg) {
     struct Entry {
         const sipgpu_contraction *e;
         std::string key;
@@ -650,7 +653,9 @@ int sipgpu::rt::build_batch_image(const sipgpu_contraction *list, int n, BatchIm
         sumN += task.N;
     }
     const int ntasks = (int)tasks.size();
-    b.cfg = pick_config(ntasks ? sumM / ntasks : 0, ntasks ? sumN / ntasks : 0);
+    b.cfg = cfg >= 0 && cfg < kGemmConfigs
+                ? cfg
+                : pick_config(ntasks ? sumM / ntasks : 0, ntasks ? sumN / ntasks : 0);
     const int bm = gemm_tile_m(b.cfg), bn = gemm_tile_n(b.cfg);
 
     std::vector<int> tms(ntasks), tns(ntasks);
@@ -689,9 +694,10 @@ int sipgpu::rt::build_batch_image(const sipgpu_contraction *list, int n, BatchIm
 
 namespace {
 
-int plan_batch_locked(Context &, const sipgpu_contraction *list, int n, sipgpu_batch_s &b) {
+int plan_batch_locked(Context &, const sipgpu_contraction *list, int n, sipgpu_batch_s &b,
+                      int cfg = -1) {
     BatchImage img;
-    if (int rc = sipgpu::rt::build_batch_image(list, n, img)) return rc;
+    if (int rc = sipgpu::rt::build_batch_image(list, n, img, cfg)) return rc;
     HIP_TRY(hipMalloc(&b.dev, img.host.size()));
     HIP_TRY(hipMemcpy(b.dev, img.host.data(), img.host.size(), hipMemcpyHostToDevice));
     b.args = img.args_at(b.dev);
@@ -764,7 +770,8 @@ int oz_constants(int n, long long kmax, OzConst &c) {
 // products scattered straight into the destinations).
 } // namespace
 
-int sipgpu::rt::build_panel_plan(const sipgpu_contraction *list, int n, int nmod, PanelPlan &P) {
+int sipgpu::rt::build_panel_plan(const sipgpu_contraction *list, int n, int nmod, PanelPlan &P,
+                                 const PanelOptions &opt) {
     if (nmod != 0 && (nmod < kOzMinModuli || nmod > kOzMaxUsable))
         return fail(SIPGPU_EARG, "split path: nmoduli must be in 6..14");
     const int eb = nmod ? 1 : 8; // bytes per panel element
@@ -927,7 +934,8 @@ int sipgpu::rt::build_panel_plan(const sipgpu_contraction *list, int n, int nmod
     std::map<std::vector<int>, int> grid_of;
     std::vector<std::pair<std::vector<int>, std::vector<int>>> grids; // (row keys, col keys)
     const char *merge_env = std::getenv("SIPGPU_MERGE"); // "0": one task per result block
-    if (merge_env && !std::strcmp(merge_env, "0")) {
+    const bool merge = opt.merge >= 0 ? opt.merge != 0 : !(merge_env && !std::strcmp(merge_env, "0"));
+    if (!merge) {
         for (int i = 0; i < ndest; ++i) grids.push_back({{d_r[i]}, {d_c[i]}});
     } else {
         for (int c = 0; c < (int)cvec.size(); ++c) {
@@ -1144,6 +1152,7 @@ int sipgpu::rt::build_panel_plan(const sipgpu_contraction *list, int n, int nmod
         const int v = std::atoi(e);
         if (nmod && (v == kOzBM || v == kOzBigTile)) tile = v;
     }
+    if (nmod && (opt.tile == kOzBM || opt.tile == kOzBigTile)) tile = opt.tile;
     // tiles: the same XCD schedule as the FP64 batch
     const char *tail_env = std::getenv("SIPGPU_SPLIT_TAIL"); // "0": one tile per workgroup
     std::vector<int> tms(ntasks), tns(ntasks);
@@ -1164,7 +1173,8 @@ int sipgpu::rt::build_panel_plan(const sipgpu_contraction *list, int n, int nmod
     std::vector<OzTile> tiles, fix;
     std::vector<int> wg_first;
     int nslots = 0;
-    if (!nmod && !(tail_env && !std::strcmp(tail_env, "0"))) {
+    const bool tail = opt.tail >= 0 ? opt.tail != 0 : !(tail_env && !std::strcmp(tail_env, "0"));
+    if (!nmod && tail) {
         // the packed FP64 GEMM runs two workgroups per CU, 32 CUs per XCD
         persistent_schedule(lists, costs, 64, tiles, wg_first, fix, nslots);
     } else {
@@ -1310,9 +1320,10 @@ namespace {
 // A panel batch with memory of its own: descriptors, planes (zeroed once:
 // the zero bytes at the start of each plane are never written again),
 // result planes and the split-tile workspace.
-int plan_panels_locked(const sipgpu_contraction *list, int n, int nmod, sipgpu_batch_s &b) {
+int plan_panels_locked(const sipgpu_contraction *list, int n, int nmod, sipgpu_batch_s &b,
+                       const sipgpu::rt::PanelOptions &opt = sipgpu::rt::PanelOptions()) {
     PanelPlan P;
-    if (int rc = sipgpu::rt::build_panel_plan(list, n, nmod, P)) return rc;
+    if (int rc = sipgpu::rt::build_panel_plan(list, n, nmod, P, opt)) return rc;
     SplitData &sd = b.sd;
     HIP_TRY(hipMalloc(&b.dev, P.host.size()));
     HIP_TRY(hipMemcpy(b.dev, P.host.data(), P.host.size(), hipMemcpyHostToDevice));
@@ -1349,6 +1360,124 @@ int execute_batch(Context &c, sipgpu_batch_s &b, int stages, sipgpu_stream_t s) 
     const hipStream_t hs = resolve(c, s).s;
     if (b.split || b.packed) return sipgpu::rt::execute_panels(b.sd, b.split, stages, hs);
     if (stages & SIPGPU_STAGE_PRODUCT) HIP_TRY(launch_gemm(b.cfg, b.args, b.ntiles, hs));
+    return 0;
+}
+
+} // namespace
+
+namespace {
+
+// How a batch is planned: path 0 = table kernel (tile configuration cfg),
+// 1 = packed FP64 panels, 2 = split precision; merge / tail / tile as in
+// PanelOptions (-1: default).
+struct PlanChoice {
+    int path = -1, cfg = -1, merge = -1, tail = -1, tile = -1;
+    bool operator==(const PlanChoice &o) const {
+        return path == o.path && cfg == o.cfg && merge == o.merge && tail == o.tail &&
+               tile == o.tile;
+    }
+};
+
+// Tuned choices by batch signature (guarded by the context lock).
+std::unordered_map<uint64_t, PlanChoice> &tuned_table() {
+    static auto *t = new std::unordered_map<uint64_t, PlanChoice>();
+    return *t;
+}
+
+// Signature of a batch: its shapes, patterns, accumulate flags and which
+// entries share operand / destination blocks (blocks numbered in order of
+// first appearance), and the precision.  Two batches with equal signatures
+// plan identically up to the block addresses.
+uint64_t batch_signature(const sipgpu_contraction *list, int n, int nmod) {
+    uint64_t h = 1469598103934665603ull;
+    auto mix = [&](uint64_t v) {
+        for (int i = 0; i < 8; ++i) {
+            h ^= (v >> (8 * i)) & 0xff;
+            h *= 1099511628211ull;
+        }
+    };
+    std::unordered_map<const double *, int> id;
+    auto block = [&](const double *p) {
+        auto it = id.emplace(p, (int)id.size());
+        return it.first->second;
+    };
+    mix((uint64_t)nmod);
+    mix((uint64_t)n);
+    for (int i = 0; i < n; ++i) {
+        const sipgpu_contraction &e = list[i];
+        mix(((uint64_t)e.lrank << 16) | ((uint64_t)e.rrank << 8) | (uint64_t)e.drank);
+        for (int k = 0; k < e.lrank + e.rrank && k < 2 * SIPGPU_MAX_RANK; ++k) mix((uint64_t)(int64_t)e.ptrn[k]);
+        for (int k = 0; k < e.lrank; ++k) mix((uint64_t)e.ldims[k]);
+        for (int k = 0; k < e.rrank; ++k) mix((uint64_t)e.rdims[k]);
+        for (int k = 0; k < e.drank; ++k) mix((uint64_t)e.ddims[k]);
+        mix((uint64_t)e.accumulate);
+        mix((uint64_t)block(e.l));
+        mix((uint64_t)block(e.r));
+        mix((uint64_t)block(e.d));
+    }
+    return h;
+}
+
+bool tuned_choice(Context &, const sipgpu_contraction *list, int n, int nmod, PlanChoice &ch) {
+    auto &t = tuned_table();
+    if (t.empty()) return false;
+    auto it = t.find(batch_signature(list, n, nmod));
+    if (it == t.end()) return false;
+    ch = it->second;
+    return true;
+}
+
+int plan_choice_locked(Context &c, const sipgpu_contraction *list, int n, int nmod,
+                       const PlanChoice &ch, std::unique_ptr<sipgpu_batch_s> &out) {
+    auto b = std::make_unique<sipgpu_batch_s>();
+    sipgpu::rt::PanelOptions opt;
+    opt.merge = ch.merge;
+    opt.tail = ch.tail;
+    opt.tile = ch.tile;
+    int rc;
+    if (ch.path == 0)
+        rc = plan_batch_locked(c, list, n, *b, ch.cfg);
+    else
+        rc = plan_panels_locked(list, n, ch.path == 2 ? nmod : 0, *b, opt);
+    if (rc) {
+        free_batch_memory(*b);
+        return rc;
+    }
+    out = std::move(b);
+    return 0;
+}
+
+// The plan sipgpu_batch_plan makes without a tuned choice.  Large batches
+// run on packed operand panels: the permutations are applied once per
+// block (one streaming pass) instead of per tile, and the GEMM runs over
+// whole 16-k stages of each task's virtual K.  Small batches (and single
+// contractions) keep the table-driven kernel: no extra pass, no panel
+// memory.  SIPGPU_FP64_PATH=tables|packed pins it.
+int plan_default_locked(Context &c, const sipgpu_contraction *list, int n,
+                        std::unique_ptr<sipgpu_batch_s> &out, PlanChoice *chosen) {
+    std::unique_ptr<sipgpu_batch_s> b;
+    PlanChoice tab;
+    tab.path = 0;
+    if (int rc = plan_choice_locked(c, list, n, 0, tab, b)) return rc;
+    bool packed = b->ntasks > 0 && b->flops >= 1e9 && b->flops / b->ntasks >= 5e7;
+    if (const char *e = std::getenv("SIPGPU_FP64_PATH")) {
+        if (!std::strcmp(e, "tables")) packed = false;
+        if (!std::strcmp(e, "packed")) packed = b->ntasks > 0;
+    }
+    PlanChoice used = tab;
+    used.cfg = b->cfg;
+    if (packed) {
+        PlanChoice pk;
+        pk.path = 1;
+        std::unique_ptr<sipgpu_batch_s> pb;
+        if (plan_choice_locked(c, list, n, 0, pk, pb) == 0) {
+            free_batch_memory(*b);
+            b = std::move(pb);
+            used = pk;
+        }
+    }
+    if (chosen) *chosen = used;
+    out = std::move(b);
     return 0;
 }
 
@@ -1475,11 +1604,13 @@ int sipgpu_device_count(int *count) {
 }
 
 int sipgpu_device_synchronize(void) {
+    if (int rc = sipgpu::rt::refuse_in_capture("device synchronisation")) return rc;
     HIP_TRY(hipDeviceSynchronize());
     return 0;
 }
 
 int sipgpu_malloc(void **dptr, size_t bytes) {
+    if (int rc = sipgpu::rt::refuse_in_capture("allocation")) return rc;
     if (!dptr) return fail(SIPGPU_EARG, "null pointer");
     Context &c = ctx();
     {
@@ -1493,6 +1624,7 @@ int sipgpu_malloc(void **dptr, size_t bytes) {
 }
 
 int sipgpu_free(void *dptr) {
+    if (int rc = sipgpu::rt::refuse_in_capture("freeing")) return rc;
     HIP_TRY(hipDeviceSynchronize()); // no launch in flight may still use it
     HIP_TRY(hipFree(dptr));
     return 0;
@@ -1501,22 +1633,26 @@ int sipgpu_free(void *dptr) {
 // The plain copies are ordered after all work enqueued so far on every
 // stream (the library's streams do not synchronise with the null stream).
 int sipgpu_memcpy_h2d(void *dst, const void *src, size_t bytes) {
+    if (int rc = sipgpu::rt::refuse_in_capture("a host copy")) return rc;
     HIP_TRY(hipDeviceSynchronize());
     HIP_TRY(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
     return 0;
 }
 int sipgpu_memcpy_d2h(void *dst, const void *src, size_t bytes) {
+    if (int rc = sipgpu::rt::refuse_in_capture("a host copy")) return rc;
     HIP_TRY(hipDeviceSynchronize());
     HIP_TRY(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
     return 0;
 }
 int sipgpu_memcpy_d2d(void *dst, const void *src, size_t bytes) {
+    if (int rc = sipgpu::rt::refuse_in_capture("a blocking copy")) return rc;
     HIP_TRY(hipDeviceSynchronize());
     HIP_TRY(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToDevice));
     HIP_TRY(hipDeviceSynchronize());
     return 0;
 }
 int sipgpu_memset(void *dptr, int value, size_t bytes) {
+    if (int rc = sipgpu::rt::refuse_in_capture("a blocking memset")) return rc;
     HIP_TRY(hipDeviceSynchronize());
     HIP_TRY(hipMemset(dptr, value, bytes));
     HIP_TRY(hipDeviceSynchronize());
@@ -1551,6 +1687,7 @@ int sipgpu_stream_destroy(sipgpu_stream_t s) {
 }
 
 int sipgpu_stream_synchronize(sipgpu_stream_t s) {
+    if (int rc = sipgpu::rt::refuse_in_capture("stream synchronisation")) return rc;
     Context &c = ctx();
     if (!s && !c.ready) return 0;
     HIP_TRY(hipStreamSynchronize(resolve(c, s).s));
@@ -1632,37 +1769,23 @@ int sipgpu_contract(sipgpu_stream_t s, const int *ptrn, const double *l, int lra
 }
 
 int sipgpu_batch_plan(const sipgpu_contraction *list, int n, sipgpu_batch_t *batch) {
+    if (int rc = sipgpu::rt::refuse_in_capture("batch planning")) return rc;
     if (!batch || (n > 0 && !list) || n < 0) return fail(SIPGPU_EARG, "bad batch arguments");
     Context &c = ctx();
     std::lock_guard<std::mutex> g(c.mu);
     int rc = ensure_init(c);
     if (rc) return rc;
-    auto b = std::make_unique<sipgpu_batch_s>();
-    rc = plan_batch_locked(c, list, n, *b);
-    if (rc) {
-        if (b->dev) (void)hipFree(b->dev);
-        return rc;
-    }
-    // Large batches run on packed operand panels: the permutations are
-    // applied once per block (one streaming pass) instead of per tile, and
-    // the GEMM runs over whole 16-k stages of each task's virtual K.  Small
-    // batches (and single contractions) keep the table-driven kernel: no
-    // extra pass, no panel memory.  SIPGPU_FP64_PATH=tables|packed pins it.
-    bool packed = b->ntasks > 0 && b->flops >= 1e9 && b->flops / b->ntasks >= 5e7;
-    if (const char *e = std::getenv("SIPGPU_FP64_PATH")) {
-        if (!std::strcmp(e, "tables")) packed = false;
-        if (!std::strcmp(e, "packed")) packed = b->ntasks > 0;
-    }
-    if (packed) {
-        auto pb = std::make_unique<sipgpu_batch_s>();
-        rc = plan_panels_locked(list, n, 0, *pb);
-        if (rc) {
-            free_batch_memory(*pb);
-        } else {
-            free_batch_memory(*b);
-            b = std::move(pb);
+    std::unique_ptr<sipgpu_batch_s> b;
+    PlanChoice ch;
+    if (tuned_choice(c, list, n, 0, ch)) {
+        rc = plan_choice_locked(c, list, n, 0, ch, b);
+        if (rc == 0) {
+            *batch = b.release();
+            return 0;
         }
     }
+    rc = plan_default_locked(c, list, n, b, nullptr);
+    if (rc) return rc;
     *batch = b.release();
     return 0;
 }
@@ -1679,6 +1802,7 @@ int sipgpu_batch_execute_stages(sipgpu_batch_t b, int stages, sipgpu_stream_t s)
 }
 
 int sipgpu_batch_destroy(sipgpu_batch_t b) {
+    if (int rc = sipgpu::rt::refuse_in_capture("destroying a batch")) return rc;
     if (!b) return fail(SIPGPU_EARG, "null batch");
     // the work list may still be read by a launch in flight on any stream
     hipError_t e = hipDeviceSynchronize();
@@ -1690,6 +1814,7 @@ int sipgpu_batch_destroy(sipgpu_batch_t b) {
 
 int sipgpu_batch_plan_split(const sipgpu_contraction *list, int n, int nmoduli,
                             sipgpu_batch_t *batch) {
+    if (int rc = sipgpu::rt::refuse_in_capture("batch planning")) return rc;
     if (!batch || (n > 0 && !list) || n < 0) return fail(SIPGPU_EARG, "bad batch arguments");
     if (nmoduli < kOzMinModuli || nmoduli > kOzMaxUsable)
         return fail(SIPGPU_EARG, "split path: nmoduli must be in 6..14");
@@ -1697,13 +1822,214 @@ int sipgpu_batch_plan_split(const sipgpu_contraction *list, int n, int nmoduli,
     std::lock_guard<std::mutex> g(c.mu);
     int rc = ensure_init(c);
     if (rc) return rc;
-    auto b = std::make_unique<sipgpu_batch_s>();
-    rc = plan_panels_locked(list, n, nmoduli, *b);
-    if (rc) {
-        free_batch_memory(*b);
-        return rc;
-    }
+    std::unique_ptr<sipgpu_batch_s> b;
+    PlanChoice ch;
+    ch.path = 2;
+    (void)tuned_choice(c, list, n, nmoduli, ch);
+    rc = plan_choice_locked(c, list, n, nmoduli, ch, b);
+    if (rc) return rc;
     *batch = b.release();
+    return 0;
+}
+
+int sipgpu_batch_tune(const sipgpu_contraction *list, int n, int nmoduli, sipgpu_batch_t *batch,
+                      sipgpu_tune_result *result) {
+    if (int rc = sipgpu::rt::refuse_in_capture("tuning")) return rc;
+    if (!batch || (n > 0 && !list) || n < 1) return fail(SIPGPU_EARG, "bad batch arguments");
+    if (nmoduli != 0 && (nmoduli < 6 || nmoduli > 14))
+        return fail(SIPGPU_EARG, "split path: nmoduli must be in 6..14");
+    Context &c = ctx();
+    std::lock_guard<std::mutex> g(c.mu);
+    int rc = ensure_init(c);
+    if (rc) return rc;
+    // scratch destinations, one per distinct result block
+    std::unordered_map<const double *, size_t> dest_off;
+    size_t total = 0;
+    for (int i = 0; i < n; ++i) {
+        if (!list[i].d) return fail(SIPGPU_EARG, "null block pointer in batch");
+        if (dest_off.count(list[i].d)) continue;
+        dest_off[list[i].d] = total;
+        total += ((size_t)std::max<int64_t>(prod(list[i].drank, list[i].ddims), 1) + 31) / 32 * 32;
+    }
+    double *scratch = nullptr;
+    HIP_TRY(hipMalloc(&scratch, total * sizeof(double)));
+    std::vector<sipgpu_contraction> tl(list, list + n);
+    for (auto &e : tl) e.d = scratch + dest_off[e.d];
+    hipError_t he = hipMemset(scratch, 0, total * sizeof(double));
+    std::vector<PlanChoice> cands;
+    if (nmoduli == 0) {
+        for (int cfg : {0, 1, 2, 3, 5, 6, 11, 12, 14}) {
+            PlanChoice ch;
+            ch.path = 0;
+            ch.cfg = cfg;
+            cands.push_back(ch);
+        }
+        for (int merge : {1, 0})
+            for (int tail : {1, 0}) {
+                PlanChoice ch;
+                ch.path = 1;
+                ch.merge = merge;
+                ch.tail = tail;
+                cands.push_back(ch);
+            }
+    } else {
+        for (int tile : {kOzBM, kOzBigTile})
+            for (int merge : {1, 0}) {
+                PlanChoice ch;
+                ch.path = 2;
+                ch.merge = merge;
+                ch.tile = tile;
+                cands.push_back(ch);
+            }
+    }
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (he == hipSuccess) he = hipEventCreate(&e0);
+    if (he == hipSuccess) he = hipEventCreate(&e1);
+    const hipStream_t hs = c.stream.s;
+    auto time_plan = [&](sipgpu_batch_s &b, double &ms) -> int {
+        if (int r = execute_batch(c, b, SIPGPU_STAGE_ALL, nullptr)) return r;
+        HIP_TRY(hipStreamSynchronize(hs));
+        ms = 1e300;
+        for (int rep = 0; rep < 3; ++rep) {
+            HIP_TRY(hipEventRecord(e0, hs));
+            if (int r = execute_batch(c, b, SIPGPU_STAGE_ALL, nullptr)) return r;
+            HIP_TRY(hipEventRecord(e1, hs));
+            HIP_TRY(hipEventSynchronize(e1));
+            float t = 0.f;
+            HIP_TRY(hipEventElapsedTime(&t, e0, e1));
+            ms = std::min(ms, (double)t);
+        }
+        return 0;
+    };
+    // the untuned plan
+    PlanChoice def;
+    double def_ms = 0.0;
+    if (he == hipSuccess) {
+        std::unique_ptr<sipgpu_batch_s> b;
+        int r = 0;
+        if (nmoduli == 0) {
+            r = plan_default_locked(c, tl.data(), n, b, &def);
+        } else {
+            def.path = 2;
+            r = plan_choice_locked(c, tl.data(), n, nmoduli, def, b);
+        }
+        if (r == 0) {
+            if (time_plan(*b, def_ms)) def_ms = 0.0;
+            (void)hipDeviceSynchronize();
+            free_batch_memory(*b);
+        }
+    }
+    PlanChoice best;
+    double best_ms = 1e300;
+    int timed = 0;
+    for (const PlanChoice &ch : cands) {
+        if (he != hipSuccess) break;
+        std::unique_ptr<sipgpu_batch_s> b;
+        if (plan_choice_locked(c, tl.data(), n, nmoduli, ch, b)) continue; // not plannable
+        double ms = 0.0;
+        const int r = time_plan(*b, ms);
+        (void)hipDeviceSynchronize();
+        free_batch_memory(*b);
+        if (r) {
+            rc = r;
+            break;
+        }
+        ++timed;
+        if (ms < best_ms) {
+            best_ms = ms;
+            best = ch;
+        }
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    (void)hipFree(scratch);
+    if (he != hipSuccess) return hip_fail(he, "tuning");
+    if (rc) return rc;
+    if (!timed) return fail(SIPGPU_EARG, "tuning: no candidate plan could be made");
+    tuned_table()[batch_signature(list, n, nmoduli)] = best;
+    std::unique_ptr<sipgpu_batch_s> b;
+    if ((rc = plan_choice_locked(c, list, n, nmoduli, best, b))) return rc;
+    *batch = b.release();
+    if (result) {
+        result->candidates = timed;
+        result->best_ms = best_ms;
+        result->default_ms = def_ms;
+        result->path = best.path;
+        result->config = best.cfg;
+        result->merge = best.merge;
+        result->tail = best.tail;
+        result->tile = best.tile;
+    }
+    return 0;
+}
+
+int sipgpu_batch_plan_choice(const sipgpu_contraction *list, int n, int nmoduli, int path,
+                             int config, int merge, int tail, int tile, sipgpu_batch_t *batch) {
+    if (int rc = sipgpu::rt::refuse_in_capture("batch planning")) return rc;
+    if (!batch || (n > 0 && !list) || n < 0) return fail(SIPGPU_EARG, "bad batch arguments");
+    if (path < 0 || path > 2 || (path == 2) != (nmoduli != 0) || config < -1 ||
+        config >= kGemmConfigs || (tile != -1 && tile != kOzBM && tile != kOzBigTile))
+        return fail(SIPGPU_EARG, "bad plan choice");
+    if (nmoduli != 0 && (nmoduli < 6 || nmoduli > 14))
+        return fail(SIPGPU_EARG, "split path: nmoduli must be in 6..14");
+    Context &c = ctx();
+    std::lock_guard<std::mutex> g(c.mu);
+    int rc = ensure_init(c);
+    if (rc) return rc;
+    PlanChoice ch;
+    ch.path = path;
+    ch.cfg = config;
+    ch.merge = merge;
+    ch.tail = tail;
+    ch.tile = tile;
+    std::unique_ptr<sipgpu_batch_s> b;
+    if ((rc = plan_choice_locked(c, list, n, nmoduli, ch, b))) return rc;
+    *batch = b.release();
+    return 0;
+}
+
+int sipgpu_tune_save(const char *path) {
+    if (!path) return fail(SIPGPU_EARG, "null path");
+    Context &c = ctx();
+    std::lock_guard<std::mutex> g(c.mu);
+    FILE *f = std::fopen(path, "w");
+    if (!f) return fail(SIPGPU_EARG, std::string("cannot write ") + path);
+    std::fprintf(f, "# sipgpu tuning table: signature path config merge tail tile\n");
+    for (const auto &kv : tuned_table())
+        std::fprintf(f, "%016llx %d %d %d %d %d\n", (unsigned long long)kv.first, kv.second.path,
+                     kv.second.cfg, kv.second.merge, kv.second.tail, kv.second.tile);
+    const bool ok = std::fclose(f) == 0;
+    return ok ? 0 : fail(SIPGPU_EARG, std::string("cannot write ") + path);
+}
+
+int sipgpu_tune_load(const char *path) {
+    if (!path) return fail(SIPGPU_EARG, "null path");
+    Context &c = ctx();
+    std::lock_guard<std::mutex> g(c.mu);
+    FILE *f = std::fopen(path, "r");
+    if (!f) return fail(SIPGPU_ENOTFOUND, std::string("cannot read ") + path);
+    char line[256];
+    int bad = 0;
+    while (std::fgets(line, sizeof line, f)) {
+        if (line[0] == '#' || line[0] == '\n') continue;
+        unsigned long long sig = 0;
+        PlanChoice ch;
+        if (std::sscanf(line, "%llx %d %d %d %d %d", &sig, &ch.path, &ch.cfg, &ch.merge, &ch.tail,
+                        &ch.tile) != 6 ||
+            ch.path < 0 || ch.path > 2 || ch.cfg < -1 || ch.cfg >= kGemmConfigs) {
+            ++bad;
+            continue;
+        }
+        tuned_table()[(uint64_t)sig] = ch;
+    }
+    std::fclose(f);
+    return bad ? fail(SIPGPU_EARG, std::string("malformed lines in ") + path) : 0;
+}
+
+int sipgpu_tune_clear(void) {
+    Context &c = ctx();
+    std::lock_guard<std::mutex> g(c.mu);
+    tuned_table().clear();
     return 0;
 }
 

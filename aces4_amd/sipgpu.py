@@ -42,7 +42,9 @@ EXPORTED = [
     "sipgpu_batch_info", "sipgpu_plan_tables", "sipgpu_plan_permute", "sipgpu_measure_peaks",
     "sipgpu_measured_clock_mhz", "sipgpu_batch_plan_split", "sipgpu_batch_split_info",
     "sipgpu_fill_random", "sipgpu_energy_denominator", "sipgpu_energy_sum", "sipgpu_batch_kind",
-    "sipgpu_batch_execute_stages",
+    "sipgpu_batch_execute_stages", "sipgpu_graph_begin", "sipgpu_graph_end",
+    "sipgpu_graph_launch", "sipgpu_graph_nodes", "sipgpu_graph_destroy", "sipgpu_batch_tune",
+    "sipgpu_batch_plan_choice", "sipgpu_tune_save", "sipgpu_tune_load", "sipgpu_tune_clear",
     "sipgpu_cache_create", "sipgpu_cache_destroy", "sipgpu_cache_insert", "sipgpu_cache_acquire",
     "sipgpu_cache_release", "sipgpu_cache_read", "sipgpu_cache_flush", "sipgpu_cache_evict",
     "sipgpu_cache_get_stats", "sipgpu_server_create", "sipgpu_server_submit",
@@ -74,6 +76,16 @@ class Request(ctypes.Structure):
                 ("ddims", ctypes.c_int * MAX_RANK), ("lkey", ctypes.c_ulonglong),
                 ("rkey", ctypes.c_ulonglong), ("dkey", ctypes.c_ulonglong),
                 ("accumulate", ctypes.c_int), ("moduli", ctypes.c_int)]
+
+
+class TuneResult(ctypes.Structure):
+    """sipgpu_tune_result of include/sipgpu.h."""
+    _fields_ = [("candidates", ctypes.c_int), ("best_ms", ctypes.c_double),
+                ("default_ms", ctypes.c_double), ("path", ctypes.c_int), ("config", ctypes.c_int),
+                ("merge", ctypes.c_int), ("tail", ctypes.c_int), ("tile", ctypes.c_int)]
+
+    def as_dict(self):
+        return {n: getattr(self, n) for n, _ in self._fields_}
 
 
 class CacheStats(ctypes.Structure):
@@ -133,6 +145,18 @@ def lib():
                                               ctypes.POINTER(ctypes.c_int)]
         L.sipgpu_batch_kind.argtypes = [_vp, ctypes.POINTER(ctypes.c_int)]
         L.sipgpu_batch_execute_stages.argtypes = [_vp, ctypes.c_int, _vp]
+        L.sipgpu_batch_tune.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
+                                        ctypes.POINTER(_vp), ctypes.POINTER(TuneResult)]
+        L.sipgpu_batch_plan_choice.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
+                                               ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                               ctypes.c_int, ctypes.c_int, ctypes.POINTER(_vp)]
+        L.sipgpu_tune_save.argtypes = [ctypes.c_char_p]
+        L.sipgpu_tune_load.argtypes = [ctypes.c_char_p]
+        L.sipgpu_graph_begin.argtypes = [_vp]
+        L.sipgpu_graph_end.argtypes = [_vp, ctypes.POINTER(_vp)]
+        L.sipgpu_graph_launch.argtypes = [_vp, _vp]
+        L.sipgpu_graph_nodes.argtypes = [_vp, ctypes.POINTER(ctypes.c_int)]
+        L.sipgpu_graph_destroy.argtypes = [_vp]
         L.sipgpu_batch_destroy.argtypes = [_vp]
         L.sipgpu_batch_info.argtypes = [_vp, ctypes.POINTER(ctypes.c_double),
                                         ctypes.POINTER(ctypes.c_int),
@@ -412,6 +436,53 @@ class Stream:
             self.handle = None
 
 
+class Graph:
+    """A captured sequence of library work, replayed by `launch`:
+
+        g = Graph.capture(lambda s: (batch.execute(s), ...), stream)
+        g.launch(stream)
+    """
+
+    def __init__(self, handle):
+        self.handle = handle
+        n = ctypes.c_int(0)
+        _check(lib().sipgpu_graph_nodes(self.handle, ctypes.byref(n)))
+        self.nodes = n.value
+
+    @staticmethod
+    def begin(stream=None):
+        _check(lib().sipgpu_graph_begin(stream.handle if stream else None))
+
+    @staticmethod
+    def end(stream=None):
+        p = _vp()
+        _check(lib().sipgpu_graph_end(stream.handle if stream else None, ctypes.byref(p)))
+        return Graph(p.value)
+
+    @staticmethod
+    def capture(fn, stream=None):
+        """Runs fn(stream) under capture; the capture is ended (and the
+        error re-raised) if fn raises."""
+        Graph.begin(stream)
+        try:
+            fn(stream)
+        except Exception:
+            try:
+                Graph.end(stream).destroy()
+            except SipGpuError:
+                pass
+            raise
+        return Graph.end(stream)
+
+    def launch(self, stream=None):
+        _check(lib().sipgpu_graph_launch(self.handle, stream.handle if stream else None))
+
+    def destroy(self):
+        if self.handle:
+            _check(lib().sipgpu_graph_destroy(self.handle))
+            self.handle = None
+
+
 class Event:
     def __init__(self):
         p = _vp()
@@ -464,16 +535,53 @@ def make_contraction(ptrn, l, ldims, r, rdims, d, ddims, accumulate=False):
 
 
 STAGE_PREPARE, STAGE_PRODUCT, STAGE_FINISH, STAGE_ALL = 1, 2, 4, 7
+PATHS = {0: "tables", 1: "packed", 2: "split"}
+
+
+def tune(contractions, moduli=None):
+    """Autotunes a batch (sipgpu_batch_tune): returns the fastest plan as a
+    Batch and the tuning result as a dict."""
+    arr = (Contraction * len(contractions))(*contractions)
+    p = _vp()
+    res = TuneResult()
+    _check(lib().sipgpu_batch_tune(arr, len(contractions), int(moduli or 0), ctypes.byref(p),
+                                   ctypes.byref(res)))
+    return Batch(contractions, moduli=moduli, _handle=p.value), res.as_dict()
+
+
+def plan_choice(contractions, path, moduli=None, config=-1, merge=-1, tail=-1, tile=-1):
+    """A batch planned one given way (sipgpu_batch_plan_choice)."""
+    arr = (Contraction * len(contractions))(*contractions)
+    p = _vp()
+    code = {v: k for k, v in PATHS.items()}[path]
+    _check(lib().sipgpu_batch_plan_choice(arr, len(contractions), int(moduli or 0), code,
+                                          int(config), int(merge), int(tail), int(tile),
+                                          ctypes.byref(p)))
+    return Batch(contractions, moduli=moduli, _handle=p.value)
+
+
+def tune_save(path):
+    _check(lib().sipgpu_tune_save(str(path).encode()))
+
+
+def tune_load(path):
+    _check(lib().sipgpu_tune_load(str(path).encode()))
+
+
+def tune_clear():
+    _check(lib().sipgpu_tune_clear())
 
 
 class Batch:
     """A planned batch of contractions: one grouped FP64-MFMA launch, or with
     `moduli` set, the split-precision (INT8 / CRT) path over that many moduli."""
 
-    def __init__(self, contractions, moduli=None):
+    def __init__(self, contractions, moduli=None, _handle=None):
         arr = (Contraction * len(contractions))(*contractions)
         p = _vp()
-        if moduli is None:
+        if _handle is not None:
+            p.value = _handle
+        elif moduli is None:
             _check(lib().sipgpu_batch_plan(arr, len(contractions), ctypes.byref(p)))
         else:
             _check(lib().sipgpu_batch_plan_split(arr, len(contractions), int(moduli),

@@ -159,17 +159,22 @@ def ladder(o_segs=(10,), v_segs=(10, 10, 10, 10), seed=1):
                 labels=("abij", "abcd", "cdij"))
 
 
-def batch_for(w, accumulate=True, moduli=None):
-    """A planned device batch for workload `w` (arrays uploaded on demand);
-    `moduli` selects the split-precision path."""
+def contractions_for(w, accumulate=True):
+    """The workload's block contractions on its device arrays (uploaded on
+    demand)."""
     for k in ("D", "L", "R"):
         if w[k].dev is None:
             w[k].to_device()
     D, L, R, ptrn = w["D"], w["L"], w["R"], w["ptrn"]
-    cs = [sg.make_contraction(ptrn, L.addr(lb), L.dims(lb), R.addr(rb), R.dims(rb), D.addr(db),
-                              D.dims(db), accumulate=accumulate)
-          for db, lb, rb in w["entries"]]
-    return sg.Batch(cs, moduli=moduli)
+    return [sg.make_contraction(ptrn, L.addr(lb), L.dims(lb), R.addr(rb), R.dims(rb), D.addr(db),
+                                D.dims(db), accumulate=accumulate)
+            for db, lb, rb in w["entries"]]
+
+
+def batch_for(w, accumulate=True, moduli=None):
+    """A planned device batch for workload `w` (arrays uploaded on demand);
+    `moduli` selects the split-precision path."""
+    return sg.Batch(contractions_for(w, accumulate), moduli=moduli)
 
 
 def flops(w):

@@ -47,6 +47,7 @@ def parse():
     p.add_argument("--no-sparse", action="store_true")
     p.add_argument("--no-serve", action="store_true")
     p.add_argument("--no-triples", action="store_true")
+    p.add_argument("--no-graphs", action="store_true")
     return p.parse_args()
 
 
@@ -283,6 +284,63 @@ def serving_bench(label, budget_gb, n, rate, seg_lo, seg_hi, max_batch=256, modu
     }
 
 
+def graph_bench(reps=20):
+    """CC iterations replayed as graphs against the same work enqueued
+    eagerly: the ladder of the CPU-runnable configuration (o = 10, v = 40,
+    256 block pairs of 100 x 100 x 100, one grouped launch, plus the energy
+    sum over its result), and one triples tuple (twelve batches and the
+    fused energy).  Per-iteration time between events on the launch stream,
+    averaged over `reps` back-to-back iterations."""
+    from aces4_amd import triples as tp
+    s = sg.Stream()
+    res = {}
+
+    def timed(fn):
+        fn(s)
+        s.synchronize()
+        e0, e1 = sg.Event(), sg.Event()
+        e0.record(s)
+        for _ in range(reps):
+            fn(s)
+        e1.record(s)
+        e1.synchronize()
+        return e0.elapsed_ms(e1) / reps
+
+    def compare(name, it, flops):
+        eager = timed(it)
+        g = sg.Graph.capture(it, s)
+        replay = timed(lambda st: g.launch(st))
+        res[name] = {"eager_ms": eager, "graph_ms": replay, "graph_nodes": g.nodes,
+                     "tflops_graph": flops / (replay * 1e-3) / 1e12, "speedup": eager / replay}
+        g.destroy()
+
+    w = wl.ladder()
+    b = wl.batch_for(w)
+    D = w["D"]
+    out = sg.DeviceArray(1)
+    nd = len(D.host)
+    eps = sg.DeviceArray.from_host(np.linspace(0.5, 1.5, nd))  # one weight per element
+
+    def ladder_it(st):
+        b.execute(st)
+        sg.energy_sum([nd], D.dev.ptr, D.dev.ptr, [eps.ptr], [1], 1.0, out.ptr, s=st)
+
+    compare("ladder_iteration", ladder_it, b.flops)
+    b.destroy()
+    w3 = tp.triples()
+    tp.to_device(w3)
+    t = (0, 1, 2, 0, 0, 0)
+    wbuf = sg.DeviceArray(int(np.prod(tp.tuple_dims(w3, t))))
+    te = tp.TupleEnergy(w3, t, wbuf.ptr, out.ptr)
+    compare("triples_tuple", lambda st: te.execute(st), te.flops)
+    te.destroy()
+    wbuf.free()
+    for k in ("V1", "V2", "T"):
+        w3[k].dev.free()
+    s.destroy()
+    return res
+
+
 def triples_bench(tuples=((0, 1, 2, 0, 0, 0), (3, 4, 5, 0, 0, 0), (6, 7, 8, 0, 0, 0)),
                   reps=2, moduli=(None, 14)):
     """Triples (o = 30 in segments 20,10; v = 240 in 12 x 20): full-size W
@@ -336,6 +394,25 @@ def triples_bench(tuples=((0, 1, 2, 0, 0, 0), (3, 4, 5, 0, 0, 0), (6, 7, 8, 0, 0
     return res
 
 
+def tune_summary(res):
+    out = dict(res)
+    out["path"] = sg.PATHS[res["path"]]
+    out["speedup_vs_untuned"] = res["default_ms"] / res["best_ms"] if res["best_ms"] else None
+    return out
+
+
+def tune_report(w):
+    """Tunes workload w's batch and reports the choice and both timings."""
+    b, res = sg.tune(wl.contractions_for(w))
+    out = tune_summary(res)
+    out["flop"] = b.flops
+    out["tflops_tuned"] = b.flops / (res["best_ms"] * 1e-3) / 1e12
+    b.destroy()
+    for k in ("D", "L", "R"):
+        w[k].dev.free()
+    return out
+
+
 GEMM_KERNEL = {"tables": "gemm_f64_kernel", "packed": "gemm_f64_packed_kernel"}
 PATH_NAME = {"tables": "plain FP64 MFMA, grouped batch, permutations fused into the GEMM loads",
              "packed": "plain FP64 MFMA, operands packed into K-contiguous panels once per "
@@ -349,8 +426,14 @@ def main():
               "MI355X (multi-GPU sharding is a design note only).", file=sys.stderr)
         sys.exit(2)
     sg.init(0)
+    # the ladder of the CPU-runnable configuration, tuned first: its choice
+    # is remembered for the graph leg
+    tuning = {"ladder_cfg1": tune_report(wl.ladder())}
     w = wl.ring()
-    batch = wl.batch_for(w)
+    # the headline batch is planned by the autotuner (a planning step, not
+    # part of the timed region)
+    batch, tres = sg.tune(wl.contractions_for(w))
+    tuning["ring"] = tune_summary(tres)
     flops = batch.flops
     stream = None
     for _ in range(a.warmup):
@@ -400,6 +483,7 @@ def main():
                                  2000.0, 8, 64, moduli=14),
                    serving_bench("LRU spill (budget 60% of working set)", 8.5, 1500, 4000.0,
                                  8, 40)]
+    graphs = None if a.no_graphs else graph_bench()
     lat = latency_sample(w, a.latency_samples) if a.latency_samples > 0 else None
     perm = permute_bench()
     cores, model = cpu_info()
@@ -442,6 +526,8 @@ def main():
         "latency": lat,
         "serving": serving,
         "triples": trip,
+        "graphs": graphs,
+        "tuning": tuning,
         "cpu_baseline": cpu,
     }
     print(json.dumps(res))

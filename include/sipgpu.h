@@ -83,7 +83,8 @@ enum {
     SIPGPU_EARG = -107,       /* invalid argument (null pointer, bad handle) */
     SIPGPU_EHAZARD = -108,    /* batch writes a block it also reads */
     SIPGPU_EBUDGET = -109,    /* block cache budget cannot hold the request */
-    SIPGPU_ENOTFOUND = -110   /* block id not present in the cache */
+    SIPGPU_ENOTFOUND = -110,  /* block id not present in the cache */
+    SIPGPU_ECAPTURE = -111    /* not allowed while the stream is being captured */
 };
 
 typedef struct sipgpu_stream_s *sipgpu_stream_t;   /* wraps a hipStream_t */
@@ -191,6 +192,36 @@ int sipgpu_batch_kind(sipgpu_batch_t batch, int *kind);
 enum { SIPGPU_STAGE_PREPARE = 1, SIPGPU_STAGE_PRODUCT = 2, SIPGPU_STAGE_FINISH = 4,
        SIPGPU_STAGE_ALL = 7 };
 int sipgpu_batch_execute_stages(sipgpu_batch_t batch, int stages, sipgpu_stream_t s);
+/* Autotuner (no reference counterpart).  Plans the batch every candidate
+ * way -- the table kernel in each of its tile configurations, packed FP64
+ * panels with and without merged result blocks and with and without the
+ * persistent water-filled schedule; for the split path (nmoduli > 0) both
+ * GEMM tiles with and without merging -- times each (one warm-up and three
+ * timed executions on the library stream, into scratch copies of the
+ * destinations, so the caller's blocks are not touched) and returns the
+ * fastest plan.  The choice is remembered under the batch's signature (its
+ * shapes, patterns, accumulate flags and which entries share blocks, not the
+ * addresses), so later sipgpu_batch_plan / _plan_split calls on batches with
+ * that signature plan the same way.  Synchronises the device. */
+typedef struct {
+    int candidates;     /* plans timed */
+    double best_ms;     /* per execution, the chosen plan */
+    double default_ms;  /* per execution, the untuned plan (0 if it failed) */
+    int path;           /* 0 table kernel, 1 packed FP64, 2 split precision */
+    int config, merge, tail, tile;
+} sipgpu_tune_result;
+int sipgpu_batch_tune(const sipgpu_contraction *list, int n, int nmoduli, sipgpu_batch_t *batch,
+                      sipgpu_tune_result *result);
+/* Plans the batch one given way (the fields of sipgpu_tune_result: path,
+ * config for the table kernel, merge / tail / tile for the panel paths, -1
+ * for a default): pins a plan, e.g. one found by tuning elsewhere. */
+int sipgpu_batch_plan_choice(const sipgpu_contraction *list, int n, int nmoduli, int path,
+                             int config, int merge, int tail, int tile, sipgpu_batch_t *batch);
+/* The tuning table as text (one "signature path config merge tail tile" line
+ * per entry): save writes it, load merges a file into it, clear empties it. */
+int sipgpu_tune_save(const char *path);
+int sipgpu_tune_load(const char *path);
+int sipgpu_tune_clear(void);
 /* Sum of 2*M*N*K over the batch, and its task / tile / pair counts. */
 int sipgpu_batch_info(sipgpu_batch_t batch, double *flops, int *ntasks,
                       int *ntiles, int *npairs);
@@ -307,6 +338,25 @@ int sipgpu_server_replay(sipgpu_server_t server, const sipgpu_request *reqs,
                          const double *arrival_s, int n, double *latency_ms, double *wall_s);
 int sipgpu_server_get_stats(sipgpu_server_t server, sipgpu_server_stats *stats);
 int sipgpu_server_destroy(sipgpu_server_t server);
+
+/* ------------------------------------------------------------------------
+ * Graphs of CC iterations (no reference counterpart: the SIP re-issues every
+ * super-instruction each iteration).  Everything the library enqueues on `s`
+ * between begin and end -- batch executions and their stages, single
+ * contractions and permutes of device blocks, energy denominators and sums,
+ * seeded fills -- is captured into one graph and replayed by launch with
+ * the same pointers and sizes; the blocks' contents may change between
+ * replays.  Batches must be planned and single-contraction shapes run once
+ * before capture: planning during capture fails with SIPGPU_ECAPTURE and
+ * leaves the capture intact.  The captured work clears memory with kernels
+ * only (no memset nodes). */
+typedef struct sipgpu_graph_s *sipgpu_graph_t;
+int sipgpu_graph_begin(sipgpu_stream_t s);
+int sipgpu_graph_end(sipgpu_stream_t s, sipgpu_graph_t *graph);
+int sipgpu_graph_launch(sipgpu_graph_t graph, sipgpu_stream_t s);
+/* Number of kernel / copy nodes in the graph. */
+int sipgpu_graph_nodes(sipgpu_graph_t graph, int *nodes);
+int sipgpu_graph_destroy(sipgpu_graph_t graph);
 
 /* Seeded synthetic data written in HBM, enqueued on `s`: n doubles of the
  * counter-based SplitMix64 sequence (seed, stream).  dist 0: uniform in

@@ -1,0 +1,118 @@
+"""GPU tests of CC iterations captured as graphs.
+
+One iteration: a packed FP64 batch, a table-kernel batch, a split-precision
+batch, a single contraction, a permute, a seeded fill and an energy sum, all
+on one stream.  It is run eagerly once, then captured and replayed three
+times; before every replay the blocks are reset from host memory and other
+eager work runs on the same stream, and every replay must reproduce the
+eager results bit for bit (the same kernels, the same fixed summation
+orders).  Planning, allocation and host copies inside a capture fail with
+SIPGPU_ECAPTURE and leave the capture usable.
+"""
+import numpy as np
+import pytest
+
+from aces4_amd import sipgpu as sg
+from aces4_amd import workloads as wl
+
+pytestmark = pytest.mark.gpu
+
+
+def test_graph_replays_match_eager(monkeypatch):
+    s = sg.Stream()
+    w1 = wl.ring(o_segs=(9, 6), v_segs=(14, 11, 8))
+    w2 = wl.ladder()
+    w3 = wl.ring(o_segs=(5, 4), v_segs=(10, 7), seed=7)
+    rng = np.random.default_rng(3)
+    for w in (w1, w2, w3):
+        w["D"].host[:] = rng.standard_normal(w["D"].host.size)
+    monkeypatch.setenv("SIPGPU_FP64_PATH", "packed")
+    b_packed = wl.batch_for(w1)
+    monkeypatch.setenv("SIPGPU_FP64_PATH", "tables")
+    b_tab = wl.batch_for(w2)
+    monkeypatch.delenv("SIPGPU_FP64_PATH")
+    b_split = wl.batch_for(w3, moduli=13)
+    assert (b_packed.kind, b_tab.kind, b_split.kind) == ("packed", "tables", "split")
+    # a single contraction, a permute, a fill and an energy sum on their own buffers
+    n = 24
+    A = sg.DeviceArray.from_host(rng.standard_normal(n * n))
+    B = sg.DeviceArray.from_host(rng.standard_normal(n * n))
+    C = sg.DeviceArray(n * n)
+    db0 = next(iter(w1["D"].block_ids()))
+    dims0 = [int(x) for x in w1["D"].dims(db0)]
+    P = sg.DeviceArray(int(np.prod(dims0)))
+    F = sg.DeviceArray(1000)
+    eps = [sg.DeviceArray.from_host(rng.uniform(0.5, 2.0, d)) for d in dims0]
+    out = sg.DeviceArray(1)
+    ptrn = [1, -1, -2, 2]  # D(i,j) = L(i,k) R(k,j)
+
+    def iteration(st):
+        b_packed.execute(st)
+        b_tab.execute(st)
+        b_split.execute(st)
+        sg.contract(ptrn, A.ptr, [n, n], B.ptr, [n, n], C.ptr, [n, n], stream=st)
+        sg.permute(dims0, [1, 4, 3, 2, 1], w1["D"].addr(db0), P.ptr, stream=st)
+        sg.fill_random(F.ptr, 1000, 11, 3, "normal", 0.5, s=st)
+        sg.energy_sum(dims0, w1["D"].addr(db0), P.ptr, [e.ptr for e in eps], [1, -1, 1, -1],
+                      0.5, out.ptr, s=st)
+
+    def reset():
+        for w in (w1, w2, w3):
+            w["D"].dev.upload(w["D"].host)
+        C.zero()
+        out.zero()
+
+    def results():
+        return [w1["D"].dev.download(), w2["D"].dev.download(), w3["D"].dev.download(),
+                C.download(), P.download(), F.download(), out.download()]
+
+    reset()
+    iteration(s)
+    s.synchronize()
+    want = results()
+    g = sg.Graph.capture(iteration, s)
+    assert g.nodes >= 9
+    for rep in range(3):
+        reset()
+        b_tab.execute(s)  # eager work on the stream between replays
+        s.synchronize()
+        reset()
+        g.launch(s)
+        s.synchronize()
+        got = results()
+        for k, (x, y) in enumerate(zip(got, want)):
+            assert np.array_equal(x, y), (rep, k)
+    g.destroy()
+    for b in (b_packed, b_tab, b_split):
+        b.destroy()
+    s.destroy()
+
+
+def test_capture_refuses_planning_and_host_copies():
+    s = sg.Stream()
+    w = wl.ring(o_segs=(5,), v_segs=(6, 4))
+    b = wl.batch_for(w)
+    x = sg.DeviceArray(16)
+    sg.Graph.begin(s)
+    b.execute(s)
+    for call in (lambda: wl.batch_for(w), lambda: x.zero(), lambda: x.download(),
+                 lambda: sg.contract([1, -1, -2, 2], x.ptr, [3, 5], x.ptr, [5, 3], x.ptr, [3, 3],
+                                     stream=s)):
+        with pytest.raises(sg.SipGpuError) as ei:
+            call()
+        assert ei.value.code == -111
+    g = sg.Graph.end(s)
+    assert g.nodes >= 1
+    with pytest.raises(sg.SipGpuError):
+        sg.Graph.end(s)  # not capturing any more
+    w["D"].dev.zero()
+    g.launch(s)
+    s.synchronize()
+    ref = w["D"].dev.download()
+    w["D"].dev.zero()
+    b.execute(s)
+    s.synchronize()
+    assert np.array_equal(w["D"].dev.download(), ref)
+    g.destroy()
+    b.destroy()
+    s.destroy()

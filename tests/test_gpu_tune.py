@@ -1,0 +1,119 @@
+"""GPU tests of the autotuner and of explicitly chosen plans.
+
+Every plan the tuner may pick is checked against the oracle first: the
+table kernel in each tile configuration the tuner times, and the packed
+panels with and without merged result blocks and the persistent schedule.
+The tuner must leave the caller's blocks untouched, return a plan that
+matches the oracle, remember its choice by batch signature (also through a
+saved and reloaded table) and never pick a plan slower than the untuned one
+beyond timing noise."""
+import numpy as np
+import pytest
+
+from test_gpu_batch import oracle_batch, per_block_err
+
+from aces4_amd import sipgpu as sg
+from aces4_amd import workloads as wl
+
+pytestmark = pytest.mark.gpu
+TUNER_CONFIGS = [0, 1, 2, 3, 5, 6, 11, 12, 14]
+MAKERS = {"ladder": lambda: wl.ladder(),
+          "ring-uneven": lambda: wl.ring(o_segs=(9, 6), v_segs=(14, 11, 8))}
+
+
+def contractions(w):
+    D, L, R, ptrn = w["D"], w["L"], w["R"], w["ptrn"]
+    for k in ("D", "L", "R"):
+        if w[k].dev is None:
+            w[k].to_device()
+    return [sg.make_contraction(ptrn, L.addr(lb), L.dims(lb), R.addr(rb), R.dims(rb), D.addr(db),
+                                D.dims(db), accumulate=True) for db, lb, rb in w["entries"]]
+
+
+def prefill(w, seed=5):
+    rng = np.random.default_rng(seed)
+    w["D"].host[:] = rng.standard_normal(w["D"].host.size)
+    w["D"].dev = None
+
+
+@pytest.mark.parametrize("make", list(MAKERS), ids=list(MAKERS))
+@pytest.mark.parametrize("cfg", TUNER_CONFIGS)
+def test_table_config_vs_oracle(make, cfg):
+    w = MAKERS[make]()
+    prefill(w)
+    print("tables", make, cfg, flush=True)
+    b = sg.plan_choice(contractions(w), "tables", config=cfg)
+    assert b.kind == "tables"
+    b.execute()
+    sg.synchronize()
+    assert per_block_err(w, w["D"].dev.download(), oracle_batch(w)) <= 1e-13
+    b.destroy()
+
+
+@pytest.mark.parametrize("make", list(MAKERS), ids=list(MAKERS))
+@pytest.mark.parametrize("merge,tail", [(1, 1), (1, 0), (0, 1), (0, 0)])
+def test_packed_choice_vs_oracle(make, merge, tail):
+    w = MAKERS[make]()
+    prefill(w)
+    b = sg.plan_choice(contractions(w), "packed", merge=merge, tail=tail)
+    assert b.kind == "packed"
+    b.execute()
+    sg.synchronize()
+    assert per_block_err(w, w["D"].dev.download(), oracle_batch(w)) <= 1e-13
+    b.destroy()
+
+
+def test_tuner_picks_a_plan_and_remembers_it(tmp_path):
+    sg.tune_clear()
+    w = wl.ladder()
+    prefill(w)
+    cs = contractions(w)
+    before = w["D"].dev.download()
+    b, res = sg.tune(cs)
+    print(res, flush=True)
+    assert np.array_equal(w["D"].dev.download(), before)  # tuning ran on scratch blocks
+    assert res["candidates"] >= 10
+    assert res["default_ms"] > 0 and res["best_ms"] <= 1.10 * res["default_ms"]
+    kind = sg.PATHS[res["path"]]
+    assert b.kind == kind
+    b.execute()
+    sg.synchronize()
+    assert per_block_err(w, w["D"].dev.download(), oracle_batch(w)) <= 1e-13
+    b.destroy()
+    # the choice is remembered by signature: the same batch shape on other blocks
+    w2 = wl.ladder(seed=9)
+    prefill(w2, seed=6)
+    assert sg.Batch(contractions(w2)).kind == kind
+    path = tmp_path / "tune.txt"
+    sg.tune_save(path)
+    sg.tune_clear()
+    sg.tune_load(path)
+    b2 = sg.Batch(contractions(w2))
+    assert b2.kind == kind
+    b2.execute()
+    sg.synchronize()
+    assert per_block_err(w2, w2["D"].dev.download(), oracle_batch(w2)) <= 1e-13
+    b2.destroy()
+    sg.tune_clear()
+
+
+def test_tuner_split_path():
+    sg.tune_clear()
+    w = wl.ring(o_segs=(9, 6), v_segs=(14, 11, 8))
+    prefill(w)
+    b, res = sg.tune(contractions(w), moduli=13)
+    assert res["path"] == 2 and b.kind == "split" and res["tile"] in (128, 256)
+    b.execute()
+    sg.synchronize()
+    assert per_block_err(w, w["D"].dev.download(), oracle_batch(w)) <= 1e-12
+    b.destroy()
+    sg.tune_clear()
+
+
+def test_tune_load_rejects_malformed(tmp_path):
+    p = tmp_path / "bad.txt"
+    p.write_text("# header\nzz 1 2\n")
+    with pytest.raises(sg.SipGpuError):
+        sg.tune_load(p)
+    with pytest.raises(sg.SipGpuError):
+        sg.tune_load(tmp_path / "missing.txt")
