@@ -327,10 +327,9 @@ struct TablePool {
 // Static XCD schedule of a grouped launch.  Slot t of the launch runs on
 // XCD t % 8, so the tile list is laid out XCD-major.  Whole tasks go to one
 // XCD (their tiles then share its L2), heaviest first onto the least loaded
-// XCD; a task with more than 1/16 of all tiles is first cut into contiguous
-// tile ranges (column-major over its tiles, so a range shares operand
-// panels) of at most that size, so a batch of few large tasks still spans
-// the chip.  Empty slots are task -1.
+// XCD; a task with more than 1/16 of all tiles is first cut into rectangular
+// blocks of tiles of at most that size, so a batch of few large tasks still
+// spans the chip.  Empty slots are task -1.
 template <class Tile>
 std::vector<std::vector<Tile>> xcd_lists(const std::vector<int> &tm, const std::vector<int> &tn,
                                          const std::vector<double> &cost, int &real_tiles) {
@@ -338,16 +337,27 @@ std::vector<std::vector<Tile>> xcd_lists(const std::vector<int> &tm, const std::
     long long total = 0;
     for (int t = 0; t < ntasks; ++t) total += (long long)tm[t] * tn[t];
     const long long cap = std::max<long long>(1, (total + 2 * kXcds - 1) / (2 * kXcds));
-    struct Unit { int task; long long first, count; double cost; };
+    // A unit is a rectangle of one task's tiles: the whole task, or for a
+    // task above `cap` tiles one block of a pr x pc cut chosen as square as
+    // possible, so the tiles an XCD runs together share few operand panels
+    // (a 4500 x 4500 grid cut in 9 x 9-tile blocks: 18 panels per 81 tiles
+    // instead of 38 for a column-major range).
+    struct Unit { int task, m0, m1, n0, n1; double cost; };
     std::vector<Unit> units;
     for (int t = 0; t < ntasks; ++t) {
         const long long nt = (long long)tm[t] * tn[t];
         const long long pieces = std::max<long long>(1, (nt + cap - 1) / cap);
-        const long long step = (nt + pieces - 1) / pieces;
-        for (long long f = 0; f < nt; f += step) {
-            const long long c = std::min(step, nt - f);
-            units.push_back(Unit{t, f, c, cost[t] * (double)c / (double)std::max<long long>(nt, 1)});
-        }
+        int pr = (int)std::lround(std::sqrt((double)pieces * tm[t] / std::max(tn[t], 1)));
+        pr = std::max(1, std::min(pr, tm[t]));
+        int pc = (int)std::min<long long>(tn[t], (pieces + pr - 1) / pr);
+        pc = std::max(pc, 1);
+        const int bm = (tm[t] + pr - 1) / pr, bn = (tn[t] + pc - 1) / pc;
+        for (int n0 = 0; n0 < tn[t]; n0 += bn)
+            for (int m0 = 0; m0 < tm[t]; m0 += bm) {
+                const int m1 = std::min(tm[t], m0 + bm), n1 = std::min(tn[t], n0 + bn);
+                const double c = cost[t] * (double)(m1 - m0) * (n1 - n0) / (double)std::max<long long>(nt, 1);
+                units.push_back(Unit{t, m0, m1, n0, n1, c});
+            }
     }
     std::stable_sort(units.begin(), units.end(),
                      [](const Unit &a, const Unit &b) { return a.cost > b.cost; });
@@ -356,12 +366,16 @@ std::vector<std::vector<Tile>> xcd_lists(const std::vector<int> &tm, const std::
     real_tiles = 0;
     for (const Unit &u : units) {
         const int x = (int)(std::min_element(load.begin(), load.end()) - load.begin());
-        for (long long i = u.first; i < u.first + u.count; ++i) {
-            const int q = (int)(i / tm[u.task]), p = (int)(i % tm[u.task]);
-            per_xcd[x].push_back(Tile{u.task, p, q, 0});
-        }
+        for (int q = u.n0; q < u.n1; ++q)
+            for (int p = u.m0; p < u.m1; ++p) {
+                Tile tl{};
+                tl.task = u.task;
+                tl.mt = p;
+                tl.nt = q;
+                per_xcd[x].push_back(tl);
+            }
         load[x] += u.cost;
-        real_tiles += (int)u.count;
+        real_tiles += (u.m1 - u.m0) * (u.n1 - u.n0);
     }
     return per_xcd;
 }
