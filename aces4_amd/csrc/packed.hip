@@ -65,13 +65,21 @@ __global__ __launch_bounds__(256) void pack_kernel(OzConvArgs a) {
             for (int j = 0; j < 16; ++j) S[(tid >> 7) + 2 * j][tid & 127] = v[j];
         }
         __syncthreads();
-        const int xx = tid >> 3, kc = (tid & 7) * 16; // 8 lanes: one 1 KB row piece
-        const int x = w.x0 + xx, k = w.k0 + kc;
-        if (x >= b.X || k >= b.Kp) continue;
-        double *dst = reinterpret_cast<double *>(a.planes + b.panel_off) + (long long)x * b.Kp + k;
-        const int n = min(16, b.Kp - k); // Kp is even: whole 16-byte pairs
-        for (int j = 0; j < n; j += 2)
-            *reinterpret_cast<f64x2 *>(dst + j) = f64x2{S[xx][kc + j], S[xx][kc + j + 1]};
+        // 8 lanes per row; at store j lane l writes the pair k = 2 (8 j + l):
+        // a row's 8 lanes cover 128 contiguous bytes of the panel, and the
+        // LDS reads of 16 consecutive lanes (two rows) fall in distinct banks
+        const int xx = tid >> 3, l = tid & 7;
+        const int x = w.x0 + xx;
+        if (x < b.X) {
+            double *dst =
+                reinterpret_cast<double *>(a.planes + b.panel_off) + (long long)x * b.Kp + w.k0;
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                const int k = 2 * (8 * j + l);
+                if (w.k0 + k < b.Kp) // Kp is even: whole 16-byte pairs
+                    *reinterpret_cast<f64x2 *>(dst + k) = f64x2{S[xx][k], S[xx][k + 1]};
+            }
+        }
     }
 }
 

@@ -187,7 +187,7 @@ def split_bench(w, fp64_batch, reps=20, moduli=(14, 13)):
     ref = D.dev.download()
     out = {}
     for nm in moduli:
-        b = wl.batch_for(w, moduli=nm)
+        b, tres = sg.tune(wl.contractions_for(w), moduli=nm)  # planned by the autotuner
         D.dev.upload(zeros)
         b.execute()
         sg.synchronize()
@@ -204,7 +204,9 @@ def split_bench(w, fp64_batch, reps=20, moduli=(14, 13)):
         ms = e0.elapsed_ms(e1) / reps
         out["moduli_%d" % nm] = {"moduli": nm, "bits": b.bits, "ms": ms,
                                  "tflops_fp64_equiv": b.flops / (ms * 1e-3) / 1e12,
-                                 "max_block_rel_err_vs_fp64": err}
+                                 "max_block_rel_err_vs_fp64": err,
+                                 "tuned": {"tile": tres["tile"], "merge": tres["merge"],
+                                           "untuned_ms": tres["default_ms"]}}
         b.destroy()
     return out
 
