@@ -436,16 +436,28 @@ __global__ __launch_bounds__(256) void reconstruct_kernel(OzRecArgs a) {
         for (int e0 = w.e0 + 4 * (int)threadIdx.x; e0 < e1; e0 += 4 * (int)blockDim.x) {
             U128 acc[4] = {{0ull, 0ull}, {0ull, 0ull}, {0ull, 0ull}, {0ull, 0ull}};
             double s[4] = {0.0, 0.0, 0.0, 0.0};
-            for (int i = 0; i < a.c.n; ++i) {
-                const uint32_t t4 =
-                    *reinterpret_cast<const uint32_t *>(res + (long long)i * a.res_plane_bytes + e0);
-                const double rp = a.c.rp[i];
-                const uint64_t mi[2] = {a.c.mi[i][0], a.c.mi[i][1]};
+            // moduli four at a time: the four planes' words are loaded
+            // before any is used, so their latencies overlap
+            for (int i0 = 0; i0 < a.c.n; i0 += 4) {
+                uint32_t w4[4];
 #pragma unroll
-                for (int q = 0; q < 4; ++q) {
-                    const unsigned t = (t4 >> (8 * q)) & 0xffu;
-                    s[q] += (double)t * rp;
-                    mad_small(acc[q], (unsigned long long)t, mi);
+                for (int d = 0; d < 4; ++d)
+                    w4[d] = i0 + d < a.c.n
+                                ? *reinterpret_cast<const uint32_t *>(
+                                      res + (long long)(i0 + d) * a.res_plane_bytes + e0)
+                                : 0u;
+#pragma unroll
+                for (int d = 0; d < 4; ++d) {
+                    if (i0 + d >= a.c.n) break;
+                    const int i = i0 + d;
+                    const double rp = a.c.rp[i];
+                    const uint64_t mi[2] = {a.c.mi[i][0], a.c.mi[i][1]};
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) {
+                        const unsigned t = (w4[d] >> (8 * q)) & 0xffu;
+                        s[q] += (double)t * rp;
+                        mad_small(acc[q], (unsigned long long)t, mi);
+                    }
                 }
             }
 #pragma unroll

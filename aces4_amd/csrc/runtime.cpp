@@ -879,7 +879,11 @@ int sipgpu::rt::build_panel_plan(const sipgpu_contraction *list, int n, int nmod
             const int i = idx[j];
             const int si = shape_of[i];
             const ContractionShape &sh = *shapes[si];
-            if (sh.M != s0.M || sh.N != s0.N || sh.moffD != s0.moffD || sh.noffD != s0.noffD)
+            // rows and columns are enumerated in destination order, so entries
+            // agree when they split the block's indices into the same row and
+            // column sets, whatever order their operands hold them in
+            if (sh.M != s0.M || sh.N != s0.N || tabs[si][4] != tabs[si0][4] ||
+                tabs[si][5] != tabs[si0][5])
                 return fail(SIPGPU_EARG, "entries writing one block disagree on its layout");
             d.a.push_back(instance(list[i].l, si, 0));
             d.b.push_back(instance(list[i].r, si, 1));
@@ -1472,7 +1476,16 @@ int plan_default_locked(Context &c, const sipgpu_contraction *list, int n,
     std::unique_ptr<sipgpu_batch_s> b;
     PlanChoice tab;
     tab.path = 0;
-    if (int rc = plan_choice_locked(c, list, n, 0, tab, b)) return rc;
+    if (int rc = plan_choice_locked(c, list, n, 0, tab, b)) {
+        // the table kernel takes one pattern per result block; a batch that
+        // writes a block through several patterns runs on panels
+        PlanChoice pk;
+        pk.path = 1;
+        if (plan_choice_locked(c, list, n, 0, pk, b)) return rc;
+        if (chosen) *chosen = pk;
+        out = std::move(b);
+        return 0;
+    }
     bool packed = b->ntasks > 0 && b->flops >= 1e9 && b->flops / b->ntasks >= 5e7;
     if (const char *e = std::getenv("SIPGPU_FP64_PATH")) {
         if (!std::strcmp(e, "tables")) packed = false;

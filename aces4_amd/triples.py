@@ -14,8 +14,10 @@ accumulated from contractions, energy_denominator_rhf applied to the block
 (energy_denominator_rhf.F:399-444), then contracted with itself to a scalar
 (rccsdpt_aaa.sialx:2164-2174).
 
-On the device every (permutation, term) pair is one planned batch over the
-d (or l) segments: the twelve batches accumulate into one W buffer in
+On the device the twelve terms run as six batches: term 1 under a pair
+permutation and term 2 under the reversed one split W's indices into the
+same row and column sets, so each such pair is one GEMM task over the d and
+l segments together.  The six batches accumulate into one W buffer in
 stream order (each has its own row / column grouping of W's six indices, so
 they cannot share a launch), then one fused pass divides by D, squares and
 reduces, without writing the divided block.
@@ -40,6 +42,21 @@ def term_labels():
         (x1, y1), (x2, y2), (x3, y3) = (PAIRS[p] for p in perm)
         out.append((x2 + x3 + "d" + y3, x1 + "d" + y1 + y2, "V1", "T", "d"))
         out.append(("l" + x3 + y2 + y3, x1 + x2 + y1 + "l", "V2", "T", "l"))
+    return out
+
+
+def term_groups():
+    """The twelve terms as six pairs that split W's indices into the same row
+    and column sets: term 1 under a pair permutation P (V1 (x2 x3 d y3) times
+    T (x1 d y1 y2)) and term 2 under P reversed, written with its operands
+    swapped (T (x3 x2 y3 l) times V2 (l x1 y2 y1)) so that in both the left
+    operand holds {x2, x3, y3}.  Each pair runs as one GEMM task over the d
+    and l segments together (one pass over W instead of two)."""
+    out = []
+    for perm in itertools.permutations(range(3)):
+        (x1, y1), (x2, y2), (x3, y3) = (PAIRS[p] for p in perm)
+        out.append([(x2 + x3 + "d" + y3, x1 + "d" + y1 + y2, "V1", "T", "d"),
+                    (x3 + x2 + y3 + "l", "l" + x1 + y2 + y1, "T", "V2", "l")])
     return out
 
 
@@ -82,14 +99,14 @@ def tuple_dims(w, tup):
     return [segs[w["kind"][lab]][s] for lab, s in zip(W_LABELS, tup)]
 
 
-def tuple_entries(w, tup):
-    """For W block `tup` = (A, B, C, I, J, K): per term, the contraction
-    pattern and the (left block, right block) pairs over the summed
-    segments."""
+def tuple_entries(w, tup, specs=None):
+    """For W block `tup` = (A, B, C, I, J, K): per term (of `specs`, default
+    the twelve of term_labels), the contraction pattern and the (left block,
+    right block) pairs over the summed segments."""
     idx = dict(zip(W_LABELS, tup))
     nseg = {"d": len(w["v_segs"]), "l": len(w["o_segs"])}
     terms = []
-    for ll, rl, ln, rn, summed in term_labels():
+    for ll, rl, ln, rn, summed in (term_labels() if specs is None else specs):
         labels = list(dict.fromkeys(W_LABELS + ll + rl))
         code = {lab: 1 + i for i, lab in enumerate(labels)}
         ptrn, e = sg.get_contraction_ptrn(6, 4, 4, [code[x] for x in W_LABELS + ll + rl])
@@ -103,22 +120,26 @@ def tuple_entries(w, tup):
 
 
 class TupleEnergy:
-    """Planned device work for one W block: twelve batches into `wbuf` and
-    the fused denominator / energy reduction into `out` (device double)."""
+    """Planned device work for one W block: six batches (the term pairs of
+    term_groups, each one GEMM task) into `wbuf` and the fused denominator /
+    energy reduction into `out` (device double)."""
 
     def __init__(self, w, tup, wbuf, out, moduli=None, factor=1.0):
         to_device(w)
         self.dims = tuple_dims(w, tup)
         self.batches = []
         first = True
-        for ptrn, pairs in tuple_entries(w, tup):
+        for group in term_groups():
             cs = []
-            for ln, lb, rn, rb in pairs:
-                L, R = w[ln], w[rn]
-                cs.append(sg.make_contraction(ptrn, L.addr(lb), L.dims(lb), R.addr(rb),
-                                              R.dims(rb), wbuf, self.dims,
-                                              accumulate=not first))
-                first = False
+            for ptrn, pairs in tuple_entries(w, tup, group):
+                for ln, lb, rn, rb in pairs:
+                    L, R = w[ln], w[rn]
+                    cs.append(sg.make_contraction(ptrn, L.addr(lb), L.dims(lb), R.addr(rb),
+                                                  R.dims(rb), wbuf, self.dims,
+                                                  accumulate=not first))
+                    first = False
+            # entries of two patterns into one block plan on panels (the table
+            # kernel takes one pattern per block)
             self.batches.append(sg.Batch(cs, moduli=moduli))
         segs = {"v": w["v_segs"], "o": w["o_segs"]}
         self.eps = []

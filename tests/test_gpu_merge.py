@@ -149,3 +149,36 @@ def test_persistent_split_tiles_vs_whole_tiles(make, monkeypatch):
         w["D"].dev.free()
         b.destroy()
     assert rel_fro(out["1"], out["0"]) <= 2 * TOL["packed"]
+
+
+@pytest.mark.parametrize("moduli", [None, 14])
+def test_two_patterns_into_one_block(moduli):
+    """One result block written through two contraction patterns whose
+    operands hold its row and column indices in different orders:
+    D(a,b,c) += L1(k,a,b) R1(k,c) and D(a,b,c) += L2(b,a,q) R2(c,q).  The
+    table kernel takes one pattern per block, so the default plan runs on
+    panels, rows and columns enumerated in destination order."""
+    rng = np.random.default_rng(21)
+    A, B, C, K, Q = 7, 5, 9, 11, 6
+    L1h, R1h = rng.standard_normal(K * A * B), rng.standard_normal(K * C)
+    L2h, R2h = rng.standard_normal(B * A * Q), rng.standard_normal(C * Q)
+    D0 = rng.standard_normal(A * B * C)
+    L1, R1, L2, R2 = (sg.DeviceArray.from_host(x) for x in (L1h, R1h, L2h, R2h))
+    D = sg.DeviceArray.from_host(D0)
+    # labels: a=1 b=2 c=3 k=4 / q=4
+    p1, e1 = sg.get_contraction_ptrn(3, 3, 2, [1, 2, 3, 4, 1, 2, 4, 3])
+    p2, e2 = sg.get_contraction_ptrn(3, 3, 2, [1, 2, 3, 2, 1, 4, 3, 4])
+    assert e1 == 0 and e2 == 0
+    cs = [sg.make_contraction(p1, L1.ptr, [K, A, B], R1.ptr, [K, C], D.ptr, [A, B, C],
+                              accumulate=True),
+          sg.make_contraction(p2, L2.ptr, [B, A, Q], R2.ptr, [C, Q], D.ptr, [A, B, C],
+                              accumulate=True)]
+    b = sg.Batch(cs, moduli=moduli)
+    assert b.kind == ("split" if moduli else "packed")
+    b.execute()
+    sg.synchronize()
+    t1 = np.einsum("kab,kc->abc", L1h.reshape(K, A, B, order="F"), R1h.reshape(K, C, order="F"))
+    t2 = np.einsum("baq,cq->abc", L2h.reshape(B, A, Q, order="F"), R2h.reshape(C, Q, order="F"))
+    want = D0 + (t1 + t2).ravel(order="F")
+    assert rel_fro(D.download(), want) <= (1e-13 if moduli is None else 1e-12)
+    b.destroy()
