@@ -10,6 +10,10 @@ namespace sipgpu {
 constexpr int kOzMaxModuli = 16;
 constexpr int kOzBM = 128, kOzBN = 128; // INT8 GEMM workgroup tile
 constexpr int kOzBigTile = 256;         // the large-task GEMM's tile (both dims)
+// tile code of the 256 x 256 GEMM with 8 waves of 64 x 128 (two per SIMD)
+constexpr int kOzWideTile = 257;
+// edge of the output tile a GEMM tile code stands for
+constexpr int oz_tile_size(int code) { return code == kOzWideTile ? kOzBigTile : code; }
 constexpr int kOzChunk = 16;            // bytes of k per panel chunk
 constexpr int kOzStage = 128;           // bytes of k per GEMM pipeline stage
 constexpr int kOzStageChunks = kOzStage / kOzChunk;

@@ -227,14 +227,16 @@ __global__ __launch_bounds__(256) void to_residues_kernel(OzConvArgs a) {
 // bytes.  MFMA v_mfma_i32_16x16x64_i8: lane l holds row l%16, bytes
 // 16*(l/16) .. +15 of a 64-byte k step; accumulator register r of lane l is
 // C(4*(l/16) + r, l%16).
-template <int BM, int BN>
-__global__ __launch_bounds__(256, 2) void gemm_i8_kernel(OzGemmArgs a) {
-    constexpr int WM = 2, WN = 2;
+template <int BM, int BN, int WM, int WN, int MINB, int SETS>
+__global__ __launch_bounds__(64 * WM * WN, MINB) void gemm_i8_kernel(OzGemmArgs a) {
+    constexpr int NT = 64 * WM * WN; // threads
     constexpr int TM = BM / WM, TN = BN / WN, FM = TM / 16, FN = TN / 16;
     constexpr int KS = kOzStage, CH = kOzStageChunks;
-    constexpr int IA = BM / 32, IB = BN / 32; // 8-row load instructions per wave per stage
+    // 8-row load instructions per wave per stage
+    constexpr int IA = BM / (8 * WM * WN), IB = BN / (8 * WM * WN);
     static_assert(CH == 8 && KS == 128, "a stage is 8 chunks of 16 bytes");
-    static_assert(BM % 32 == 0 && BN % 32 == 0 && TM % 16 == 0 && TN % 16 == 0, "tile shape");
+    static_assert(IA * 8 * WM * WN == BM && IB * 8 * WM * WN == BN && TM % 16 == 0 &&
+                      TN % 16 == 0, "tile shape");
     constexpr int kWin = 512; // chunk entries per LDS window (64 stages)
     static_assert(kWin % (2 * CH) == 0, "a window holds whole pairs of stages");
     __shared__ __attribute__((aligned(1024))) int8_t As[2][BM * KS];
@@ -270,10 +272,11 @@ __global__ __launch_bounds__(256, 2) void gemm_i8_kernel(OzGemmArgs a) {
 
         const OzChunkRef *ch = a.chunks + task.chunk0;
         const int nst = task.nchunks / CH; // even: the planner pads to pairs of stages
-        // two register sets: the loads of stage s + 2 are issued while stage s
-        // is multiplied, and land in LDS after stage s + 1
-        int32x4 xa[IA], xb[IB], ya[IA], yb[IB];
-        auto put = [&](int buf, const int32x4 (&va)[IA], const int32x4 (&vb)[IB]) {
+        // SETS == 2: two register sets, the loads of stage s + 2 are issued
+        // while stage s is multiplied and land in LDS after stage s + 1;
+        // SETS == 1 (the 64 x 128 wave tiles): one set, one stage ahead
+        int32x4 xa[IA], xb[IB], ya[SETS == 2 ? IA : 1], yb[SETS == 2 ? IB : 1];
+        auto put = [&](int buf, const int32x4 *va, const int32x4 *vb) {
 #pragma unroll
             for (int i = 0; i < IA; ++i)
                 *reinterpret_cast<int32x4 *>(&As[buf][(wave * IA + i) * 1024 + lane * 16]) = va[i];
@@ -315,7 +318,7 @@ __global__ __launch_bounds__(256, 2) void gemm_i8_kernel(OzGemmArgs a) {
         auto load_window = [&](int st0) {
             __syncthreads(); // nobody still reads the previous window
             const int c0 = st0 * CH, n = min(kWin, task.nchunks - c0);
-            for (int i = tid; i < n; i += 256)
+            for (int i = tid; i < n; i += NT)
                 *reinterpret_cast<int32x4 *>(&Es[i]) =
                     *reinterpret_cast<const int32x4 *>(&ch[c0 + i]);
             __syncthreads();
@@ -325,7 +328,7 @@ __global__ __launch_bounds__(256, 2) void gemm_i8_kernel(OzGemmArgs a) {
             const OzChunkRef e = Es[idx];
             return Sel{e.a, e.b, e.stride};
         };
-        auto fetch2 = [&](const Sel &e, int32x4 (&va)[IA], int32x4 (&vb)[IB]) {
+        auto fetch2 = [&](const Sel &e, int32x4 *va, int32x4 *vb) {
 #pragma unroll
             for (int i = 0; i < IA; ++i)
                 va[i] = *reinterpret_cast<const __attribute__((address_space(1))) int32x4 *>(
@@ -336,6 +339,21 @@ __global__ __launch_bounds__(256, 2) void gemm_i8_kernel(OzGemmArgs a) {
                     plane + e.b + rb[i] * e.stride);
         };
         load_window(0);
+        if constexpr (SETS == 1) {
+            fetch2(entry(0), xa, xb);
+            put(0, xa, xb);
+            __syncthreads();
+            int buf = 0;
+            for (int s = 0; s < nst; ++s) {
+                if (s + 1 < nst && ((s + 1) * CH) % kWin == 0) load_window(s + 1);
+                fetch2(entry(s + 1), xa, xb);
+                __builtin_amdgcn_sched_barrier(0); // keep the loads ahead of the MFMAs
+                compute(buf);
+                put(buf ^ 1, xa, xb); // after the last stage this writes a copy nothing reads
+                __syncthreads();
+                buf ^= 1;
+            }
+        } else {
         fetch2(entry(0), xa, xb);
         fetch2(entry(1), ya, yb);
         put(0, xa, xb);
@@ -356,6 +374,7 @@ __global__ __launch_bounds__(256, 2) void gemm_i8_kernel(OzGemmArgs a) {
             compute(1);
             put(0, xa, xb); // after the last pair of stages this writes a copy nothing reads
             __syncthreads();
+        }
         }
 
         // t = (C mod p) u mod p of the tile into this modulus' result plane,
@@ -528,7 +547,13 @@ hipError_t oz_to_residues(const OzConvArgs &a, hipStream_t s) {
 hipError_t oz_gemm(const OzGemmArgs &a, hipStream_t s) {
     if (a.ntiles <= 0) return hipSuccess;
     if (a.tile == kOzBigTile) return oz_gemm_big(a, s);
-    hipLaunchKernelGGL((gemm_i8_kernel<kOzBM, kOzBN>), dim3(a.ntiles, a.c.n), dim3(256), 0, s, a);
+    if (a.tile == kOzWideTile) {
+        hipLaunchKernelGGL((gemm_i8_kernel<kOzBigTile, kOzBigTile, 4, 2, 1, 1>),
+                           dim3(a.ntiles, a.c.n), dim3(512), 0, s, a);
+        return hipGetLastError();
+    }
+    hipLaunchKernelGGL((gemm_i8_kernel<kOzBM, kOzBN, 2, 2, 2, 2>), dim3(a.ntiles, a.c.n), dim3(256),
+                       0, s, a);
     return hipGetLastError();
 }
 

@@ -1157,26 +1157,30 @@ int sipgpu::rt::build_panel_plan(const sipgpu_contraction *list, int n, int nmod
         for (long long e0 = 0; e0 < mn; e0 += kOzRecChunk) rec.push_back(OzRecItem{t, (int)e0});
         res_elems += (mn + 15) / 16 * 16;
     }
-    // GEMM tile: 256 x 256 when the tasks' virtual K averages at least 8 KB
-    // (half the LDS traffic per MFMA; its per-tile prologue and eight-round
-    // epilogue need a long K to amortize), else 128 x 128 (two workgroups
-    // per CU).  Measured: ring tasks (K = 4.7 KB) run 1.25x faster on 128,
-    // block-sparse ladder tasks (K = 20 KB) 1.18x faster on 256.
-    // SIPGPU_OZ_TILE=128|256 pins it.
-    long long sumK = 0;
-    for (const OzTask &t : tasks) sumK += (long long)t.nchunks * kOzChunk;
-    int tile = (nmod && ntasks && sumK / ntasks >= 8192) ? kOzBigTile : kOzBM;
+    // GEMM tile: 256 x 256 with eight waves of 64 x 128 (two per SIMD, two
+    // thirds of the LDS traffic per MFMA of the 128 x 128 tile) when the batch has at
+    // least one such tile per CU over all moduli, else 128 x 128.  Measured
+    // with 14 moduli: ring 1.98 ms against 2.24 (128) and 2.97 (256, four
+    // waves of 128 x 128, one per SIMD); the block-sparse ladder likewise.
+    // SIPGPU_OZ_TILE=128|256|257 or the tuner pins it.
+    long long big_tiles = 0;
+    for (const OzTask &t : tasks)
+        big_tiles += (long long)((t.M + kOzBigTile - 1) / kOzBigTile) *
+                     ((t.N + kOzBigTile - 1) / kOzBigTile);
+    int tile = (nmod && big_tiles * nmod >= 256) ? kOzWideTile : kOzBM;
     if (const char *e = std::getenv("SIPGPU_OZ_TILE")) {
         const int v = std::atoi(e);
-        if (nmod && (v == kOzBM || v == kOzBigTile)) tile = v;
+        if (nmod && (v == kOzBM || v == kOzBigTile || v == kOzWideTile)) tile = v;
     }
-    if (nmod && (opt.tile == kOzBM || opt.tile == kOzBigTile)) tile = opt.tile;
+    if (nmod && (opt.tile == kOzBM || opt.tile == kOzBigTile || opt.tile == kOzWideTile))
+        tile = opt.tile;
+    const int tsize = oz_tile_size(tile);
     // tiles: the same XCD schedule as the FP64 batch
     const char *tail_env = std::getenv("SIPGPU_SPLIT_TAIL"); // "0": one tile per workgroup
     std::vector<int> tms(ntasks), tns(ntasks);
     for (int t = 0; t < ntasks; ++t) {
-        tms[t] = (tasks[t].M + tile - 1) / tile;
-        tns[t] = (tasks[t].N + tile - 1) / tile;
+        tms[t] = (tasks[t].M + tsize - 1) / tsize;
+        tns[t] = (tasks[t].N + tsize - 1) / tsize;
     }
     int real_tiles = 0;
     std::vector<std::vector<OzTile>> lists = xcd_lists<OzTile>(tms, tns, task_cost, real_tiles);
@@ -1900,7 +1904,7 @@ int sipgpu_batch_tune(const sipgpu_contraction *list, int n, int nmoduli, sipgpu
                 cands.push_back(ch);
             }
     } else {
-        for (int tile : {kOzBM, kOzBigTile})
+        for (int tile : {kOzBM, kOzBigTile, kOzWideTile})
             for (int merge : {1, 0}) {
                 PlanChoice ch;
                 ch.path = 2;
@@ -1995,7 +1999,8 @@ int sipgpu_batch_plan_choice(const sipgpu_contraction *list, int n, int nmoduli,
     if (int rc = sipgpu::rt::refuse_in_capture("batch planning")) return rc;
     if (!batch || (n > 0 && !list) || n < 0) return fail(SIPGPU_EARG, "bad batch arguments");
     if (path < 0 || path > 2 || (path == 2) != (nmoduli != 0) || config < -1 ||
-        config >= kGemmConfigs || (tile != -1 && tile != kOzBM && tile != kOzBigTile))
+        config >= kGemmConfigs ||
+        (tile != -1 && tile != kOzBM && tile != kOzBigTile && tile != kOzWideTile))
         return fail(SIPGPU_EARG, "bad plan choice");
     if (nmoduli != 0 && (nmoduli < 6 || nmoduli > 14))
         return fail(SIPGPU_EARG, "split path: nmoduli must be in 6..14");

@@ -227,7 +227,7 @@ def sparse_bench(reps=5, moduli=14):
            "v_segments": w["v_segs"], "o_segments": w["o_segs"]}
     ref = None
     for name, nm in (("fp64", None), ("split", moduli)):
-        b = wl.batch_for(w, moduli=nm)
+        b, tres = sg.tune(wl.contractions_for(w), moduli=nm)  # planned by the autotuner
         D.dev.zero()
         b.execute()
         sg.synchronize()
@@ -246,7 +246,7 @@ def sparse_bench(reps=5, moduli=14):
         e1.synchronize()
         ms = e0.elapsed_ms(e1) / reps
         out[name] = {"ms": ms, "tflops_fp64_equiv": b.flops / (ms * 1e-3) / 1e12,
-                     "moduli": b.moduli, "bits": b.bits}
+                     "moduli": b.moduli, "bits": b.bits, "tuned": tune_summary(tres)}
         if err is not None:
             out[name]["max_block_rel_err_vs_fp64"] = err
         b.destroy()

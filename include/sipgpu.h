@@ -195,8 +195,10 @@ int sipgpu_batch_execute_stages(sipgpu_batch_t batch, int stages, sipgpu_stream_
 /* Autotuner (no reference counterpart).  Plans the batch every candidate
  * way -- the table kernel in each of its tile configurations, packed FP64
  * panels with and without merged result blocks and with and without the
- * persistent water-filled schedule; for the split path (nmoduli > 0) both
- * GEMM tiles with and without merging -- times each (one warm-up and three
+ * persistent water-filled schedule; for the split path (nmoduli > 0) the
+ * three INT8 GEMMs (tile 128: 128 x 128, four waves; 256: 256 x 256, four
+ * waves of 128 x 128; 257: 256 x 256, eight waves of 64 x 128) with and
+ * without merging -- times each (one warm-up and three
  * timed executions on the library stream, into scratch copies of the
  * destinations, so the caller's blocks are not touched) and returns the
  * fastest plan.  The choice is remembered under the batch's signature (its

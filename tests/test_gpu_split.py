@@ -165,9 +165,13 @@ def test_split_rejects_bad_moduli():
         assert ei.value.code == -107
 
 
-def test_split_long_k_crosses_entry_windows():
+@pytest.mark.parametrize("tile", [None, "128", "257"])
+def test_split_long_k_crosses_entry_windows(tile, monkeypatch):
     """One result block summing three pairs of K = 3000 (564 chunks of 16
-    bytes of virtual K, more than one staged window of chunk entries)."""
+    bytes of virtual K, more than one staged window of chunk entries), on
+    the default GEMM for its K and on the other two pinned."""
+    if tile:
+        monkeypatch.setenv("SIPGPU_OZ_TILE", tile)
     rng = np.random.default_rng(11)
     M, N, K = 20, 30, 3000
     ptrn, e = sg.get_contraction_ptrn(2, 2, 2, [1, 2, 3, 1, 3, 2])  # D(m,n) = L(k,m) R(k,n)
@@ -191,14 +195,15 @@ def test_split_long_k_crosses_entry_windows():
         b.destroy()
 
 
-@pytest.mark.parametrize("tile", ["128", "256"])
+@pytest.mark.parametrize("tile", ["128", "256", "257"])
 @pytest.mark.parametrize("make", [
     lambda: wl.ladder(),
     lambda: wl.ring(o_segs=(15,), v_segs=(30, 15)),
 ], ids=["ladder", "ring-uneven"])
 def test_split_both_gemm_tiles(make, tile, monkeypatch):
-    """The 128 x 128 and the 256 x 256 INT8 GEMMs (chosen per batch by task
-    size) give the same parity; SIPGPU_OZ_TILE pins one."""
+    """The 128 x 128 and both 256 x 256 INT8 GEMMs (four waves of 128 x 128,
+    code 256; eight waves of 64 x 128, code 257) give the same parity;
+    SIPGPU_OZ_TILE pins one."""
     monkeypatch.setenv("SIPGPU_OZ_TILE", tile)
     w = make()
     w["D"].host[:] = 0.0
