@@ -137,6 +137,21 @@ __global__ __launch_bounds__(256) void row_exponent_kernel(OzConvArgs a, const O
     }
 }
 
+// t_i = (v mod p) u mod p in [0, p) of an INT32 accumulator |v| < 2^31:
+// v = h 2^17 + l with l in [0, 2^17), so h c17 + l + p 2^17 (c17 = 2^17 mod
+// p) is congruent to v, positive and below 2^26; two Barrett steps with
+// bar = floor(2^32 / p) (each leaves r in [0, 2p)) and the multiplication
+// by u (< p) finish it in 32-bit integer arithmetic.
+__device__ __forceinline__ uint32_t crt_digit(int v, int p, int c17, int u, uint32_t bar) {
+    const uint32_t t = (uint32_t)((v >> 17) * c17 + (v & 0x1ffff)) + ((uint32_t)p << 17);
+    uint32_t c = t - __umulhi(t, bar) * (uint32_t)p;
+    if (c >= (uint32_t)p) c -= p;
+    uint32_t tt = c * (uint32_t)u;
+    tt -= __umulhi(tt, bar) * (uint32_t)p;
+    if (tt >= (uint32_t)p) tt -= p;
+    return tt;
+}
+
 // Residue of an exact integer |v| < 2^52 held in a double, in [0, p):
 // q = floor(v / p) from the rounded reciprocal is off by at most one, and
 // v - p q is exact (fma of integers below 2^53).  (The residue panels use
@@ -249,8 +264,8 @@ __global__ __launch_bounds__(64 * WM * WN, MINB) void gemm_i8_kernel(OzGemmArgs 
     const int li = lane & 15, lk = lane >> 4;
     const int g = (lane & 7) ^ (lane >> 3); // chunk of the stage this lane loads
     const int mod = blockIdx.y;
-    const int p = a.c.p[mod], u = a.c.u[mod];
-    const double pd = p, rp = a.c.rp[mod];
+    const int p = a.c.p[mod], u = a.c.u[mod], c17 = a.c.c17[mod];
+    const uint32_t bar = a.c.bar[mod];
     gi8_t *plane = (gi8_t *)(a.planes + (long long)mod * a.plane_bytes);
 
     for (int t = blockIdx.x; t < a.ntiles; t += gridDim.x) {
@@ -389,8 +404,7 @@ __global__ __launch_bounds__(64 * WM * WN, MINB) void gemm_i8_kernel(OzGemmArgs 
                 for (int r = 0; r < 4; ++r) {
                     const int m = m0 + wm * TM + i * 16 + 4 * lk + r;
                     if (m < task.M && n < task.N) {
-                        const int c = mod_exact((double)acc[i][j][r], pd, rp);
-                        const int tt = mod_exact((double)(c * u), pd, rp);
+                        const uint32_t tt = crt_digit(acc[i][j][r], p, c17, u, bar);
                         const long long idx = task.nfast ? (long long)m * task.N + n
                                                          : (long long)n * task.M + m;
                         res[idx] = (uint8_t)tt;

@@ -143,7 +143,8 @@ __global__ __launch_bounds__(256, 1) void gemm_i8_big_kernel(OzGemmArgs a) {
         // the task's element order (|C| < 2^31, so C + p 2^23 is positive and
         // below 2^32 for the Barrett step).
         uint8_t *res = a.res_planes + (long long)mod * a.res_plane_bytes + task.res_off;
-        const uint32_t cofs = (uint32_t)p << 23;
+        const uint32_t cofs = (uint32_t)p << 17;
+        const int c17 = a.c.c17[mod];
         int32_t *Cs = reinterpret_cast<int32_t *>(Lds);
 #pragma unroll
         for (int q = 0; q < FM; ++q) {
@@ -166,7 +167,10 @@ __global__ __launch_bounds__(256, 1) void gemm_i8_big_kernel(OzGemmArgs a) {
                 const int m = m0 + (w % WM) * TM + q * 16 + row;
                 const int n = n0 + (w / WM) * TN + col;
                 if (m < task.M && n < task.N) {
-                    const uint32_t cv = (uint32_t)Cs[w * 128 * kPitch + col * kPitch + row] + cofs;
+                    // v = h 2^17 + l: h c17 + l + p 2^17 is congruent to v,
+                    // positive and below 2^26 for every |v| < 2^31
+                    const int v = Cs[w * 128 * kPitch + col * kPitch + row];
+                    const uint32_t cv = (uint32_t)((v >> 17) * c17 + (v & 0x1ffff)) + cofs;
                     uint32_t c = cv - __umulhi(cv, bar) * (uint32_t)p;
                     if (c >= (uint32_t)p) c -= p;
                     uint32_t tt = c * (uint32_t)u;
