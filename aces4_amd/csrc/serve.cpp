@@ -465,11 +465,12 @@ int take_slot(sipgpu_server_s &S, int &slot) {
     return 0;
 }
 
-// A split-precision batch: its descriptors through a slot, its planes in the
-// scratch buffer.  Batches run in stream order, so one scratch buffer serves
-// them all; it grows (after the stream drains) when a batch needs more.
-int server_split(sipgpu_server_s &S, const sipgpu::rt::PanelPlan &plan,
-                 const std::vector<unsigned char> &desc) {
+// A panel batch (split precision, or packed FP64): its descriptors through a
+// slot, its planes in the scratch buffer.  Batches run in stream order, so
+// one scratch buffer serves them all; it grows (after the stream drains)
+// when a batch needs more.
+int server_panels(sipgpu_server_s &S, const sipgpu::rt::PanelPlan &plan,
+                  const std::vector<unsigned char> &desc) {
     if (plan.ntiles == 0) return 0;
     auto up = [](size_t v) { return (v + 255) & ~(size_t)255; };
     const size_t need = up(plan.plane_total()) + up(plan.res_total()) + up(plan.work_total());
@@ -531,6 +532,7 @@ int server_batch(sipgpu_server_s &S, std::vector<Pending> &batch) {
     sipgpu::rt::BatchImage img;
     sipgpu::rt::PanelPlan plan;
     const int nmod = batch.empty() ? 0 : batch[0].req.moduli;
+    bool panels = nmod > 0; // split, or packed FP64
     if (!rc) {
         std::vector<sipgpu_contraction> list(batch.size());
         for (size_t i = 0; i < batch.size(); ++i) {
@@ -548,16 +550,22 @@ int server_batch(sipgpu_server_s &S, std::vector<Pending> &batch) {
             std::memcpy(e.ddims, r.ddims, sizeof(e.ddims));
             e.accumulate = r.accumulate;
         }
-        if (nmod)
+        if (nmod) {
             rc = sipgpu::rt::build_panel_plan(list.data(), (int)list.size(), nmod, plan);
-        else
+        } else {
             rc = sipgpu::rt::build_batch_image(list.data(), (int)list.size(), img);
+            // large FP64 batches run on packed panels, as sipgpu_batch_plan
+            // plans them (the table kernel gathers per tile)
+            if (!rc && img.ntasks > 0 && img.flops >= 1e9 && img.flops / img.ntasks >= 5e7 &&
+                sipgpu::rt::build_panel_plan(list.data(), (int)list.size(), 0, plan) == 0)
+                panels = true;
+        }
     }
-    const std::vector<unsigned char> &desc = nmod ? plan.host : img.host;
+    const std::vector<unsigned char> &desc = panels ? plan.host : img.host;
     if (!rc && desc.size() > kSlotBytes)
         rc = fail(SIPGPU_ESIZE, "server: batch work list exceeds its descriptor slot");
-    if (!rc && nmod) rc = server_split(S, plan, desc);
-    if (!rc && !nmod && img.ntiles > 0) {
+    if (!rc && panels) rc = server_panels(S, plan, desc);
+    if (!rc && !panels && img.ntiles > 0) {
         int slot = 0;
         rc = take_slot(S, slot);
         if (!rc) {
@@ -579,7 +587,7 @@ int server_batch(sipgpu_server_s &S, std::vector<Pending> &batch) {
     if (!rc) {
         S.st.batches += 1;
         S.st.requests += batch.size();
-        S.st.flops += nmod ? plan.flops : img.flops;
+        S.st.flops += panels ? plan.flops : img.flops;
         S.st.max_batch = std::max<unsigned long long>(S.st.max_batch, batch.size());
     }
     return rc;

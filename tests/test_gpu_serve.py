@@ -250,3 +250,32 @@ def test_server_replay_latencies():
     assert np.all(lat > 0) and wall >= arrivals[-1]
     srv.destroy()
     c.destroy()
+
+
+def test_server_large_requests_on_panels():
+    """Requests of over 1e9 flop run as packed FP64 panel batches bound into
+    the server's scratch buffer; each result matches the product to 1e-13,
+    including one that accumulates."""
+    rng = np.random.default_rng(14)
+    shapes = [(1000, 1100, 900), (1030, 970, 1010)]  # (M, K, N): ~2e9 flop each
+    c = sg.Cache(256 * MB)
+    ptrn = [1, -1, -2, 2]  # D(i,j) = L(i,k) R(k,j)
+    reqs, want, key = [], [], 1
+    for idx, (m, k, n) in enumerate(shapes):
+        L, R, D0 = rng.standard_normal(m * k), rng.standard_normal(k * n), rng.standard_normal(m * n)
+        c.insert(key, m * k, host=L)
+        c.insert(key + 1, k * n, host=R)
+        c.insert(key + 2, m * n, host=D0)
+        acc = idx == 1
+        reqs.append(sg.make_request(ptrn, key, [m, k], key + 1, [k, n], key + 2, [m, n],
+                                    accumulate=acc))
+        prod = (L.reshape(m, k, order="F") @ R.reshape(k, n, order="F")).ravel(order="F")
+        want.append(prod + (D0 if acc else 0.0))
+        key += 3
+    srv = sg.Server(c, max_batch=8)
+    for t in [srv.submit(r) for r in reqs]:
+        srv.wait(t)
+    srv.destroy()
+    for r, wv in zip(reqs, want):
+        assert rel_fro(c.read(r.dkey, wv.size), wv) <= 1e-13
+    c.destroy()
