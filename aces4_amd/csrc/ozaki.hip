@@ -299,24 +299,32 @@ __global__ __launch_bounds__(64 * WM * WN, MINB) void gemm_i8_kernel(OzGemmArgs 
             for (int i = 0; i < IB; ++i)
                 *reinterpret_cast<int32x4 *>(&Bs[buf][(wave * IB + i) * 1024 + lane * 16]) = vb[i];
         };
+        // B fragments in groups of FH (the 64 x 128 wave tiles read them in
+        // two halves, which keeps the second prefetch register set in budget)
+        constexpr int FH = FN > 4 ? 4 : FN;
         auto compute = [&](int buf) {
             const int8_t *sa = As[buf], *sb = Bs[buf];
 #pragma unroll
             for (int ks = 0; ks < KS / 64; ++ks) {
                 const int slot = ((ks * 4 + lk) ^ (li & 7)) * 16;
-                int32x4 fa[FM], fb[FN];
+                int32x4 fa[FM];
 #pragma unroll
                 for (int i = 0; i < FM; ++i)
                     fa[i] = *reinterpret_cast<const int32x4 *>(sa + (wm * TM + i * 16 + li) * KS + slot);
 #pragma unroll
-                for (int j = 0; j < FN; ++j)
-                    fb[j] = *reinterpret_cast<const int32x4 *>(sb + (wn * TN + j * 16 + li) * KS + slot);
+                for (int j0 = 0; j0 < FN; j0 += FH) {
+                    int32x4 fb[FH];
 #pragma unroll
-                for (int i = 0; i < FM; ++i)
+                    for (int j = 0; j < FH; ++j)
+                        fb[j] = *reinterpret_cast<const int32x4 *>(
+                            sb + (wn * TN + (j0 + j) * 16 + li) * KS + slot);
 #pragma unroll
-                    for (int j = 0; j < FN; ++j)
-                        acc[i][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(fa[i], fb[j], acc[i][j],
-                                                                          0, 0, 0);
+                    for (int i = 0; i < FM; ++i)
+#pragma unroll
+                        for (int j = 0; j < FH; ++j)
+                            acc[i][j0 + j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(
+                                fa[i], fb[j], acc[i][j0 + j], 0, 0, 0);
+                }
             }
         };
 
