@@ -2065,6 +2065,36 @@ int sipgpu_tune_clear(void) {
     return 0;
 }
 
+int sipgpu_batch_plan_split_tol(const sipgpu_contraction *list, int n, double rel_err,
+                                sipgpu_batch_t *batch) {
+    if (int rc = sipgpu::rt::refuse_in_capture("batch planning")) return rc;
+    if (!batch || (n > 0 && !list) || n < 0 || !(rel_err > 0.0))
+        return fail(SIPGPU_EARG, "bad batch arguments");
+    // longest summed K over the result blocks (all entries counted: an
+    // upper bound, so the width chosen is never too wide)
+    std::unordered_map<const double *, long long> ksum;
+    long long kmax = 1;
+    for (int i = 0; i < n; ++i) {
+        const sipgpu_contraction &e = list[i];
+        if (e.lrank < 1 || e.lrank > SIPGPU_MAX_RANK || e.rrank < 0 ||
+            e.rrank > SIPGPU_MAX_RANK)
+            return fail(SIPGPU_ERANK, "batch entries must be block = block * block, rank 1..6");
+        long long k = 1;
+        for (int d = 0; d < e.lrank; ++d)
+            if (e.ptrn[d] < 0) k *= std::max(e.ldims[d], 1);
+        kmax = std::max(kmax, ksum[e.d] += k);
+    }
+    int nmod = 0;
+    for (int m = kOzMinModuli; m <= kOzMaxUsable && !nmod; ++m) {
+        OzConst oc;
+        const int P = oz_constants(m, kmax, oc);
+        if (P >= 8 && 16.0 * std::ldexp(1.0, -P) <= rel_err) nmod = m;
+    }
+    if (!nmod)
+        return fail(SIPGPU_EARG, "split path: 14 moduli do not meet this error target");
+    return sipgpu_batch_plan_split(list, n, nmod, batch);
+}
+
 int sipgpu_batch_split_info(sipgpu_batch_t b, int *nmoduli, int *bits) {
     if (!b) return fail(SIPGPU_EARG, "null batch");
     if (nmoduli) *nmoduli = b->split ? b->sd.c.n : 0;

@@ -45,6 +45,7 @@ EXPORTED = [
     "sipgpu_batch_execute_stages", "sipgpu_graph_begin", "sipgpu_graph_end",
     "sipgpu_graph_launch", "sipgpu_graph_nodes", "sipgpu_graph_destroy", "sipgpu_batch_tune",
     "sipgpu_batch_plan_choice", "sipgpu_tune_save", "sipgpu_tune_load", "sipgpu_tune_clear",
+    "sipgpu_batch_plan_split_tol",
     "sipgpu_cache_create", "sipgpu_cache_destroy", "sipgpu_cache_insert", "sipgpu_cache_acquire",
     "sipgpu_cache_release", "sipgpu_cache_read", "sipgpu_cache_flush", "sipgpu_cache_evict",
     "sipgpu_cache_get_stats", "sipgpu_server_create", "sipgpu_server_submit",
@@ -150,6 +151,8 @@ def lib():
         L.sipgpu_batch_plan_choice.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
                                                ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                                ctypes.c_int, ctypes.c_int, ctypes.POINTER(_vp)]
+        L.sipgpu_batch_plan_split_tol.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_double,
+                                                  ctypes.POINTER(_vp)]
         L.sipgpu_tune_save.argtypes = [ctypes.c_char_p]
         L.sipgpu_tune_load.argtypes = [ctypes.c_char_p]
         L.sipgpu_graph_begin.argtypes = [_vp]
@@ -576,11 +579,14 @@ class Batch:
     """A planned batch of contractions: one grouped FP64-MFMA launch, or with
     `moduli` set, the split-precision (INT8 / CRT) path over that many moduli."""
 
-    def __init__(self, contractions, moduli=None, _handle=None):
+    def __init__(self, contractions, moduli=None, _handle=None, tol=None):
         arr = (Contraction * len(contractions))(*contractions)
         p = _vp()
         if _handle is not None:
             p.value = _handle
+        elif tol is not None:  # split path, moduli set by the error target
+            _check(lib().sipgpu_batch_plan_split_tol(arr, len(contractions), float(tol),
+                                                     ctypes.byref(p)))
         elif moduli is None:
             _check(lib().sipgpu_batch_plan(arr, len(contractions), ctypes.byref(p)))
         else:

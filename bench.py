@@ -480,6 +480,10 @@ def main():
         best = max(ok, key=lambda v: v["tflops_fp64_equiv"]) if ok else None
         step_rate = flops / (float(step_ms.mean()) * 1e-3) / 1e12
         split["best_within_1e-12"] = best["moduli"] if best else None
+        bt = sg.Batch(wl.contractions_for(w), tol=1e-12)  # moduli from the error target
+        split["moduli_for_target_1e-12"] = {"moduli": bt.moduli, "bits": bt.bits,
+                                            "rule": "fewest moduli with 16 * 2^-P <= 1e-12"}
+        bt.destroy()
         split["speedup_vs_plain"] = best["tflops_fp64_equiv"] / step_rate if best else None
         split["speedup_vs_plain_gemm_only"] = best["tflops_fp64_equiv"] / achieved if best else None
     sparse = None if a.no_sparse else sparse_bench()

@@ -170,6 +170,15 @@ int sipgpu_batch_destroy(sipgpu_batch_t batch);
  * Blocks that share rows through a result block share their row scales. */
 int sipgpu_batch_plan_split(const sipgpu_contraction *list, int n, int nmoduli,
                             sipgpu_batch_t *batch);
+/* The split path with the moduli count set by an error target: the fewest
+ * moduli (6..14) whose operand width P, for this batch's longest K, meets
+ * 16 * 2^-P <= rel_err.  The split products' relative error (Frobenius, per
+ * result block) is about kappa 2^-P with kappa 4-5 on the benchmark blocks
+ * (Gaussian and uniform data); 16 leaves room for rows with a wider spread.
+ * Fails with SIPGPU_EARG when 14 moduli do not meet the target (run the
+ * plain FP64 path instead).  sipgpu_batch_split_info reports the choice. */
+int sipgpu_batch_plan_split_tol(const sipgpu_contraction *list, int n, double rel_err,
+                                sipgpu_batch_t *batch);
 /* Moduli count and operand bits P of a split batch (0 and 53 for FP64 batches). */
 int sipgpu_batch_split_info(sipgpu_batch_t batch, int *nmoduli, int *bits);
 /* How a planned batch runs.  sipgpu_batch_plan picks the packed path for

@@ -209,3 +209,27 @@ def test_split_both_gemm_tiles(make, tile, monkeypatch):
     w["D"].host[:] = 0.0
     _, got = run_split(w, 14)
     assert per_block_err(w, got, oracle_batch(w)) <= TOL
+
+
+@pytest.mark.parametrize("tol", [1e-10, 1e-12, 1e-13])
+def test_split_moduli_from_error_target(tol):
+    """sipgpu_batch_plan_split_tol takes the fewest moduli with 16 2^-P <=
+    tol (one modulus fewer misses it) and the result meets the target
+    against the oracle; a target no moduli count reaches is refused."""
+    w = wl.ring(o_segs=(15,), v_segs=(30, 15))
+    rng = np.random.default_rng(17)
+    w["D"].host[:] = rng.standard_normal(w["D"].host.size)
+    w["D"].dev = None
+    cs = wl.contractions_for(w)
+    b = sg.Batch(cs, tol=tol)
+    assert b.kind == "split" and 16 * 2.0 ** -b.bits <= tol
+    if b.moduli > 6:
+        fewer = sg.Batch(cs, moduli=b.moduli - 1)
+        assert 16 * 2.0 ** -fewer.bits > tol
+        fewer.destroy()
+    b.execute()
+    sg.synchronize()
+    assert per_block_err(w, w["D"].dev.download(), oracle_batch(w)) <= tol
+    b.destroy()
+    with pytest.raises(sg.SipGpuError):
+        sg.Batch(cs, tol=1e-17)
