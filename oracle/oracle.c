@@ -327,3 +327,84 @@ double orc_fcontract(long long n, const double *l, const double *r, double d) {
     for (long long i = 0; i < n; ++i) val = val + l[i] * r[i];
     return d + val;
 }
+
+/* Elements of a block of rank > 0; 0 when the product is not positive. */
+static long long orc_elems(int rank, const int *dims) {
+    long long n = 1;
+    for (int i = 0; i < rank; ++i) n *= dims[i];
+    return n > 0 ? n : 0;
+}
+
+/* tensor_dil_omp.F90:144-187 */
+void orc_block_init(int rank, const int *dims, double *t, double sval, int *ierr) {
+    *ierr = 0;
+    if (rank < 0) { *ierr = -1; return; }
+    if (rank == 0) { t[0] = sval; return; }
+    long long n = orc_elems(rank, dims);
+    for (long long i = 0; i < n; ++i) t[i] = sval;
+}
+
+/* tensor_dil_omp.F90:189-219 */
+void orc_block_scale(int rank, const int *dims, double *t, double s, int *ierr) {
+    *ierr = 0;
+    if (rank < 0) { *ierr = -1; return; }
+    if (rank == 0) { t[0] = t[0] * s; return; }
+    long long n = orc_elems(rank, dims);
+    for (long long i = 0; i < n; ++i) t[i] = t[i] * s;
+}
+
+/* tensor_dil_omp.F90:221-256 */
+double orc_block_norm2(int rank, const int *dims, const double *t, int *ierr) {
+    *ierr = 0;
+    if (rank < 0) { *ierr = -1; return 0.0; }
+    if (rank == 0) return t[0] * t[0];
+    long long n = orc_elems(rank, dims);
+    double v = 0.0;
+    for (long long i = 0; i < n; ++i) v = v + t[i] * t[i];
+    return v;
+}
+
+/* Column-major offset in `tens` of element `l` of the slice: the slice's
+ * multi-index shifted by ext_beg (tensor_dil_omp.F90:283-284, :338-339). */
+static long long orc_slice_offset(int rank, const int *tens_ext, const int *slice_ext,
+                                  const int *ext_beg, long long l) {
+    long long off = 0, stride = 1;
+    for (int i = 0; i < rank; ++i) {
+        long long im = l % slice_ext[i];
+        l /= slice_ext[i];
+        off += (ext_beg[i] + im) * stride;
+        stride *= tens_ext[i];
+    }
+    return off;
+}
+
+/* Shared rank / size rules of slice and insert (tensor_dil_omp.F90:274-276,
+ * :301-307 and divide_segment_ :1248-1254); returns the slice's element
+ * count, or 0 when there is nothing to copy. */
+static long long orc_slice_count(int rank, const int *slice_ext, int *ierr) {
+    *ierr = 0;
+    if (rank < 0) { *ierr = 1; return 0; }
+    if (rank == 0) return 1;
+    long long n = 1;
+    for (int i = 0; i < rank; ++i) n *= slice_ext[i];
+    if (n <= 0) { *ierr = -1; return 0; }
+    return n;
+}
+
+/* tensor_dil_omp.F90:258-311 */
+void orc_block_slice(int rank, const double *tens, const int *tens_ext, double *slice,
+                     const int *slice_ext, const int *ext_beg, int *ierr) {
+    long long n = orc_slice_count(rank, slice_ext, ierr);
+    if (rank == 0 && !*ierr) { slice[0] = tens[0]; return; }
+    for (long long l = 0; l < n; ++l)
+        slice[l] = tens[orc_slice_offset(rank, tens_ext, slice_ext, ext_beg, l)];
+}
+
+/* tensor_dil_omp.F90:313-368 */
+void orc_block_insert(int rank, double *tens, const int *tens_ext, const double *slice,
+                      const int *slice_ext, const int *ext_beg, int *ierr) {
+    long long n = orc_slice_count(rank, slice_ext, ierr);
+    if (rank == 0 && !*ierr) { tens[0] = slice[0]; return; }
+    for (long long l = 0; l < n; ++l)
+        tens[orc_slice_offset(rank, tens_ext, slice_ext, ext_beg, l)] = slice[l];
+}

@@ -101,6 +101,56 @@ class Oracle:
                                            ctypes.c_void_p, ctypes.c_double]
         return self.lib.orc_fcontract(l.size, l.ctypes.data, r.ctypes.data, float(d))
 
+    def block_init(self, dims, t, sval):
+        d = _ints(dims)
+        t = _f64(t).ravel().copy()
+        ierr = ctypes.c_int(0)
+        self.lib.orc_block_init(len(dims), _iptr(d), _dptr(t), ctypes.c_double(sval),
+                                ctypes.byref(ierr))
+        return t, ierr.value
+
+    def block_scale(self, dims, t, s):
+        d = _ints(dims)
+        t = _f64(t).ravel().copy()
+        ierr = ctypes.c_int(0)
+        self.lib.orc_block_scale(len(dims), _iptr(d), _dptr(t), ctypes.c_double(s),
+                                 ctypes.byref(ierr))
+        return t, ierr.value
+
+    def block_norm2(self, dims, t):
+        d = _ints(dims)
+        t = _f64(t).ravel()
+        ierr = ctypes.c_int(0)
+        self.lib.orc_block_norm2.restype = ctypes.c_double
+        v = self.lib.orc_block_norm2(len(dims), _iptr(d), _dptr(t), ctypes.byref(ierr))
+        return v, ierr.value
+
+    def block_add(self, dims, t0, t1, s):
+        d = _ints(dims)
+        t0 = _f64(t0).ravel().copy()
+        ierr = ctypes.c_int(0)
+        self.lib.orc_tensor_block_add(len(dims), _iptr(d), _dptr(t0), _dptr(_f64(t1).ravel()),
+                                      ctypes.c_double(s), ctypes.byref(ierr))
+        return t0, ierr.value
+
+    def block_slice(self, tens, tens_ext, slice_ext, ext_beg, out=None):
+        te, se, eb = _ints(tens_ext), _ints(slice_ext), _ints(ext_beg)
+        n = max(int(np.prod(slice_ext)) if len(slice_ext) else 1, 1)
+        out = np.full(n, np.nan) if out is None else _f64(out).ravel().copy()
+        ierr = ctypes.c_int(0)
+        self.lib.orc_block_slice(len(tens_ext), _dptr(_f64(tens).ravel()), _iptr(te),
+                                 _dptr(out), _iptr(se), _iptr(eb), ctypes.byref(ierr))
+        return out, ierr.value
+
+    def block_insert(self, tens, tens_ext, slc, slice_ext, ext_beg):
+        te, se, eb = _ints(tens_ext), _ints(slice_ext), _ints(ext_beg)
+        tens = _f64(tens).ravel().copy()
+        ierr = ctypes.c_int(0)
+        self.lib.orc_block_insert(len(tens_ext), _dptr(tens), _iptr(te),
+                                  _dptr(_f64(slc).ravel()), _iptr(se), _iptr(eb),
+                                  ctypes.byref(ierr))
+        return tens, ierr.value
+
     def perms(self, ptrn, ldims, rdims, ddims):
         """determine_index_permutations: (lo2n, ro2n, do2n, flags, M, N, K)."""
 
@@ -183,3 +233,50 @@ class RefLib:
                                     _dptr(_f64(t1)), ctypes.byref(ctypes.c_double(scale)),
                                     ctypes.byref(ierr))
         return ierr.value
+
+    def _c(self, v, ctype=ctypes.c_int):
+        return ctypes.byref(ctype(v))
+
+    def block_init(self, dims, t, sval, nthreads=1):
+        d = _ints(dims)
+        t = _f64(t).ravel().copy()
+        ierr = ctypes.c_int(0)
+        self.lib.tensor_block_init__(self._c(nthreads), _dptr(t), self._c(len(dims)), _iptr(d),
+                                     self._c(sval, ctypes.c_double), ctypes.byref(ierr))
+        return t, ierr.value
+
+    def block_scale(self, dims, t, s, nthreads=1):
+        d = _ints(dims)
+        t = _f64(t).ravel().copy()
+        ierr = ctypes.c_int(0)
+        self.lib.tensor_block_scale__(self._c(nthreads), _dptr(t), self._c(len(dims)), _iptr(d),
+                                      self._c(s, ctypes.c_double), ctypes.byref(ierr))
+        return t, ierr.value
+
+    def block_norm2(self, dims, t, nthreads=1):
+        d = _ints(dims)
+        t = _f64(t).ravel()
+        ierr = ctypes.c_int(0)
+        self.lib.tensor_block_norm2__.restype = ctypes.c_double
+        v = self.lib.tensor_block_norm2__(self._c(nthreads), _dptr(t), self._c(len(dims)),
+                                          _iptr(d), ctypes.byref(ierr))
+        return v, ierr.value
+
+    def block_slice(self, tens, tens_ext, slice_ext, ext_beg, nthreads=1):
+        te, se, eb = _ints(tens_ext), _ints(slice_ext), _ints(ext_beg)
+        n = max(int(np.prod(slice_ext)) if len(slice_ext) else 1, 1)
+        out = np.full(n, np.nan)
+        ierr = ctypes.c_int(0)
+        self.lib.tensor_block_slice__(self._c(nthreads), self._c(len(tens_ext)),
+                                      _dptr(_f64(tens).ravel()), _iptr(te), _dptr(out),
+                                      _iptr(se), _iptr(eb), ctypes.byref(ierr))
+        return out, ierr.value
+
+    def block_insert(self, tens, tens_ext, slc, slice_ext, ext_beg, nthreads=1):
+        te, se, eb = _ints(tens_ext), _ints(slice_ext), _ints(ext_beg)
+        tens = _f64(tens).ravel().copy()
+        ierr = ctypes.c_int(0)
+        self.lib.tensor_block_insert__(self._c(nthreads), self._c(len(tens_ext)), _dptr(tens),
+                                       _iptr(te), _dptr(_f64(slc).ravel()), _iptr(se),
+                                       _iptr(eb), ctypes.byref(ierr))
+        return tens, ierr.value

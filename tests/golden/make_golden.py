@@ -11,6 +11,7 @@ the few larger cases store the generator parameters (SplitMix64 uniform
 draws, bit-reproducible on any machine) and, for permutes, a SHA-256 of the
 reference output, since a permute result is bit-exact by definition.
 """
+import ctypes
 import hashlib
 import json
 import os
@@ -79,7 +80,6 @@ def main():
             np.zeros(0, np.int32), None)
         if dr < 0:
             # negative rank: call directly with the negative value
-            import ctypes
             out = np.zeros(max(lr + rr, 1), np.int32)
             a = np.array(aces, np.int32)
             e = ctypes.c_int(0)
@@ -255,5 +255,135 @@ def main():
     print("wrote", OUT, os.path.getsize(OUT), "bytes;", manifest["cases"])
 
 
+def slice_shapes(rng, count):
+    """Random (tens_ext, slice_ext, ext_beg) triples of rank 1..6 inside bounds,
+    at most 4096 elements."""
+    out = []
+    while len(out) < count:
+        rank = int(rng.integers(1, 7))
+        te = [int(x) for x in rng.integers(1, 8, size=rank)]
+        if np.prod(te) > 4096:
+            continue
+        se, eb = [], []
+        for t in te:
+            n = int(rng.integers(1, t + 1))
+            se.append(n)
+            eb.append(int(rng.integers(0, t - n + 1)))
+        out.append((te, se, eb))
+    return out
+
+
+def as_int(v):
+    """Position-coded slice data are small nonzero integers: stored exactly as
+    int32, with 0 for an element left unwritten (NaN)."""
+    v = np.asarray(v)
+    out = np.where(np.isnan(v), 0, v).astype(np.int32)
+    assert np.array_equal(np.where(out == 0, np.nan, out.astype(np.float64)), v, equal_nan=True)
+    return out
+
+
+def blockops(ref):
+    """tensor_block_{init,scale,norm2,add,slice,insert}__ vectors, written to
+    reference_blockops.npz (kept apart so the contraction vectors above stay
+    byte-identical).  One OpenMP thread, so norm2 sums in ascending order."""
+    rng = np.random.default_rng(20261202)
+    arrays = {}
+    keys = []
+    ip = ctypes.POINTER(ctypes.c_int)
+    dp = ctypes.POINTER(ctypes.c_double)
+
+    def put(key, **kw):
+        for k, v in kw.items():
+            arrays[key + "_" + k] = np.asarray(v)
+        keys.append(key)
+
+    elem = [([], "scalar"), ([3, 4, 5], "normal"), ([7, 43], "normal"), ([256], "normal"),
+            ([2, 0, 3], "normal"), ([1, 1, 1, 1, 1, 1], "normal"), ([6, 5, 4, 3, 2, 1], "normal"),
+            ([-1], "neg")]
+    for i, (dims, kind) in enumerate(elem):
+        rank = -1 if kind == "neg" else len(dims)
+        n = max(int(np.prod(dims)) if dims and kind != "neg" else 1, 1)
+        x = rng.standard_normal(n)
+        y = rng.standard_normal(n)
+        d = dims if kind != "neg" else []
+        dd = np.array(d if d else [0], np.int32)
+        put("shape%03d" % i, rank=[rank], dims=dd, x=x, y=y)
+        for op, arg in (("init", 2.5), ("init", -1.25), ("scale", 0.3), ("scale", -2.0),
+                        ("scale", 1.0), ("norm2", None), ("add", 0.7), ("add", 1.0 + 5e-14),
+                        ("add", 1.0 - 2e-13), ("add", 1.0)):
+            t = x.copy()
+            e = ctypes.c_int(0)
+            one = ctypes.byref(ctypes.c_int(1))
+            rk = ctypes.byref(ctypes.c_int(rank))
+            val = None
+            if op == "init":
+                ref.lib.tensor_block_init__(one, t.ctypes.data_as(dp), rk, dd.ctypes.data_as(ip),
+                                            ctypes.byref(ctypes.c_double(arg)), ctypes.byref(e))
+            elif op == "scale":
+                ref.lib.tensor_block_scale__(one, t.ctypes.data_as(dp), rk,
+                                             dd.ctypes.data_as(ip),
+                                             ctypes.byref(ctypes.c_double(arg)), ctypes.byref(e))
+            elif op == "norm2":
+                ref.lib.tensor_block_norm2__.restype = ctypes.c_double
+                val = ref.lib.tensor_block_norm2__(one, t.ctypes.data_as(dp), rk,
+                                                   dd.ctypes.data_as(ip), ctypes.byref(e))
+            else:
+                ref.lib.tensor_block_add__(one, rk, dd.ctypes.data_as(ip), t.ctypes.data_as(dp),
+                                           y.ctypes.data_as(dp),
+                                           ctypes.byref(ctypes.c_double(arg)), ctypes.byref(e))
+            key = "%s%03d" % (op, len([k for k in keys if k.startswith(op)]))
+            put(key, shape=[i], arg=[np.nan if arg is None else arg],
+                out=t if op != "norm2" else [val], ierr=[e.value])
+
+    shapes = slice_shapes(rng, 40)
+    shapes += [([5], [5], [0]), ([9, 7], [1, 7], [4, 0]), ([9, 7], [9, 1], [0, 6]),
+               ([4, 4, 4, 4], [4, 4, 2, 1], [0, 0, 1, 3]), ([3, 5, 2, 6, 2, 4], [3, 5, 2, 6, 2, 4],
+               [0, 0, 0, 0, 0, 0]), ([8, 1, 8], [2, 1, 3], [6, 0, 5])]
+    for j, shp in enumerate(shapes + [([], [], []), ([3], [2], [1])]):
+        te, se, eb = shp
+        neg = j == len(shapes) + 1
+        # distinct values name their position: tens(l) = l + 1, slice(l) = -(l + 1)
+        nt = max(int(np.prod(te)) if te else 1, 1)
+        tens = 1.0 + np.arange(nt, dtype=np.float64)
+        ns = max(int(np.prod(se)) if se else 1, 1)
+        slc = -1.0 - np.arange(ns, dtype=np.float64)
+        tea, sea, eba = (np.array(v if v else [0], np.int32) for v in (te, se, eb))
+        rk = ctypes.byref(ctypes.c_int(-1 if neg else len(te)))
+        one = ctypes.byref(ctypes.c_int(1))
+        out = np.full(ns, np.nan)
+        e1 = ctypes.c_int(0)
+        ref.lib.tensor_block_slice__(one, rk, tens.ctypes.data_as(dp), tea.ctypes.data_as(ip),
+                                     out.ctypes.data_as(dp), sea.ctypes.data_as(ip),
+                                     eba.ctypes.data_as(ip), ctypes.byref(e1))
+        ins = tens.copy()
+        e2 = ctypes.c_int(0)
+        ref.lib.tensor_block_insert__(one, rk, ins.ctypes.data_as(dp), tea.ctypes.data_as(ip),
+                                      slc.ctypes.data_as(dp), sea.ctypes.data_as(ip),
+                                      eba.ctypes.data_as(ip), ctypes.byref(e2))
+        put("slc%03d" % j, rank=[-1 if neg else len(te)], tens_ext=tea, slice_ext=sea,
+            ext_beg=eba, slice_out=as_int(out), insert_out=as_int(ins),
+            ierr=[e1.value, e2.value])
+
+    path = os.path.join(HERE, "reference_blockops.npz")
+    np.savez_compressed(path, **arrays)
+    with open(MANIFEST) as f:
+        manifest = json.load(f)
+    manifest["blockops"] = dict(
+        file="reference_blockops.npz", numpy_seed=20261202, cases=len(keys),
+        note="tensor_block_{init,scale,norm2,add}__ with nthreads=1; slice/insert with the "
+             "default OpenMP team; empty slices are left out (the reference's "
+             "divide_segment_ leaves its segment table unset for them)")
+    with open(MANIFEST, "w") as f:
+        json.dump(manifest, f, indent=1)
+    print("wrote", path, os.path.getsize(path), "bytes;", len(keys), "block-op cases")
+
+
 if __name__ == "__main__":
-    main()
+    # `blockops` writes only the block-operation vectors: a few contraction
+    # outputs of the 8-thread reference GEMM vary in their last bit between
+    # runs, so the committed contraction vectors are not rewritten with them.
+    if sys.argv[1:] == ["blockops"]:
+        blockops(RefLib())
+    else:
+        main()
+        blockops(RefLib())
