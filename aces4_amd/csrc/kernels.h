@@ -5,6 +5,8 @@
 
 #include <cstdint>
 
+#include "plan.h"
+
 namespace sipgpu {
 
 // ---------------------------------------------------------------- permute --
@@ -116,5 +118,30 @@ hipError_t launch_energy_sum(const EnergyArgs &a, const double *x, const double 
 hipError_t launch_fill_random(double *out, long long n, unsigned long long seed,
                               unsigned long long stream, int dist, double a, double b,
                               hipStream_t s);
+
+// ------------------------------------------------------------- block ops --
+// Elementwise block operations (tensor_dil_omp.F90:144-219, :394-436):
+// fill a = s, scale a *= s, scaled add a = a + b * s (product and sum rounded
+// separately), add a = a + b; n elements, nothing done for n <= 0.
+enum BlockOp { kOpFill = 0, kOpScale = 1, kOpAddScaled = 2, kOpAdd = 3 };
+hipError_t launch_block_op(int op, double *a, const double *b, long long n, double s,
+                           hipStream_t st);
+// out[0] = sum x[i]^2 over n >= 0 elements: per-workgroup partials (at most
+// kNormBlocks, in `part`) then one workgroup; the order depends on n only.
+constexpr int kNormBlocks = 1024;
+int norm2_blocks(long long n);
+hipError_t launch_block_norm2(const double *x, long long n, double *out, double *part,
+                              hipStream_t st);
+// A slice plan in kernel form: n slice elements, rank >= 1 dims.
+struct SliceArgs {
+    int rank, n;
+    FastDiv ext[kSliceMaxRank];
+    int tstride[kSliceMaxRank];
+};
+SliceArgs make_slice_args(const SlicePlan &p);
+// insert = false: slc[l] = tens[off(l)]; true: tens[off(l)] = slc[l], where
+// `tens` points at the slice's first element (the plan's base).
+hipError_t launch_slice(const SliceArgs &a, bool insert, double *tens, double *slc,
+                        hipStream_t st);
 
 } // namespace sipgpu

@@ -311,4 +311,50 @@ int plan_permute(int rank, const int *dims, const int *dim_transp,
     return 0;
 }
 
+int plan_slice(int rank, const int *tens_ext, const int *slice_ext, const int *ext_beg,
+               SlicePlan &p) {
+    p = SlicePlan();
+    if (rank < 0 || rank > kRefMaxRank) return SIPGPU_ERANK;
+    const int64_t kLimit = int64_t(1) << 31;
+    int64_t tsize = 1, count = 1, tstride = 1;
+    for (int i = 0; i < rank; ++i) {
+        if (ext_beg[i] < 0 || slice_ext[i] < 0 || tens_ext[i] < 0 ||
+            int64_t(ext_beg[i]) + slice_ext[i] > tens_ext[i])
+            return SIPGPU_EARG;
+        tsize *= tens_ext[i];
+        count *= slice_ext[i];
+        if (tsize >= kLimit || count >= kLimit) return SIPGPU_ESIZE;
+    }
+    p.tsize = tsize;
+    p.count = count;
+    if (count == 0) return 0;
+    // Canonical dims: element offsets grow with the slice's digits, so
+    // element 0 is the lowest tensor offset touched and the last the highest.
+    int cr = 0;
+    int64_t ce[kRefMaxRank], cs[kRefMaxRank];
+    for (int i = 0; i < rank; ++i) {
+        p.base += int64_t(ext_beg[i]) * tstride;
+        if (slice_ext[i] > 1) {
+            if (cr > 0 && cs[cr - 1] * ce[cr - 1] == tstride) {
+                ce[cr - 1] *= slice_ext[i]; // continues the previous dim's run
+            } else {
+                ce[cr] = slice_ext[i];
+                cs[cr] = tstride;
+                ++cr;
+            }
+        }
+        tstride *= tens_ext[i];
+    }
+    if (cr > kSliceMaxRank) return SIPGPU_ERANK; // needs rank > 8; the SIP stops at 6
+    p.rank = cr;
+    int64_t last = 0;
+    for (int i = 0; i < cr; ++i) {
+        p.ext[i] = (int)ce[i];
+        p.tstride[i] = cs[i];
+        last += (ce[i] - 1) * cs[i];
+    }
+    p.span = last + 1;
+    return 0;
+}
+
 } // namespace sipgpu

@@ -14,6 +14,8 @@
 #include <string>
 #include <vector>
 
+#include "fastdiv.h"
+
 namespace sipgpu {
 
 constexpr int kMaxRank = 6;     // MAX_RANK of the SIP (aces_defs.h:13)
@@ -76,5 +78,30 @@ struct PermutePlan {
 // Returns 0, SIPGPU_EPERM or SIPGPU_ESIZE.
 int plan_permute(int rank, const int *dims, const int *dim_transp,
                  PermutePlan &out);
+
+// Slice extraction / insertion (tensor_dil_omp.F90:258-368) in canonical
+// form: slice dims of extent 1 dropped, and a dim merged into the previous
+// one where the slice's element order stays contiguous in the tensor block.
+// Slice element l (column-major digits d_i over ext) is tensor element
+// base + sum_i d_i * tstride[i].
+constexpr int kSliceMaxRank = 8;
+struct SlicePlan {
+    int rank = 0;                         // canonical dims (0: one element)
+    int ext[kSliceMaxRank] = {0};         // their slice extents
+    int64_t tstride[kSliceMaxRank] = {0}; // their tensor strides
+    int64_t base = 0;                     // tensor offset of slice element 0
+    int64_t count = 0;                    // slice elements (0: empty slice)
+    int64_t span = 0;                     // tensor elements base .. last touched, inclusive
+    int64_t tsize = 0;                    // tensor elements
+};
+
+// Rank 0 is the reference's scalar case (one element each side).  Checks
+// every dim: 0 <= ext_beg, 0 <= slice_ext and ext_beg + slice_ext <=
+// tens_ext (the reference checks none of this and would read or write out
+// of bounds).  Returns 0, SIPGPU_ERANK (rank < 0 or > kRefMaxRank, or more
+// than kSliceMaxRank canonical dims), SIPGPU_EARG (a bound violated) or
+// SIPGPU_ESIZE (a block of 2^31 elements or more).
+int plan_slice(int rank, const int *tens_ext, const int *slice_ext, const int *ext_beg,
+               SlicePlan &out);
 
 } // namespace sipgpu

@@ -1512,6 +1512,29 @@ int plan_default_locked(Context &c, const sipgpu_contraction *list, int n,
     return 0;
 }
 
+// Elements of a block for the elementwise operations: rank 0 is one element
+// and a product that is not positive is none, as the reference's loops run
+// over 0 .. product - 1 (tensor_dil_omp.F90:167-178).  A product beyond 2^40
+// elements is rejected.
+int block_elems(int rank, const int *dims, long long &n) {
+    n = 1;
+    if (rank < 0) return SIPGPU_ERANK;
+    if (rank > 0 && !dims) return SIPGPU_EARG;
+    for (int i = 0; i < rank; ++i) {
+        n *= dims[i];
+        if (n == 0) break;
+        if (n > (1LL << 40) || n < -(1LL << 40)) return SIPGPU_ESIZE;
+    }
+    if (n < 0) n = 0;
+    return 0;
+}
+
+// tensor_block_add_ applies the factor unless it lies within 1e-13 of one,
+// except for scalars, which are always multiplied (tensor_dil_omp.F90:415-431).
+int add_op(int rank, double factor) {
+    return rank == 0 || std::fabs(factor - 1.0) > 1e-13 ? sipgpu::kOpAddScaled : sipgpu::kOpAdd;
+}
+
 } // namespace
 
 int sipgpu::rt::ensure_ready() {
@@ -1610,6 +1633,161 @@ void tensor_block_contract__(int *nthreads, int *ptrn, double *l, int *lrank, in
     e = hipMemcpyAsync(d, c.stage_d.p, dsz * 8, hipMemcpyDeviceToHost, s);
     if (e == hipSuccess) e = hipStreamSynchronize(s);
     if (e != hipSuccess) *ierr = hip_fail(e, "tensor_block_contract__ download");
+}
+
+
+// The remaining kernels of the reference module.  Host blocks are staged
+// through the library's device buffers; every call is synchronous.
+
+void tensor_block_init__(int *nthreads, double *tens, int *tens_rank, int *dims, double *sval,
+                         int *ierr) {
+    (void)nthreads;
+    *ierr = 0;
+    if (*tens_rank < 0) { *ierr = -1; return; }
+    long long n = 0;
+    if (int rc = block_elems(*tens_rank, dims, n)) { *ierr = rc; return; }
+    if (n == 0) return;
+    if (int rc = sipgpu::rt::refuse_in_capture("a host block operation")) { *ierr = rc; return; }
+    Context &c = ctx();
+    std::lock_guard<std::mutex> g(c.mu);
+    int rc = ensure_init(c);
+    if (!rc) rc = c.stage_d.reserve(n * 8);
+    if (rc) { *ierr = rc; return; }
+    hipStream_t s = c.stream.s;
+    double *d = static_cast<double *>(c.stage_d.p);
+    hipError_t e = launch_block_op(kOpFill, d, d, n, *sval, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(tens, d, n * 8, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) *ierr = hip_fail(e, "tensor_block_init__");
+}
+
+void tensor_block_scale__(int *nthreads, double *tens, int *tens_rank, int *dims,
+                          double *scale_fac, int *ierr) {
+    (void)nthreads;
+    *ierr = 0;
+    if (*tens_rank < 0) { *ierr = -1; return; }
+    long long n = 0;
+    if (int rc = block_elems(*tens_rank, dims, n)) { *ierr = rc; return; }
+    if (n == 0) return;
+    if (int rc = sipgpu::rt::refuse_in_capture("a host block operation")) { *ierr = rc; return; }
+    Context &c = ctx();
+    std::lock_guard<std::mutex> g(c.mu);
+    int rc = ensure_init(c);
+    if (!rc) rc = c.stage_d.reserve(n * 8);
+    if (rc) { *ierr = rc; return; }
+    hipStream_t s = c.stream.s;
+    double *d = static_cast<double *>(c.stage_d.p);
+    hipError_t e = hipMemcpyAsync(d, tens, n * 8, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = launch_block_op(kOpScale, d, d, n, *scale_fac, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(tens, d, n * 8, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) *ierr = hip_fail(e, "tensor_block_scale__");
+}
+
+double tensor_block_norm2__(int *nthreads, double *tens, int *tens_rank, int *dims, int *ierr) {
+    (void)nthreads;
+    *ierr = 0;
+    if (*tens_rank < 0) { *ierr = -1; return 0.0; }
+    long long n = 0;
+    if (int rc = block_elems(*tens_rank, dims, n)) { *ierr = rc; return 0.0; }
+    if (n == 0) return 0.0;
+    if (int rc = sipgpu::rt::refuse_in_capture("a host block operation")) { *ierr = rc; return 0.0; }
+    Context &c = ctx();
+    std::lock_guard<std::mutex> g(c.mu);
+    int rc = ensure_init(c);
+    if (!rc) rc = c.stage_l.reserve(n * 8);
+    if (!rc) rc = c.stage_r.reserve(8);
+    if (rc) { *ierr = rc; return 0.0; }
+    hipStream_t s = c.stream.s;
+    double *x = static_cast<double *>(c.stage_l.p), *r = static_cast<double *>(c.stage_r.p);
+    double v = 0.0;
+    hipError_t e = hipMemcpyAsync(x, tens, n * 8, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = launch_block_norm2(x, n, r, c.stream.scratch, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(&v, r, 8, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) { *ierr = hip_fail(e, "tensor_block_norm2__"); return 0.0; }
+    return v;
+}
+
+} // extern "C"
+
+namespace {
+
+// Shared body of tensor_block_slice__ / tensor_block_insert__: the span of
+// the tensor block the slice touches is staged, never the whole block.
+int host_slice(bool insert, int rank, double *tens, const int *tens_ext, double *slc,
+               const int *slice_ext, const int *ext_beg) {
+    if (rank < 0) return 1; // invalid number of dimensions (tensor_dil_omp.F90:306)
+    SlicePlan p;
+    if (int rc = plan_slice(rank, tens_ext, slice_ext, ext_beg, p))
+        return fail(rc, "slice outside the tensor block or too large");
+    // an empty slice: the reference's divide_segment_ rejects the empty range
+    // (tensor_dil_omp.F90:1248-1254) and its ierr is returned
+    if (p.count == 0) return -1;
+    if (int rc = sipgpu::rt::refuse_in_capture("a host block operation")) return rc;
+    Context &c = ctx();
+    std::lock_guard<std::mutex> g(c.mu);
+    int rc = ensure_init(c);
+    if (!rc) rc = c.stage_l.reserve(p.span * 8);
+    if (!rc) rc = c.stage_d.reserve(p.count * 8);
+    if (rc) return rc;
+    hipStream_t s = c.stream.s;
+    double *t = static_cast<double *>(c.stage_l.p), *x = static_cast<double *>(c.stage_d.p);
+    const SliceArgs a = make_slice_args(p);
+    hipError_t e = hipMemcpyAsync(t, tens + p.base, p.span * 8, hipMemcpyHostToDevice, s);
+    if (insert) {
+        if (e == hipSuccess) e = hipMemcpyAsync(x, slc, p.count * 8, hipMemcpyHostToDevice, s);
+        if (e == hipSuccess) e = launch_slice(a, true, t, x, s);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(tens + p.base, t, p.span * 8, hipMemcpyDeviceToHost, s);
+    } else {
+        if (e == hipSuccess) e = launch_slice(a, false, t, x, s);
+        if (e == hipSuccess) e = hipMemcpyAsync(slc, x, p.count * 8, hipMemcpyDeviceToHost, s);
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) return hip_fail(e, insert ? "tensor_block_insert__" : "tensor_block_slice__");
+    return 0;
+}
+
+} // namespace
+
+extern "C" {
+
+void tensor_block_slice__(int *nthreads, int *dim_num, double *tens, int *tens_ext, double *slice,
+                          int *slice_ext, int *ext_beg, int *ierr) {
+    (void)nthreads;
+    *ierr = host_slice(false, *dim_num, tens, tens_ext, slice, slice_ext, ext_beg);
+}
+
+void tensor_block_insert__(int *nthreads, int *dim_num, double *tens, int *tens_ext,
+                           double *slice, int *slice_ext, int *ext_beg, int *ierr) {
+    (void)nthreads;
+    *ierr = host_slice(true, *dim_num, tens, tens_ext, slice, slice_ext, ext_beg);
+}
+
+void tensor_block_add__(int *nthreads, int *dim_num, int *dim_extents, double *tens0,
+                        double *tens1, double *scale_fac, int *ierr) {
+    (void)nthreads;
+    *ierr = 0;
+    if (*dim_num < 0) { *ierr = -1; return; }
+    long long n = 0;
+    if (int rc = block_elems(*dim_num, dim_extents, n)) { *ierr = rc; return; }
+    if (n == 0) return;
+    if (int rc = sipgpu::rt::refuse_in_capture("a host block operation")) { *ierr = rc; return; }
+    Context &c = ctx();
+    std::lock_guard<std::mutex> g(c.mu);
+    int rc = ensure_init(c);
+    if (!rc) rc = c.stage_d.reserve(n * 8);
+    if (!rc) rc = c.stage_l.reserve(n * 8);
+    if (rc) { *ierr = rc; return; }
+    hipStream_t s = c.stream.s;
+    double *d = static_cast<double *>(c.stage_d.p), *x = static_cast<double *>(c.stage_l.p);
+    hipError_t e = hipMemcpyAsync(d, tens0, n * 8, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(x, tens1, n * 8, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = launch_block_op(add_op(*dim_num, *scale_fac), d, x, n, *scale_fac, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(tens0, d, n * 8, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) *ierr = hip_fail(e, "tensor_block_add__");
 }
 
 // ---------------------------------------------------------------------------
@@ -2176,6 +2354,112 @@ int sipgpu_fill_random(sipgpu_stream_t s, double *out, long long n, unsigned lon
     int rc = ensure_init(c);
     if (rc) return rc;
     HIP_TRY(launch_fill_random(out, n, seed, stream, dist, a, b, resolve(c, s).s));
+    return 0;
+}
+
+int sipgpu_block_init(sipgpu_stream_t s, double *t, int rank, const int *dims, double value) {
+    long long n = 0;
+    if (int rc = block_elems(rank, dims, n)) return fail(rc, "block_init: bad rank or extents");
+    if (n == 0) return 0;
+    if (!t) return fail(SIPGPU_EARG, "block_init: null block");
+    Context &c = ctx();
+    std::lock_guard<std::mutex> g(c.mu);
+    if (int rc = ensure_init(c)) return rc;
+    HIP_TRY(launch_block_op(kOpFill, t, t, n, value, resolve(c, s).s));
+    return 0;
+}
+
+int sipgpu_block_scale(sipgpu_stream_t s, double *t, int rank, const int *dims, double factor) {
+    long long n = 0;
+    if (int rc = block_elems(rank, dims, n)) return fail(rc, "block_scale: bad rank or extents");
+    if (n == 0) return 0;
+    if (!t) return fail(SIPGPU_EARG, "block_scale: null block");
+    Context &c = ctx();
+    std::lock_guard<std::mutex> g(c.mu);
+    if (int rc = ensure_init(c)) return rc;
+    HIP_TRY(launch_block_op(kOpScale, t, t, n, factor, resolve(c, s).s));
+    return 0;
+}
+
+int sipgpu_block_add(sipgpu_stream_t s, double *t0, const double *t1, int rank, const int *dims,
+                     double factor) {
+    long long n = 0;
+    if (int rc = block_elems(rank, dims, n)) return fail(rc, "block_add: bad rank or extents");
+    if (n == 0) return 0;
+    if (!t0 || !t1) return fail(SIPGPU_EARG, "block_add: null block");
+    Context &c = ctx();
+    std::lock_guard<std::mutex> g(c.mu);
+    if (int rc = ensure_init(c)) return rc;
+    HIP_TRY(launch_block_op(add_op(rank, factor), t0, t1, n, factor, resolve(c, s).s));
+    return 0;
+}
+
+int sipgpu_block_norm2(sipgpu_stream_t s, const double *t, int rank, const int *dims,
+                       double *result) {
+    long long n = 0;
+    if (int rc = block_elems(rank, dims, n)) return fail(rc, "block_norm2: bad rank or extents");
+    if (!result || (n > 0 && !t)) return fail(SIPGPU_EARG, "block_norm2: null pointer");
+    Context &c = ctx();
+    std::lock_guard<std::mutex> g(c.mu);
+    if (int rc = ensure_init(c)) return rc;
+    sipgpu_stream_s &st = resolve(c, s);
+    if (int rc = ensure_stream_scratch(st)) return rc;
+    HIP_TRY(launch_block_norm2(t, n, result, st.scratch, st.s));
+    return 0;
+}
+
+} // extern "C"
+
+namespace {
+
+int device_slice(sipgpu_stream_t s, bool insert, int rank, double *tens, const int *tens_ext,
+                 double *slc, const int *slice_ext, const int *ext_beg) {
+    if (rank < 0) return fail(SIPGPU_ERANK, "slice: negative rank");
+    if (rank > 0 && (!tens_ext || !slice_ext || !ext_beg))
+        return fail(SIPGPU_EARG, "slice: null extents");
+    SlicePlan p;
+    if (int rc = plan_slice(rank, tens_ext, slice_ext, ext_beg, p))
+        return fail(rc, "slice outside the tensor block or too large");
+    if (p.count == 0) return 0;
+    if (!tens || !slc) return fail(SIPGPU_EARG, "slice: null block");
+    Context &c = ctx();
+    std::lock_guard<std::mutex> g(c.mu);
+    if (int rc = ensure_init(c)) return rc;
+    HIP_TRY(launch_slice(make_slice_args(p), insert, tens + p.base, slc, resolve(c, s).s));
+    return 0;
+}
+
+} // namespace
+
+extern "C" {
+
+int sipgpu_block_slice(sipgpu_stream_t s, int rank, const double *tens, const int *tens_ext,
+                       double *slice, const int *slice_ext, const int *ext_beg) {
+    return device_slice(s, false, rank, const_cast<double *>(tens), tens_ext, slice, slice_ext,
+                        ext_beg);
+}
+
+int sipgpu_block_insert(sipgpu_stream_t s, int rank, double *tens, const int *tens_ext,
+                        const double *slice, const int *slice_ext, const int *ext_beg) {
+    return device_slice(s, true, rank, tens, tens_ext, const_cast<double *>(slice), slice_ext,
+                        ext_beg);
+}
+
+int sipgpu_plan_slice(int rank, const int *tens_ext, const int *slice_ext, const int *ext_beg,
+                      int *canon_rank, int *ext, long long *tstride, long long *info) {
+    if (rank < 0) return fail(SIPGPU_ERANK, "slice: negative rank");
+    if (!canon_rank || !ext || !tstride || !info ||
+        (rank > 0 && (!tens_ext || !slice_ext || !ext_beg)))
+        return fail(SIPGPU_EARG, "null argument");
+    SlicePlan p;
+    if (int rc = plan_slice(rank, tens_ext, slice_ext, ext_beg, p))
+        return fail(rc, "slice outside the tensor block or too large");
+    *canon_rank = p.rank;
+    for (int i = 0; i < p.rank; ++i) { ext[i] = p.ext[i]; tstride[i] = p.tstride[i]; }
+    info[0] = p.base;
+    info[1] = p.count;
+    info[2] = p.span;
+    info[3] = p.tsize;
     return 0;
 }
 

@@ -45,7 +45,10 @@ EXPORTED = [
     "sipgpu_batch_execute_stages", "sipgpu_graph_begin", "sipgpu_graph_end",
     "sipgpu_graph_launch", "sipgpu_graph_nodes", "sipgpu_graph_destroy", "sipgpu_batch_tune",
     "sipgpu_batch_plan_choice", "sipgpu_tune_save", "sipgpu_tune_load", "sipgpu_tune_clear",
-    "sipgpu_batch_plan_split_tol",
+    "sipgpu_batch_plan_split_tol", "tensor_block_init__", "tensor_block_scale__",
+    "tensor_block_norm2__", "tensor_block_slice__", "tensor_block_insert__", "tensor_block_add__",
+    "sipgpu_block_init", "sipgpu_block_scale", "sipgpu_block_add", "sipgpu_block_norm2",
+    "sipgpu_block_slice", "sipgpu_block_insert", "sipgpu_plan_slice",
     "sipgpu_cache_create", "sipgpu_cache_destroy", "sipgpu_cache_insert", "sipgpu_cache_acquire",
     "sipgpu_cache_release", "sipgpu_cache_read", "sipgpu_cache_flush", "sipgpu_cache_evict",
     "sipgpu_cache_get_stats", "sipgpu_server_create", "sipgpu_server_submit",
@@ -199,6 +202,16 @@ def lib():
                                          ctypes.c_double]
         L.sipgpu_measure_peaks.argtypes = [ctypes.POINTER(ctypes.c_double),
                                            ctypes.POINTER(ctypes.c_double), ctypes.c_size_t]
+        L.tensor_block_norm2__.restype = ctypes.c_double
+        L.sipgpu_block_init.argtypes = [_vp, _vp, ctypes.c_int, _ip, ctypes.c_double]
+        L.sipgpu_block_scale.argtypes = [_vp, _vp, ctypes.c_int, _ip, ctypes.c_double]
+        L.sipgpu_block_add.argtypes = [_vp, _vp, _vp, ctypes.c_int, _ip, ctypes.c_double]
+        L.sipgpu_block_norm2.argtypes = [_vp, _vp, ctypes.c_int, _ip, _vp]
+        for f in ("sipgpu_block_slice", "sipgpu_block_insert"):
+            getattr(L, f).argtypes = [_vp, ctypes.c_int, _vp, _ip, _vp, _ip, _ip]
+        L.sipgpu_plan_slice.argtypes = [ctypes.c_int, _ip, _ip, _ip, _ip, _ip,
+                                        ctypes.POINTER(ctypes.c_longlong),
+                                        ctypes.POINTER(ctypes.c_longlong)]
         _lib = L
     return _lib
 
@@ -254,6 +267,86 @@ def tensor_block_copy(dims, dim_transp, x):
                               ctypes.byref(ctypes.c_int(len(dims))), _iptr(d), _iptr(t),
                               x.ctypes.data_as(_dp), out.ctypes.data_as(_dp), ctypes.byref(ierr))
     return out, ierr.value
+
+
+def _byref_i(v):
+    return ctypes.byref(ctypes.c_int(v))
+
+
+def _byref_d(v):
+    return ctypes.byref(ctypes.c_double(v))
+
+
+def _host(x):
+    return np.ascontiguousarray(x, dtype=np.float64).ravel().copy()
+
+
+def tensor_block_init(dims, t, sval, rank=None):
+    """Fill a host block on the GPU; returns (block, ierr).  `rank` overrides
+    len(dims) (to pass a negative rank)."""
+    d = _ints(dims)
+    t = _host(t)
+    ierr = ctypes.c_int(0)
+    lib().tensor_block_init__(_byref_i(8), t.ctypes.data_as(_dp),
+                              _byref_i(len(dims) if rank is None else rank), _iptr(d),
+                              _byref_d(sval), ctypes.byref(ierr))
+    return t, ierr.value
+
+
+def tensor_block_scale(dims, t, factor, rank=None):
+    d = _ints(dims)
+    t = _host(t)
+    ierr = ctypes.c_int(0)
+    lib().tensor_block_scale__(_byref_i(8), t.ctypes.data_as(_dp),
+                               _byref_i(len(dims) if rank is None else rank), _iptr(d),
+                               _byref_d(factor), ctypes.byref(ierr))
+    return t, ierr.value
+
+
+def tensor_block_norm2(dims, t, rank=None):
+    d = _ints(dims)
+    t = _host(t)
+    ierr = ctypes.c_int(0)
+    v = lib().tensor_block_norm2__(_byref_i(8), t.ctypes.data_as(_dp),
+                                   _byref_i(len(dims) if rank is None else rank), _iptr(d),
+                                   ctypes.byref(ierr))
+    return v, ierr.value
+
+
+def tensor_block_add(dims, t0, t1, factor, rank=None):
+    d = _ints(dims)
+    t0 = _host(t0)
+    t1 = _host(t1)
+    ierr = ctypes.c_int(0)
+    lib().tensor_block_add__(_byref_i(8), _byref_i(len(dims) if rank is None else rank),
+                             _iptr(d), t0.ctypes.data_as(_dp), t1.ctypes.data_as(_dp),
+                             _byref_d(factor), ctypes.byref(ierr))
+    return t0, ierr.value
+
+
+def tensor_block_slice(tens, tens_ext, slice_ext, ext_beg, out=None, rank=None):
+    """Extract a slice of a host block on the GPU; returns (slice, ierr)."""
+    te, se, eb = _ints(tens_ext), _ints(slice_ext), _ints(ext_beg)
+    tens = np.ascontiguousarray(tens, dtype=np.float64).ravel()
+    n = max(int(np.prod(slice_ext)) if len(slice_ext) else 1, 1)
+    out = np.full(n, np.nan) if out is None else _host(out)
+    ierr = ctypes.c_int(0)
+    lib().tensor_block_slice__(_byref_i(8), _byref_i(len(tens_ext) if rank is None else rank),
+                               tens.ctypes.data_as(_dp), _iptr(te), out.ctypes.data_as(_dp),
+                               _iptr(se), _iptr(eb), ctypes.byref(ierr))
+    return out, ierr.value
+
+
+def tensor_block_insert(tens, tens_ext, slc, slice_ext, ext_beg, rank=None):
+    """Insert a slice into a host block on the GPU; returns (block, ierr)."""
+    te, se, eb = _ints(tens_ext), _ints(slice_ext), _ints(ext_beg)
+    tens = _host(tens)
+    slc = np.ascontiguousarray(slc, dtype=np.float64).ravel()
+    ierr = ctypes.c_int(0)
+    lib().tensor_block_insert__(_byref_i(8), _byref_i(len(tens_ext) if rank is None else rank),
+                                tens.ctypes.data_as(_dp), _iptr(te), slc.ctypes.data_as(_dp),
+                                _iptr(se), _iptr(eb), ctypes.byref(ierr))
+    return tens, ierr.value
 
 
 def tensor_block_contract(ptrn, l, ldims, r, rdims, ddims, d_init=None):
@@ -502,6 +595,57 @@ class Event:
         ms = ctypes.c_float(0)
         _check(lib().sipgpu_event_elapsed_ms(self.handle, later.handle, ctypes.byref(ms)))
         return ms.value
+
+
+def block_init(t, dims, value, stream=None, rank=None):
+    """Device block `t` (address) = value, enqueued on `stream`."""
+    _check(lib().sipgpu_block_init(stream.handle if stream else None, t,
+                                   len(dims) if rank is None else rank, _iptr(_ints(dims)),
+                                   float(value)))
+
+
+def block_scale(t, dims, factor, stream=None, rank=None):
+    _check(lib().sipgpu_block_scale(stream.handle if stream else None, t,
+                                    len(dims) if rank is None else rank, _iptr(_ints(dims)),
+                                    float(factor)))
+
+
+def block_add(t0, t1, dims, factor, stream=None, rank=None):
+    _check(lib().sipgpu_block_add(stream.handle if stream else None, t0, t1,
+                                  len(dims) if rank is None else rank, _iptr(_ints(dims)),
+                                  float(factor)))
+
+
+def block_norm2(t, dims, result, stream=None, rank=None):
+    """*result (device double) = sum of squares of device block `t`."""
+    _check(lib().sipgpu_block_norm2(stream.handle if stream else None, t,
+                                    len(dims) if rank is None else rank, _iptr(_ints(dims)),
+                                    result))
+
+
+def block_slice(tens, tens_ext, slc, slice_ext, ext_beg, stream=None):
+    _check(lib().sipgpu_block_slice(stream.handle if stream else None, len(tens_ext), tens,
+                                    _iptr(_ints(tens_ext)), slc, _iptr(_ints(slice_ext)),
+                                    _iptr(_ints(ext_beg))))
+
+
+def block_insert(tens, tens_ext, slc, slice_ext, ext_beg, stream=None):
+    _check(lib().sipgpu_block_insert(stream.handle if stream else None, len(tens_ext), tens,
+                                     _iptr(_ints(tens_ext)), slc, _iptr(_ints(slice_ext)),
+                                     _iptr(_ints(ext_beg))))
+
+
+def plan_slice(tens_ext, slice_ext, ext_beg):
+    """Host-only slice plan: (canonical extents, tensor strides, base, count,
+    span, tensor elements)."""
+    cr = ctypes.c_int(0)
+    ext = np.zeros(8, np.int32)
+    st = (ctypes.c_longlong * 8)()
+    info = (ctypes.c_longlong * 4)()
+    _check(lib().sipgpu_plan_slice(len(tens_ext), _iptr(_ints(tens_ext)), _iptr(_ints(slice_ext)),
+                                   _iptr(_ints(ext_beg)), ctypes.byref(cr), _iptr(ext), st, info))
+    r = cr.value
+    return [int(x) for x in ext[:r]], [int(x) for x in st[:r]], info[0], info[1], info[2], info[3]
 
 
 def permute(rank_dims, dim_transp, din, dout, stream=None):

@@ -66,6 +66,55 @@ void tensor_block_contract__(int *nthreads, int *ptrn, double *l, int *lrank,
                              int *ldims, double *r, int *rrank, int *rdims,
                              double *d, int *drank, int *ddims, int *ierr);
 
+/* The rest of the reference module's C interface (prototypes
+ * tensor_ops_c_prototypes.h:70-133).  Same conventions as above: host blocks,
+ * synchronous, nthreads accepted and ignored; device failures give a
+ * negative SIPGPU_E* ierr.  Element counts follow the reference: rank 0 is
+ * one element, and a product of extents that is not positive is no element. */
+
+/* Replaces tensor_block_init__ (tensor_dil_omp.F90:144-187; prototype
+ * tensor_ops_c_prototypes.h:70): every element = *sval.  ierr -1 for a
+ * negative rank. */
+void tensor_block_init__(int *nthreads, double *tens, int *tens_rank, int *dims, double *sval,
+                         int *ierr);
+
+/* Replaces tensor_block_scale__ (tensor_dil_omp.F90:189-219; prototype
+ * tensor_ops_c_prototypes.h:84): every element *= *scale_fac.  ierr -1 for
+ * a negative rank. */
+void tensor_block_scale__(int *nthreads, double *tens, int *tens_rank, int *dims,
+                          double *scale_fac, int *ierr);
+
+/* Replaces tensor_block_norm2__ (tensor_dil_omp.F90:221-256; prototype
+ * tensor_ops_c_prototypes.h:86): the sum of squares.  The reference's
+ * OpenMP reduction has no fixed order; this one is fixed for a given size.
+ * ierr -1 (result 0) for a negative rank. */
+double tensor_block_norm2__(int *nthreads, double *tens, int *tens_rank, int *dims, int *ierr);
+
+/* Replaces tensor_block_slice__ (tensor_dil_omp.F90:258-311; prototype
+ * tensor_ops_c_prototypes.h:100), called by Block::extract_slice
+ * (dynamic_data/block.cpp:294): slice(i) = tens(ext_beg + i) for every
+ * column-major multi-index i of the slice.  ierr: 1 negative dim_num, -1
+ * empty slice (as the reference's segment division reports it),
+ * SIPGPU_EARG for a slice that leaves the tensor block (which the reference
+ * does not check), SIPGPU_ESIZE for blocks of 2^31 elements or more.  Only
+ * the span of `tens` the slice touches is staged to the device. */
+void tensor_block_slice__(int *nthreads, int *dim_num, double *tens, int *tens_ext, double *slice,
+                          int *slice_ext, int *ext_beg, int *ierr);
+
+/* Replaces tensor_block_insert__ (tensor_dil_omp.F90:313-368; prototype
+ * tensor_ops_c_prototypes.h:117), called by Block::insert_slice
+ * (dynamic_data/block.cpp:320): tens(ext_beg + i) = slice(i).  Errors as
+ * tensor_block_slice__. */
+void tensor_block_insert__(int *nthreads, int *dim_num, double *tens, int *tens_ext,
+                           double *slice, int *slice_ext, int *ext_beg, int *ierr);
+
+/* Replaces tensor_block_add__ (tensor_dil_omp.F90:394-436; prototype
+ * tensor_ops_c_prototypes.h:133): tens0 += tens1 * s, the product and the
+ * sum rounded separately; for rank > 0 a factor within 1e-13 of one is not
+ * applied (tens0 += tens1).  ierr -1 for a negative rank. */
+void tensor_block_add__(int *nthreads, int *dim_num, int *dim_extents, double *tens0,
+                        double *tens1, double *scale_fac, int *ierr);
+
 /* ------------------------------------------------------------------------ */
 /* Native layer                                                              */
 /* ------------------------------------------------------------------------ */
@@ -258,6 +307,27 @@ int sipgpu_energy_sum(sipgpu_stream_t s, int rank, const int *dims, const double
                       double factor, double *out);
 
 /* ------------------------------------------------------------------------
+ * Block operations on device blocks, enqueued on `s` (asynchronous): the
+ * drop-in layer's init / scale / add / norm2 / slice / insert with device
+ * pointers, same element rules and arithmetic (tensor_dil_omp.F90:144-436).
+ * Errors: SIPGPU_ERANK for a negative rank, SIPGPU_EARG for null pointers or
+ * a slice outside the tensor block, SIPGPU_ESIZE for oversized blocks.  An
+ * empty block or slice is a successful no-op. */
+int sipgpu_block_init(sipgpu_stream_t s, double *t, int rank, const int *dims, double value);
+int sipgpu_block_scale(sipgpu_stream_t s, double *t, int rank, const int *dims, double factor);
+/* t0 += t1 * factor (a factor within 1e-13 of one is not applied for rank > 0). */
+int sipgpu_block_add(sipgpu_stream_t s, double *t0, const double *t1, int rank, const int *dims,
+                     double factor);
+/* *result (a device double) = sum of squares, in an order fixed by the
+ * element count (per-workgroup partials, then one workgroup). */
+int sipgpu_block_norm2(sipgpu_stream_t s, const double *t, int rank, const int *dims,
+                       double *result);
+int sipgpu_block_slice(sipgpu_stream_t s, int rank, const double *tens, const int *tens_ext,
+                       double *slice, const int *slice_ext, const int *ext_beg);
+int sipgpu_block_insert(sipgpu_stream_t s, int rank, double *tens, const int *tens_ext,
+                        const double *slice, const int *slice_ext, const int *ext_beg);
+
+/* ------------------------------------------------------------------------
  * Device block cache.  Replaces the SIP's BlockManager / CachedBlockMap /
  * LRUArrayPolicy residency logic (block_manager.cpp:106-206,
  * cached_block_map.cpp:31-147, lru_array_policy.h:37-96) for blocks in
@@ -398,6 +468,12 @@ int sipgpu_plan_tables(const int *ptrn, int lrank, const int *ldims, int rrank,
 /* Host-only introspection of a permute plan: info[0] canonical rank,
  * info[1] leading tile dims, info[2] row mode, info[3] plain copy. */
 int sipgpu_plan_permute(int rank, const int *dims, const int *dim_transp, int *info);
+/* Host-only introspection of a slice plan: the canonical dims (extent-1
+ * slice dims dropped, contiguous runs merged; at most 8), their slice
+ * extents and tensor strides, and info[0..3] = tensor offset of the first
+ * slice element, slice elements, tensor span touched, tensor elements. */
+int sipgpu_plan_slice(int rank, const int *tens_ext, const int *slice_ext, const int *ext_beg,
+                      int *canon_rank, int *ext, long long *tstride, long long *info);
 
 #ifdef __cplusplus
 }

@@ -146,3 +146,106 @@ def test_product_path_fails_loudly_without_gpu():
     assert ierr != 0
     _, ierr = sg.tensor_block_copy([2, 3], [1, 2, 1], np.ones(6))
     assert ierr != 0
+
+
+def _slice_offsets(te, se, eb):
+    """Tensor offsets of the slice's elements, in slice order (numpy closed form)."""
+    idx = np.arange(int(np.prod(te)), dtype=np.int64).reshape(te, order="F")
+    return idx[tuple(slice(b, b + n) for b, n in zip(eb, se))].ravel(order="F")
+
+
+@pytest.mark.parametrize("seed", range(10))
+def test_slice_plans_reproduce_offsets(seed):
+    rng = np.random.default_rng(seed)
+    rank = 1 + seed % 6
+    te = [int(v) for v in rng.integers(1, 7, size=rank)]
+    se = [int(rng.integers(1, t + 1)) for t in te]
+    if seed % 3 == 0:  # full leading dims: they merge
+        se[:2] = te[:2]
+    eb = [int(rng.integers(0, t - n + 1)) for t, n in zip(te, se)]
+    ext, st, base, count, span, tsize = sg.plan_slice(te, se, eb)
+    want = _slice_offsets(te, se, eb)
+    assert count == want.size and tsize == int(np.prod(te))
+    # canonical digits reproduce every offset
+    l = np.arange(count, dtype=np.int64)
+    off = np.full(count, base, dtype=np.int64)
+    for e, s in zip(ext, st):
+        off += (l % e) * s
+        l //= e
+    np.testing.assert_array_equal(off, want)
+    assert base == want.min() and span == want.max() - want.min() + 1
+    assert all(e > 1 for e in ext)
+    assert all(st[i + 1] != st[i] * ext[i] for i in range(len(ext) - 1))  # nothing left to merge
+
+
+def test_slice_plan_canonical_forms_and_errors():
+    assert sg.plan_slice([4, 5, 6], [4, 5, 6], [0, 0, 0])[:2] == ([120], [1])
+    assert sg.plan_slice([4, 5, 6], [4, 2, 1], [0, 3, 5])[:3] == ([8], [1], 12 + 100)
+    assert sg.plan_slice([4, 5, 6], [1, 1, 1], [3, 4, 5])[:5] == ([], [], 119, 1, 1)
+    assert sg.plan_slice([], [], [])[2:] == (0, 1, 1, 1)       # rank 0: one element
+    assert sg.plan_slice([4, 5], [0, 5], [0, 0])[3] == 0       # empty slice
+    for te, se, eb in (([4, 5], [3, 5], [2, 0]), ([4, 5], [1, 1], [-1, 0]),
+                       ([4, 5], [-1, 1], [0, 0])):
+        with pytest.raises(sg.SipGpuError) as ei:
+            sg.plan_slice(te, se, eb)
+        assert ei.value.code == -107
+    with pytest.raises(sg.SipGpuError) as ei:
+        sg.plan_slice([1 << 16, 1 << 16], [1, 1], [0, 0])
+    assert ei.value.code == -104
+
+
+def test_fastdiv_is_exact(tmp_path):
+    """fastdiv.h against the hardware divide: every n < 2^16 for divisors up
+    to 300, and boundary and random numerators up to 2^31 - 1 for divisors
+    across the 32-bit range."""
+    src = tmp_path / "fd.cpp"
+    src.write_text(r'''
+#include "fastdiv.h"
+#include <cstdio>
+#include <cstdint>
+using namespace sipgpu;
+static uint64_t st = 0x9E3779B97F4A7C15ull;
+static uint32_t rnd() { st ^= st << 13; st ^= st >> 7; st ^= st << 17; return uint32_t(st); }
+static int check(uint32_t d, uint32_t n) {
+    const FastDiv f = make_fastdiv(d);
+    if (fastdiv(n, f) != n / d) { std::printf("d=%u n=%u\n", d, n); return 1; }
+    return 0;
+}
+int main() {
+    int bad = 0;
+    for (uint32_t d = 1; d <= 300; ++d)
+        for (uint32_t n = 0; n < (1u << 16); ++n) bad += check(d, n);
+    const uint32_t top = (1u << 31) - 1;
+    for (int i = 0; i < 200000; ++i) {
+        uint32_t d = 1 + rnd() % top;
+        if (i & 1) d = 1 + d % 4096;
+        const uint32_t q = top / d;
+        bad += check(d, top) + check(d, rnd() % (top + 1u)) + check(d, d - 1) + check(d, d);
+        bad += check(d, q * d) + check(d, q * d - 1);
+    }
+    for (int s = 0; s < 31; ++s) {
+        const uint32_t p = 1u << s;
+        bad += check(p, top) + check(p + 1, top) + check(p - (s > 0), top);
+    }
+    std::printf("%d\n", bad);
+    return bad != 0;
+}
+''')
+    exe = tmp_path / "fd"
+    import subprocess
+    subprocess.run(["g++", "-O2", "-std=c++17", "-I", os.path.join(ROOT, "aces4_amd", "csrc"),
+                    str(src), "-o", str(exe)], check=True)
+    r = subprocess.run([str(exe)], capture_output=True, text=True)
+    assert r.returncode == 0, r.stdout[-2000:]
+
+
+def test_block_operations_fail_loudly_without_gpu():
+    if sg.device_count() > 0:
+        pytest.skip("a GPU is present")
+    assert sg.tensor_block_init([2, 3], np.zeros(6), 1.0)[1] != 0
+    assert sg.tensor_block_slice(np.arange(6.0), [2, 3], [1, 2], [1, 0])[1] != 0
+    # argument errors come first and keep the reference's codes
+    assert sg.tensor_block_init([2], np.zeros(2), 1.0, rank=-1)[1] == -1
+    assert sg.tensor_block_slice(np.arange(6.0), [2, 3], [1, 2], [1, 0], rank=-1)[1] == 1
+    assert sg.tensor_block_slice(np.arange(6.0), [2, 3], [0, 2], [1, 0])[1] == -1
+    assert sg.tensor_block_slice(np.arange(6.0), [2, 3], [2, 2], [1, 0])[1] == -107
