@@ -120,7 +120,7 @@ hipError_t launch_fill_random(double *out, long long n, unsigned long long seed,
                               hipStream_t s);
 
 // ------------------------------------------------------------- block ops --
-// Elementwise block operations (tensor_dil_omp.F90:144-219, :394-436):
+// Elementwise block operations (tensor_dil_omp.F90:144-228, :394-436):
 // fill a = s, scale a *= s, scaled add a = a + b * s (product and sum rounded
 // separately), add a = a + b; n elements, nothing done for n <= 0.
 enum BlockOp { kOpFill = 0, kOpScale = 1, kOpAddScaled = 2, kOpAdd = 3 };

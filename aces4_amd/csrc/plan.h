@@ -79,7 +79,7 @@ struct PermutePlan {
 int plan_permute(int rank, const int *dims, const int *dim_transp,
                  PermutePlan &out);
 
-// Slice extraction / insertion (tensor_dil_omp.F90:258-368) in canonical
+// Slice extraction / insertion (tensor_dil_omp.F90:271-392) in canonical
 // form: slice dims of extent 1 dropped, and a dim merged into the previous
 // one where the slice's element order stays contiguous in the tensor block.
 // Slice element l (column-major digits d_i over ext) is tensor element

@@ -1514,7 +1514,7 @@ int plan_default_locked(Context &c, const sipgpu_contraction *list, int n,
 
 // Elements of a block for the elementwise operations: rank 0 is one element
 // and a product that is not positive is none, as the reference's loops run
-// over 0 .. product - 1 (tensor_dil_omp.F90:167-178).  A product beyond 2^40
+// over 0 .. product - 1 (tensor_dil_omp.F90:172-187).  A product beyond 2^40
 // elements is rejected.
 int block_elems(int rank, const int *dims, long long &n) {
     n = 1;
@@ -1530,7 +1530,7 @@ int block_elems(int rank, const int *dims, long long &n) {
 }
 
 // tensor_block_add_ applies the factor unless it lies within 1e-13 of one,
-// except for scalars, which are always multiplied (tensor_dil_omp.F90:415-431).
+// except for scalars, which are always multiplied (tensor_dil_omp.F90:417-421).
 int add_op(int rank, double factor) {
     return rank == 0 || std::fabs(factor - 1.0) > 1e-13 ? sipgpu::kOpAddScaled : sipgpu::kOpAdd;
 }
@@ -1717,7 +1717,7 @@ namespace {
 // the tensor block the slice touches is staged, never the whole block.
 int host_slice(bool insert, int rank, double *tens, const int *tens_ext, double *slc,
                const int *slice_ext, const int *ext_beg) {
-    if (rank < 0) return 1; // invalid number of dimensions (tensor_dil_omp.F90:306)
+    if (rank < 0) return 1; // invalid number of dimensions (tensor_dil_omp.F90:325)
     SlicePlan p;
     if (int rc = plan_slice(rank, tens_ext, slice_ext, ext_beg, p))
         return fail(rc, "slice outside the tensor block or too large");

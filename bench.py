@@ -31,6 +31,7 @@ from aces4_amd import sipgpu as sg  # noqa: E402
 from aces4_amd import workloads as wl  # noqa: E402
 
 FP64_PEAK_TFLOPS = 78.6  # MI355X dense FP64 matrix peak
+HBM_PEAK_GBS = 8000.0    # MI355X HBM3E peak
 PROFILE_TRAFFIC = os.path.join(ROOT, "profiles", "ring_gemm_traffic.txt")
 
 
@@ -169,6 +170,54 @@ def permute_bench(reps=20):
         out[name] = {"dims": dims, "perm": perm, "ms": ms, "GBps": 16.0 * n / (ms * 1e-3) / 1e9}
         din.free()
         dout.free()
+    return out
+
+
+def blockops_bench(reps=10):
+    """Achieved HBM GB/s of the block kernels (algorithmic bytes: slice and
+    insert 16 B per slice element, add 24 B, scale 16 B, fill 8 B, norm2 8 B
+    per element), timed with HIP events on the library stream.  The slices
+    are cut from a 120^4 block (1.66 GB): a half-extent slice (inner run of
+    60 elements) and a 15^4 one (inner run of 15)."""
+    out = {}
+    te = [120, 120, 120, 120]
+    nt = int(np.prod(te))
+    tens = sg.DeviceArray(nt)
+    sg.fill_random(tens.ptr, nt, 21, 0)
+    big = sg.DeviceArray(nt)
+    sg.fill_random(big.ptr, nt, 22, 0)
+    res = sg.DeviceArray(1)
+
+    def timed(name, fn, nbytes, **info):
+        fn()
+        sg.synchronize()
+        e0, e1 = sg.Event(), sg.Event()
+        e0.record()
+        for _ in range(reps):
+            fn()
+        e1.record()
+        e1.synchronize()
+        ms = e0.elapsed_ms(e1) / reps
+        gbs = nbytes / (ms * 1e-3) / 1e9
+        out[name] = dict(info, ms=ms, GBps=gbs, frac=gbs / HBM_PEAK_GBS)
+
+    for se, eb in (([60, 120, 60, 120], [30, 0, 60, 0]), ([15, 15, 15, 15], [100, 3, 50, 7])):
+        ns = int(np.prod(se))
+        slc = sg.DeviceArray(ns)
+        timed("slice %s of 120^4" % "x".join(map(str, se)),
+              lambda: sg.block_slice(tens.ptr, te, slc.ptr, se, eb), 16.0 * ns,
+              slice_ext=se, ext_beg=eb)
+        timed("insert %s into 120^4" % "x".join(map(str, se)),
+              lambda: sg.block_insert(tens.ptr, te, slc.ptr, se, eb), 16.0 * ns,
+              slice_ext=se, ext_beg=eb)
+        slc.free()
+    n = nt
+    timed("add (scaled) 120^4", lambda: sg.block_add(big.ptr, tens.ptr, [n], 0.5), 24.0 * n)
+    timed("scale 120^4", lambda: sg.block_scale(big.ptr, [n], 1.0000001), 16.0 * n)
+    timed("init 120^4", lambda: sg.block_init(big.ptr, [n], 0.25), 8.0 * n)
+    timed("norm2 120^4", lambda: sg.block_norm2(tens.ptr, [n], res.ptr), 8.0 * n)
+    for d in (tens, big, res):
+        d.free()
     return out
 
 
@@ -498,6 +547,7 @@ def main():
     graphs = None if a.no_graphs else graph_bench()
     lat = latency_sample(w, a.latency_samples) if a.latency_samples > 0 else None
     perm = permute_bench()
+    blockops = blockops_bench()
     cores, model = cpu_info()
     cpu = None if a.no_cpu else cpu_baseline(w, a.cpu_seconds, got, steps_run)
     if cpu:
@@ -533,6 +583,7 @@ def main():
                      "measured_ceiling": measured,
                      "frac_of_measured": achieved / mfma_meas if mfma_meas else None},
         "permute": perm,
+        "block_ops": blockops,
         "split": split,
         "sparse_ladder": sparse,
         "latency": lat,

@@ -72,25 +72,25 @@ void tensor_block_contract__(int *nthreads, int *ptrn, double *l, int *lrank,
  * negative SIPGPU_E* ierr.  Element counts follow the reference: rank 0 is
  * one element, and a product of extents that is not positive is no element. */
 
-/* Replaces tensor_block_init__ (tensor_dil_omp.F90:144-187; prototype
+/* Replaces tensor_block_init__ (tensor_dil_omp.F90:144-191; prototype
  * tensor_ops_c_prototypes.h:70): every element = *sval.  ierr -1 for a
  * negative rank. */
 void tensor_block_init__(int *nthreads, double *tens, int *tens_rank, int *dims, double *sval,
                          int *ierr);
 
-/* Replaces tensor_block_scale__ (tensor_dil_omp.F90:189-219; prototype
+/* Replaces tensor_block_scale__ (tensor_dil_omp.F90:193-228; prototype
  * tensor_ops_c_prototypes.h:84): every element *= *scale_fac.  ierr -1 for
  * a negative rank. */
 void tensor_block_scale__(int *nthreads, double *tens, int *tens_rank, int *dims,
                           double *scale_fac, int *ierr);
 
-/* Replaces tensor_block_norm2__ (tensor_dil_omp.F90:221-256; prototype
+/* Replaces tensor_block_norm2__ (tensor_dil_omp.F90:230-269; prototype
  * tensor_ops_c_prototypes.h:86): the sum of squares.  The reference's
  * OpenMP reduction has no fixed order; this one is fixed for a given size.
  * ierr -1 (result 0) for a negative rank. */
 double tensor_block_norm2__(int *nthreads, double *tens, int *tens_rank, int *dims, int *ierr);
 
-/* Replaces tensor_block_slice__ (tensor_dil_omp.F90:258-311; prototype
+/* Replaces tensor_block_slice__ (tensor_dil_omp.F90:271-330; prototype
  * tensor_ops_c_prototypes.h:100), called by Block::extract_slice
  * (dynamic_data/block.cpp:294): slice(i) = tens(ext_beg + i) for every
  * column-major multi-index i of the slice.  ierr: 1 negative dim_num, -1
@@ -101,7 +101,7 @@ double tensor_block_norm2__(int *nthreads, double *tens, int *tens_rank, int *di
 void tensor_block_slice__(int *nthreads, int *dim_num, double *tens, int *tens_ext, double *slice,
                           int *slice_ext, int *ext_beg, int *ierr);
 
-/* Replaces tensor_block_insert__ (tensor_dil_omp.F90:313-368; prototype
+/* Replaces tensor_block_insert__ (tensor_dil_omp.F90:332-392; prototype
  * tensor_ops_c_prototypes.h:117), called by Block::insert_slice
  * (dynamic_data/block.cpp:320): tens(ext_beg + i) = slice(i).  Errors as
  * tensor_block_slice__. */

@@ -335,7 +335,7 @@ static long long orc_elems(int rank, const int *dims) {
     return n > 0 ? n : 0;
 }
 
-/* tensor_dil_omp.F90:144-187 */
+/* tensor_dil_omp.F90:144-191 */
 void orc_block_init(int rank, const int *dims, double *t, double sval, int *ierr) {
     *ierr = 0;
     if (rank < 0) { *ierr = -1; return; }
@@ -344,7 +344,7 @@ void orc_block_init(int rank, const int *dims, double *t, double sval, int *ierr
     for (long long i = 0; i < n; ++i) t[i] = sval;
 }
 
-/* tensor_dil_omp.F90:189-219 */
+/* tensor_dil_omp.F90:193-228 */
 void orc_block_scale(int rank, const int *dims, double *t, double s, int *ierr) {
     *ierr = 0;
     if (rank < 0) { *ierr = -1; return; }
@@ -353,7 +353,7 @@ void orc_block_scale(int rank, const int *dims, double *t, double s, int *ierr) 
     for (long long i = 0; i < n; ++i) t[i] = t[i] * s;
 }
 
-/* tensor_dil_omp.F90:221-256 */
+/* tensor_dil_omp.F90:230-269 */
 double orc_block_norm2(int rank, const int *dims, const double *t, int *ierr) {
     *ierr = 0;
     if (rank < 0) { *ierr = -1; return 0.0; }
@@ -365,7 +365,7 @@ double orc_block_norm2(int rank, const int *dims, const double *t, int *ierr) {
 }
 
 /* Column-major offset in `tens` of element `l` of the slice: the slice's
- * multi-index shifted by ext_beg (tensor_dil_omp.F90:283-284, :338-339). */
+ * multi-index shifted by ext_beg (tensor_dil_omp.F90:308, :370). */
 static long long orc_slice_offset(int rank, const int *tens_ext, const int *slice_ext,
                                   const int *ext_beg, long long l) {
     long long off = 0, stride = 1;
@@ -378,8 +378,8 @@ static long long orc_slice_offset(int rank, const int *tens_ext, const int *slic
     return off;
 }
 
-/* Shared rank / size rules of slice and insert (tensor_dil_omp.F90:274-276,
- * :301-307 and divide_segment_ :1248-1254); returns the slice's element
+/* Shared rank / size rules of slice and insert (tensor_dil_omp.F90:293,
+ * :322-326 and divide_segment_ :1248-1254); returns the slice's element
  * count, or 0 when there is nothing to copy. */
 static long long orc_slice_count(int rank, const int *slice_ext, int *ierr) {
     *ierr = 0;
@@ -391,7 +391,7 @@ static long long orc_slice_count(int rank, const int *slice_ext, int *ierr) {
     return n;
 }
 
-/* tensor_dil_omp.F90:258-311 */
+/* tensor_dil_omp.F90:271-330 */
 void orc_block_slice(int rank, const double *tens, const int *tens_ext, double *slice,
                      const int *slice_ext, const int *ext_beg, int *ierr) {
     long long n = orc_slice_count(rank, slice_ext, ierr);
@@ -400,7 +400,7 @@ void orc_block_slice(int rank, const double *tens, const int *tens_ext, double *
         slice[l] = tens[orc_slice_offset(rank, tens_ext, slice_ext, ext_beg, l)];
 }
 
-/* tensor_dil_omp.F90:313-368 */
+/* tensor_dil_omp.F90:332-392 */
 void orc_block_insert(int rank, double *tens, const int *tens_ext, const double *slice,
                       const int *slice_ext, const int *ext_beg, int *ierr) {
     long long n = orc_slice_count(rank, slice_ext, ierr);

@@ -76,21 +76,21 @@ void orc_tensor_block_contract(const int *ptrn, const double *l, int lrank,
 void orc_tensor_block_add(int rank, const int *dims, double *t0,
                           const double *t1, double scale, int *ierr);
 
-/* tensor_block_init_ (tensor_dil_omp.F90:144-187): rank 0 sets t[0]; a
+/* tensor_block_init_ (tensor_dil_omp.F90:144-191): rank 0 sets t[0]; a
  * positive rank sets all prod(dims) elements (none when the product is not
  * positive); a negative rank sets ierr = -1 and writes nothing. */
 void orc_block_init(int rank, const int *dims, double *t, double sval, int *ierr);
 
-/* tensor_block_scale_ (tensor_dil_omp.F90:189-219): t *= s elementwise, same
+/* tensor_block_scale_ (tensor_dil_omp.F90:193-228): t *= s elementwise, same
  * rank rules as orc_block_init. */
 void orc_block_scale(int rank, const int *dims, double *t, double s, int *ierr);
 
-/* tensor_block_norm2_ (tensor_dil_omp.F90:221-256): sum of t[i]^2 in
+/* tensor_block_norm2_ (tensor_dil_omp.F90:230-269): sum of t[i]^2 in
  * ascending i (the reference reduces with OpenMP; one thread gives this
  * order); rank 0 gives t[0]^2, a negative rank 0 with ierr = -1. */
 double orc_block_norm2(int rank, const int *dims, const double *t, int *ierr);
 
-/* tensor_block_slice_ (tensor_dil_omp.F90:258-311): slice(i) =
+/* tensor_block_slice_ (tensor_dil_omp.F90:271-330): slice(i) =
  * tens(ext_beg + i) for every multi-index i of the slice, both column-major.
  * rank 0 copies the scalar; a negative rank gives ierr = 1; an empty slice
  * (a non-positive element count) gives ierr = -1 and writes nothing (the
@@ -99,7 +99,7 @@ double orc_block_norm2(int rank, const int *dims, const double *t, int *ierr);
 void orc_block_slice(int rank, const double *tens, const int *tens_ext, double *slice,
                      const int *slice_ext, const int *ext_beg, int *ierr);
 
-/* tensor_block_insert_ (tensor_dil_omp.F90:313-368): tens(ext_beg + i) =
+/* tensor_block_insert_ (tensor_dil_omp.F90:332-392): tens(ext_beg + i) =
  * slice(i); error rules as orc_block_slice. */
 void orc_block_insert(int rank, double *tens, const int *tens_ext, const double *slice,
                       const int *slice_ext, const int *ext_beg, int *ierr);

@@ -222,7 +222,7 @@ def test_native_elementwise_vs_oracle(n, misalign):
 
 def test_native_scalar_add_always_scales():
     """Rank 0 multiplies by the factor even inside the 1e-13 window
-    (tensor_dil_omp.F90:415-416)."""
+    (tensor_dil_omp.F90:417-418)."""
     a = sg.DeviceArray.from_host(np.array([1.0]))
     b = sg.DeviceArray.from_host(np.array([3.0]))
     s = 1.0 + 5e-14
