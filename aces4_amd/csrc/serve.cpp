@@ -125,6 +125,8 @@ struct sipgpu_cache_s {
     std::list<unsigned long long> lru;          // front: least recently released
     std::vector<hipEvent_t> fences;             // spills whose copies may be in flight
     sipgpu_cache_stats st{};
+    std::vector<std::vector<unsigned long long>> scopes; // temp blocks of each open scope
+    std::unordered_set<unsigned long long> persistent;   // kept by the end of a program
 };
 
 namespace {
@@ -171,6 +173,38 @@ int evict_locked(sipgpu_cache_s &c, unsigned long long key, CBlock &b) {
     b.off = kNotResident;
     c.st.evictions += 1;
     (void)key;
+    return 0;
+}
+
+// Forget a block without writing it back.  Its arena range is fenced like an
+// eviction's; an owned spill buffer is freed once the copy stream (which may
+// still be filling it) has drained.
+int remove_locked(sipgpu_cache_s &c, unsigned long long key) {
+    auto it = c.blocks.find(key);
+    if (it == c.blocks.end()) return 0;
+    CBlock &b = it->second;
+    if (b.in_lru) {
+        c.lru.erase(b.lru_it);
+        b.in_lru = false;
+    }
+    if (b.off != kNotResident) {
+        if (b.used_on_device) {
+            hipEvent_t f;
+            if (int rc = make_event(&f)) return rc;
+            SIPGPU_HIP_TRY(hipStreamWaitEvent(c.copy, b.last_use, 0));
+            SIPGPU_HIP_TRY(hipEventRecord(f, c.copy));
+            c.fences.push_back(f);
+        }
+        c.arena.release(b.off, b.elems * sizeof(double));
+        b.off = kNotResident;
+    }
+    if (b.home_owned) {
+        SIPGPU_HIP_TRY(hipStreamSynchronize(c.copy));
+        SIPGPU_HIP_TRY(hipHostFree(b.home));
+    }
+    if (b.last_use) (void)hipEventDestroy(b.last_use);
+    c.blocks.erase(it);
+    c.persistent.erase(key);
     return 0;
 }
 
@@ -383,6 +417,80 @@ int sipgpu_cache_evict(sipgpu_cache_t c, unsigned long long key) {
     if (it->second.pins) return fail(SIPGPU_EARG, "cache: block in use");
     if (it->second.off == kNotResident) return 0;
     return evict_locked(*c, key, it->second);
+}
+
+int sipgpu_cache_remove(sipgpu_cache_t c, unsigned long long key) {
+    if (!c) return fail(SIPGPU_EARG, "null cache");
+    if (int rc = sipgpu::rt::refuse_in_capture("removing a cached block")) return rc;
+    std::lock_guard<std::mutex> g(c->mu);
+    auto it = c->blocks.find(key);
+    if (it == c->blocks.end()) return fail(SIPGPU_ENOTFOUND, "cache: unknown block key");
+    if (it->second.pins) return fail(SIPGPU_EARG, "cache: block in use");
+    return remove_locked(*c, key);
+}
+
+int sipgpu_cache_scope_enter(sipgpu_cache_t c) {
+    if (!c) return fail(SIPGPU_EARG, "null cache");
+    std::lock_guard<std::mutex> g(c->mu);
+    c->scopes.emplace_back();
+    return 0;
+}
+
+int sipgpu_cache_scope_add(sipgpu_cache_t c, unsigned long long key) {
+    if (!c) return fail(SIPGPU_EARG, "null cache");
+    std::lock_guard<std::mutex> g(c->mu);
+    if (c->scopes.empty()) return fail(SIPGPU_EARG, "cache: no open scope");
+    if (!c->blocks.count(key)) return fail(SIPGPU_ENOTFOUND, "cache: unknown block key");
+    c->scopes.back().push_back(key);
+    return 0;
+}
+
+int sipgpu_cache_scope_leave(sipgpu_cache_t c) {
+    if (!c) return fail(SIPGPU_EARG, "null cache");
+    if (int rc = sipgpu::rt::refuse_in_capture("leaving a cache scope")) return rc;
+    std::lock_guard<std::mutex> g(c->mu);
+    if (c->scopes.empty()) return fail(SIPGPU_EARG, "cache: no open scope");
+    // nothing is removed unless everything can be
+    for (unsigned long long key : c->scopes.back()) {
+        auto it = c->blocks.find(key);
+        if (it != c->blocks.end() && it->second.pins)
+            return fail(SIPGPU_EARG, "cache: a temp block of the scope is in use");
+    }
+    for (unsigned long long key : c->scopes.back())
+        if (int rc = remove_locked(*c, key)) return rc;
+    c->scopes.pop_back();
+    return 0;
+}
+
+int sipgpu_cache_set_persistent(sipgpu_cache_t c, unsigned long long key, int persistent) {
+    if (!c) return fail(SIPGPU_EARG, "null cache");
+    std::lock_guard<std::mutex> g(c->mu);
+    if (!c->blocks.count(key)) return fail(SIPGPU_ENOTFOUND, "cache: unknown block key");
+    if (persistent)
+        c->persistent.insert(key);
+    else
+        c->persistent.erase(key);
+    return 0;
+}
+
+int sipgpu_cache_end_program(sipgpu_cache_t c, unsigned long long *kept) {
+    if (!c) return fail(SIPGPU_EARG, "null cache");
+    if (int rc = sipgpu::rt::refuse_in_capture("ending a program")) return rc;
+    std::lock_guard<std::mutex> g(c->mu);
+    if (!c->scopes.empty()) return fail(SIPGPU_EARG, "cache: a scope is still open");
+    std::vector<unsigned long long> drop;
+    for (auto &kv : c->blocks) {
+        if (c->persistent.count(kv.first)) continue;
+        if (kv.second.pins) return fail(SIPGPU_EARG, "cache: a block is still in use");
+        drop.push_back(kv.first);
+    }
+    for (unsigned long long key : drop)
+        if (int rc = remove_locked(*c, key)) return rc;
+    // a restored block belongs to the next program: it must be marked again
+    // to outlive that one too
+    if (kept) *kept = c->persistent.size();
+    c->persistent.clear();
+    return 0;
 }
 
 int sipgpu_cache_get_stats(sipgpu_cache_t c, sipgpu_cache_stats *out) {

@@ -48,7 +48,9 @@ EXPORTED = [
     "sipgpu_batch_plan_split_tol", "tensor_block_init__", "tensor_block_scale__",
     "tensor_block_norm2__", "tensor_block_slice__", "tensor_block_insert__", "tensor_block_add__",
     "sipgpu_block_init", "sipgpu_block_scale", "sipgpu_block_add", "sipgpu_block_norm2",
-    "sipgpu_block_slice", "sipgpu_block_insert", "sipgpu_plan_slice",
+    "sipgpu_block_slice", "sipgpu_block_insert", "sipgpu_plan_slice", "sipgpu_cache_remove",
+    "sipgpu_cache_scope_enter", "sipgpu_cache_scope_add", "sipgpu_cache_scope_leave",
+    "sipgpu_cache_set_persistent", "sipgpu_cache_end_program",
     "sipgpu_cache_create", "sipgpu_cache_destroy", "sipgpu_cache_insert", "sipgpu_cache_acquire",
     "sipgpu_cache_release", "sipgpu_cache_read", "sipgpu_cache_flush", "sipgpu_cache_evict",
     "sipgpu_cache_get_stats", "sipgpu_server_create", "sipgpu_server_submit",
@@ -184,6 +186,12 @@ def lib():
         L.sipgpu_cache_flush.argtypes = [_vp, _u64, ctypes.c_int]
         L.sipgpu_cache_evict.argtypes = [_vp, _u64]
         L.sipgpu_cache_get_stats.argtypes = [_vp, ctypes.POINTER(CacheStats)]
+        L.sipgpu_cache_remove.argtypes = [_vp, _u64]
+        L.sipgpu_cache_scope_enter.argtypes = [_vp]
+        L.sipgpu_cache_scope_add.argtypes = [_vp, _u64]
+        L.sipgpu_cache_scope_leave.argtypes = [_vp]
+        L.sipgpu_cache_set_persistent.argtypes = [_vp, _u64, ctypes.c_int]
+        L.sipgpu_cache_end_program.argtypes = [_vp, ctypes.POINTER(ctypes.c_ulonglong)]
         L.sipgpu_server_create.argtypes = [_vp, ctypes.c_int, ctypes.POINTER(_vp)]
         L.sipgpu_server_submit.argtypes = [_vp, ctypes.POINTER(Request), ctypes.POINTER(_u64)]
         L.sipgpu_server_wait.argtypes = [_vp, _u64, ctypes.POINTER(ctypes.c_double)]
@@ -814,6 +822,29 @@ class Cache:
 
     def evict(self, key):
         _check(lib().sipgpu_cache_evict(self.handle, key))
+
+    def remove(self, key):
+        """Forget a block without writing it back."""
+        _check(lib().sipgpu_cache_remove(self.handle, key))
+        self._homes.pop(key, None)
+
+    def scope_enter(self):
+        _check(lib().sipgpu_cache_scope_enter(self.handle))
+
+    def scope_add(self, key):
+        _check(lib().sipgpu_cache_scope_add(self.handle, key))
+
+    def scope_leave(self):
+        _check(lib().sipgpu_cache_scope_leave(self.handle))
+
+    def set_persistent(self, key, persistent=True):
+        _check(lib().sipgpu_cache_set_persistent(self.handle, key, 1 if persistent else 0))
+
+    def end_program(self):
+        """Remove every block not marked persistent; returns how many were kept."""
+        kept = ctypes.c_ulonglong(0)
+        _check(lib().sipgpu_cache_end_program(self.handle, ctypes.byref(kept)))
+        return kept.value
 
     def stats(self):
         st = CacheStats()

@@ -378,6 +378,34 @@ int sipgpu_cache_flush(sipgpu_cache_t cache, unsigned long long key, int all);
 int sipgpu_cache_evict(sipgpu_cache_t cache, unsigned long long key);
 int sipgpu_cache_get_stats(sipgpu_cache_t cache, sipgpu_cache_stats *stats);
 
+/* Block lifetimes of a SIAL program.
+ *
+ * sipgpu_cache_remove forgets a block without writing it back: the
+ * BlockManager's delete of a block whose value is no longer needed
+ * (block_manager.cpp:91-101).  SIPGPU_EARG while the block is acquired.
+ *
+ * Scopes replace BlockManager::enter_scope / leave_scope
+ * (block_manager.cpp:182-206): sipgpu_cache_scope_add makes a block a temp
+ * of the innermost open scope (what get_block_for_writing does for a block
+ * created inside a scope, :105-115), and sipgpu_cache_scope_leave removes
+ * all of that scope's temps (none if any of them is still acquired) and
+ * closes it.  Scopes nest.
+ *
+ * Persistence replaces set_persistent / save_marked_arrays /
+ * restore_persistent (worker_persistent_array_manager.h:20-57):
+ * sipgpu_cache_end_program removes every block not marked persistent and
+ * keeps the marked ones with their values, resident or spilled, for the next
+ * program; the marks are cleared, so a block outlives the next program only
+ * if that program marks it again (an object "can only be restored once").
+ * `kept` (may be NULL) receives the number of blocks kept.  It fails while a
+ * scope is open or an unmarked block is acquired. */
+int sipgpu_cache_remove(sipgpu_cache_t cache, unsigned long long key);
+int sipgpu_cache_scope_enter(sipgpu_cache_t cache);
+int sipgpu_cache_scope_add(sipgpu_cache_t cache, unsigned long long key);
+int sipgpu_cache_scope_leave(sipgpu_cache_t cache);
+int sipgpu_cache_set_persistent(sipgpu_cache_t cache, unsigned long long key, int persistent);
+int sipgpu_cache_end_program(sipgpu_cache_t cache, unsigned long long *kept);
+
 /* ------------------------------------------------------------------------
  * Contraction server: continuous batching of single block contractions
  * whose blocks live in a cache.  A scheduler thread takes every request that

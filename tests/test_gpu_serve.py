@@ -90,6 +90,101 @@ def test_cache_pinned_blocks_are_not_evicted():
     c.destroy()
 
 
+def test_cache_scopes_remove_temps_without_write_back():
+    """leave_scope semantics: the scope's temps are forgotten (no spill, no
+    write to their homes), their HBM is reusable without evictions, other
+    blocks and outer scopes are untouched."""
+    n = MB // 8
+    c = sg.Cache(int(4.5 * MB))         # all four blocks fit: nothing is evicted
+    home = wl.uniform(7, n)
+    keep = home.copy()
+    h3 = home.copy()
+    c.insert(1, n, host=home)           # an ordinary block
+    c.scope_enter()
+    c.insert(2, n, init="zero")
+    c.insert(3, n, host=h3)
+    c.scope_add(2)
+    c.scope_add(3)
+    c.scope_enter()                     # nested scope
+    c.insert(4, n, init="zero")
+    c.scope_add(4)
+    for k in (1, 2, 3, 4):
+        p = c.acquire(k, "w")
+        h2d(p, np.full(n, float(k)))
+        c.release(k)
+    c.scope_leave()                     # removes 4 only
+    with pytest.raises(sg.SipGpuError) as ei:
+        c.read(4, n)
+    assert ei.value.code == -110        # SIPGPU_ENOTFOUND
+    p = c.acquire(2, "r")               # a pinned temp blocks leave_scope...
+    with pytest.raises(sg.SipGpuError) as ei:
+        c.scope_leave()
+    assert ei.value.code == -107
+    assert np.array_equal(d2h(p, n), np.full(n, 2.0))   # ...and nothing was removed
+    c.release(2)
+    st0 = c.stats()
+    c.scope_leave()
+    st = c.stats()
+    assert st["blocks"] == 1 and st["spills"] == st0["spills"] and st["bytes_d2h"] == st0["bytes_d2h"]
+    assert st["resident_bytes"] == MB
+    assert np.array_equal(h3, keep)     # the dirty temp was not written back
+    # the freed space takes three new blocks without evicting block 1
+    for k in (5, 6, 7):
+        c.insert(k, n, init="zero")
+        c.acquire(k, "r")
+        c.release(k)
+    assert c.stats()["evictions"] == st["evictions"] == 0
+    assert np.array_equal(c.read(1, n), np.full(n, 1.0))
+    assert np.array_equal(home, keep)   # block 1 stayed resident: its home is untouched
+    with pytest.raises(sg.SipGpuError):
+        c.scope_leave()                 # no scope open
+    c.destroy()
+
+
+def test_cache_remove_and_persistent_blocks():
+    """delete_block and set_persistent / end of program semantics."""
+    n = MB // 8
+    c = sg.Cache(int(2.5 * MB))
+    c.insert(1, n, init="zero")
+    c.insert(2, n, init="zero")
+    c.insert(3, n, init="uniform", seed=3, stream=0)
+    p = c.acquire(1, "w")
+    h2d(p, np.full(n, 1.5))
+    with pytest.raises(sg.SipGpuError) as ei:
+        c.remove(1)                     # pinned
+    assert ei.value.code == -107
+    c.release(1)
+    p = c.acquire(2, "w")
+    h2d(p, np.full(n, 2.5))
+    c.release(2)
+    c.acquire(3, "r")                   # evicts and spills block 1
+    c.release(3)
+    assert c.stats()["spills"] == 1
+    c.remove(3)
+    with pytest.raises(sg.SipGpuError) as ei:
+        c.remove(3)
+    assert ei.value.code == -110
+    c.set_persistent(1)                 # spilled, persistent
+    c.set_persistent(2)                 # resident, persistent
+    c.insert(4, n, init="zero")         # dies with the program
+    assert c.end_program() == 2
+    assert c.stats()["blocks"] == 2
+    assert np.array_equal(c.read(1, n), np.full(n, 1.5))
+    assert np.array_equal(c.read(2, n), np.full(n, 2.5))
+    # the next program keeps only what it marks again
+    c.set_persistent(2)
+    assert c.end_program() == 1
+    assert c.stats()["blocks"] == 1
+    assert np.array_equal(c.read(2, n), np.full(n, 2.5))
+    c.scope_enter()
+    with pytest.raises(sg.SipGpuError):
+        c.end_program()                 # a scope is open
+    c.scope_leave()
+    assert c.end_program() == 0 and c.stats()["blocks"] == 0
+    assert c.stats()["resident_bytes"] == 0
+    c.destroy()
+
+
 def ring_keys(w):
     keys = {}
     for tag, arr in ((1, w["L"]), (2, w["R"]), (3, w["D"])):
