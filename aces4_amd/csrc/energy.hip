@@ -83,6 +83,12 @@ __global__ __launch_bounds__(kEnergyThreads) void denominator_divide_kernel(Ener
     }
 }
 
+// Each thread keeps the loads of its next kEnergyAhead elements in flight (a
+// queue rotated every iteration), so the latency of one element's load hides
+// behind the arithmetic of the previous ones.  With y == x each element is
+// loaded once.
+constexpr int kEnergyAhead = 4;
+
 __global__ __launch_bounds__(kEnergyThreads) void energy_partial_kernel(EnergyArgs a,
                                                                         const double *x,
                                                                         const double *y,
@@ -90,12 +96,29 @@ __global__ __launch_bounds__(kEnergyThreads) void energy_partial_kernel(EnergyAr
     __shared__ double red[kEnergyThreads];
     const long long e0 = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     const long long S = (long long)gridDim.x * blockDim.x;
+    const bool same = x == y;
     int idx[kEnergyMaxRank], step[kEnergyMaxRank];
     decode(a, e0, idx);
     decode(a, S, step);
+    double qx[kEnergyAhead], qy[kEnergyAhead];
+#pragma unroll
+    for (int u = 0; u < kEnergyAhead; ++u) {
+        const long long eu = e0 + u * S;
+        qx[u] = eu < a.n ? x[eu] : 0.0;
+        qy[u] = (eu < a.n && !same) ? y[eu] : 0.0;
+    }
     double v = 0.0;
     for (long long e = e0; e < a.n; e += S) {
-        v += (x[e] / denominator(a, idx)) * y[e];
+        const double xv = qx[0], yv = same ? qx[0] : qy[0];
+#pragma unroll
+        for (int u = 0; u + 1 < kEnergyAhead; ++u) {
+            qx[u] = qx[u + 1];
+            qy[u] = qy[u + 1];
+        }
+        const long long ea = e + kEnergyAhead * S;
+        qx[kEnergyAhead - 1] = ea < a.n ? x[ea] : 0.0;
+        qy[kEnergyAhead - 1] = (ea < a.n && !same) ? y[ea] : 0.0;
+        v += (xv / denominator(a, idx)) * yv;
         advance(a, idx, step);
     }
     red[threadIdx.x] = v;
