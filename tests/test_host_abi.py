@@ -249,3 +249,22 @@ def test_block_operations_fail_loudly_without_gpu():
     assert sg.tensor_block_slice(np.arange(6.0), [2, 3], [1, 2], [1, 0], rank=-1)[1] == 1
     assert sg.tensor_block_slice(np.arange(6.0), [2, 3], [0, 2], [1, 0])[1] == -1
     assert sg.tensor_block_slice(np.arange(6.0), [2, 3], [2, 2], [1, 0])[1] == -107
+
+
+# Every bind(c) name of the reference's tensor module (tensor_dil_omp.F90,
+# prototypes tensor_ops_c_prototypes.h:41-178): the library linked in its
+# place must export all of them.
+REFERENCE_MODULE_SYMBOLS = [
+    "tensor_size_by_shape_", "get_contraction_ptrn_", "tensor_block_init__",
+    "tensor_block_scale__", "tensor_block_norm2__", "tensor_block_slice__",
+    "tensor_block_insert__", "tensor_block_add__", "tensor_block_copy__",
+    "tensor_block_contract__",
+]
+
+
+def test_library_exports_the_whole_reference_module():
+    L = sg.lib()
+    missing = [n for n in REFERENCE_MODULE_SYMBOLS if not hasattr(L, n)]
+    assert not missing, missing
+    names = declared_functions()
+    assert all(n in names for n in REFERENCE_MODULE_SYMBOLS)
