@@ -91,6 +91,46 @@ hipError_t launch_op(double *a, const double *b, long long n, double s, hipStrea
     return hipGetLastError();
 }
 
+// t = ((t + p[0]) + p[1]) + ... elementwise, the sources added in order.
+template <int VEC>
+__global__ __launch_bounds__(kOpThreads) void add_many_kernel(double *t, AddManyArgs a,
+                                                              long long n) {
+    using V = typename std::conditional<VEC == 2, double2, double>::type;
+    V *tv = reinterpret_cast<V *>(t);
+    const long long units = n / VEC;
+    const long long stride = (long long)gridDim.x * kOpThreads;
+    const long long i0 = (long long)blockIdx.x * kOpThreads + threadIdx.x;
+    constexpr int U = 2; // units per thread per round: 2 x (1 + nparts) loads in flight
+    for (long long i = i0; i < units; i += U * stride) {
+        V acc[U], x[U][kAddManyMax];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const long long iu = i + u * stride;
+            if (iu < units) {
+                acc[u] = tv[iu];
+#pragma unroll
+                for (int q = 0; q < kAddManyMax; ++q)
+                    if (q < a.n) x[u][q] = reinterpret_cast<const V *>(a.p[q])[iu];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const long long iu = i + u * stride;
+            if (iu < units) {
+#pragma unroll
+                for (int q = 0; q < kAddManyMax; ++q)
+                    if (q < a.n) acc[u] += x[u][q];
+                tv[iu] = acc[u];
+            }
+        }
+    }
+    if (VEC == 2 && (n & 1) && i0 == 0) {
+        double v = t[n - 1];
+        for (int q = 0; q < a.n; ++q) v += a.p[q][n - 1];
+        t[n - 1] = v;
+    }
+}
+
 // Sum of the kOpThreads values of `v` over the workgroup (fixed tree).
 __device__ __forceinline__ double block_sum(double v, double *red) {
     red[threadIdx.x] = v;
@@ -205,6 +245,18 @@ hipError_t launch_block_op(int op, double *a, const double *b, long long n, doub
     case kOpAdd: return launch_op<kOpAdd>(a, b, n, s, st);
     default: return hipErrorInvalidValue;
     }
+}
+
+hipError_t launch_block_add_many(double *t, const AddManyArgs &a, long long n, hipStream_t st) {
+    if (n <= 0 || a.n <= 0) return hipSuccess;
+    uintptr_t bits = reinterpret_cast<uintptr_t>(t);
+    for (int q = 0; q < a.n; ++q) bits |= reinterpret_cast<uintptr_t>(a.p[q]);
+    if ((bits & 15) == 0)
+        hipLaunchKernelGGL(add_many_kernel<2>, dim3(op_blocks(n / 2)), dim3(kOpThreads), 0, st, t, a,
+                           n);
+    else
+        hipLaunchKernelGGL(add_many_kernel<1>, dim3(op_blocks(n)), dim3(kOpThreads), 0, st, t, a, n);
+    return hipGetLastError();
 }
 
 int norm2_blocks(long long n) {

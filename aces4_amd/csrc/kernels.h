@@ -126,6 +126,13 @@ hipError_t launch_fill_random(double *out, long long n, unsigned long long seed,
 enum BlockOp { kOpFill = 0, kOpScale = 1, kOpAddScaled = 2, kOpAdd = 3 };
 hipError_t launch_block_op(int op, double *a, const double *b, long long n, double s,
                            hipStream_t st);
+// t = ((t + p[0]) + p[1]) + ... over n elements, at most kAddManyMax sources.
+constexpr int kAddManyMax = 8;
+struct AddManyArgs {
+    const double *p[kAddManyMax];
+    int n;
+};
+hipError_t launch_block_add_many(double *t, const AddManyArgs &a, long long n, hipStream_t st);
 // out[0] = sum x[i]^2 over n >= 0 elements: per-workgroup partials (at most
 // kNormBlocks, in `part`) then one workgroup; the order depends on n only.
 constexpr int kNormBlocks = 1024;

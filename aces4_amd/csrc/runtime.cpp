@@ -2394,6 +2394,27 @@ int sipgpu_block_add(sipgpu_stream_t s, double *t0, const double *t1, int rank, 
     return 0;
 }
 
+int sipgpu_block_add_many(sipgpu_stream_t s, double *t, const double *const *parts, int nparts,
+                          int rank, const int *dims) {
+    long long n = 0;
+    if (int rc = block_elems(rank, dims, n)) return fail(rc, "block_add_many: bad rank or extents");
+    if (nparts < 0 || nparts > kAddManyMax)
+        return fail(SIPGPU_EARG, "block_add_many: 0..8 sources");
+    if (n == 0 || nparts == 0) return 0;
+    if (!t || !parts) return fail(SIPGPU_EARG, "block_add_many: null pointer");
+    AddManyArgs a{};
+    a.n = nparts;
+    for (int q = 0; q < nparts; ++q) {
+        if (!parts[q]) return fail(SIPGPU_EARG, "block_add_many: null source");
+        a.p[q] = parts[q];
+    }
+    Context &c = ctx();
+    std::lock_guard<std::mutex> g(c.mu);
+    if (int rc = ensure_init(c)) return rc;
+    HIP_TRY(launch_block_add_many(t, a, n, resolve(c, s).s));
+    return 0;
+}
+
 int sipgpu_block_norm2(sipgpu_stream_t s, const double *t, int rank, const int *dims,
                        double *result) {
     long long n = 0;

@@ -50,7 +50,7 @@ EXPORTED = [
     "sipgpu_block_init", "sipgpu_block_scale", "sipgpu_block_add", "sipgpu_block_norm2",
     "sipgpu_block_slice", "sipgpu_block_insert", "sipgpu_plan_slice", "sipgpu_cache_remove",
     "sipgpu_cache_scope_enter", "sipgpu_cache_scope_add", "sipgpu_cache_scope_leave",
-    "sipgpu_cache_set_persistent", "sipgpu_cache_end_program",
+    "sipgpu_cache_set_persistent", "sipgpu_cache_end_program", "sipgpu_block_add_many",
     "sipgpu_cache_create", "sipgpu_cache_destroy", "sipgpu_cache_insert", "sipgpu_cache_acquire",
     "sipgpu_cache_release", "sipgpu_cache_read", "sipgpu_cache_flush", "sipgpu_cache_evict",
     "sipgpu_cache_get_stats", "sipgpu_server_create", "sipgpu_server_submit",
@@ -215,6 +215,8 @@ def lib():
         L.sipgpu_block_scale.argtypes = [_vp, _vp, ctypes.c_int, _ip, ctypes.c_double]
         L.sipgpu_block_add.argtypes = [_vp, _vp, _vp, ctypes.c_int, _ip, ctypes.c_double]
         L.sipgpu_block_norm2.argtypes = [_vp, _vp, ctypes.c_int, _ip, _vp]
+        L.sipgpu_block_add_many.argtypes = [_vp, _vp, ctypes.POINTER(_vp), ctypes.c_int,
+                                            ctypes.c_int, _ip]
         for f in ("sipgpu_block_slice", "sipgpu_block_insert"):
             getattr(L, f).argtypes = [_vp, ctypes.c_int, _vp, _ip, _vp, _ip, _ip]
         L.sipgpu_plan_slice.argtypes = [ctypes.c_int, _ip, _ip, _ip, _ip, _ip,
@@ -622,6 +624,13 @@ def block_add(t0, t1, dims, factor, stream=None, rank=None):
     _check(lib().sipgpu_block_add(stream.handle if stream else None, t0, t1,
                                   len(dims) if rank is None else rank, _iptr(_ints(dims)),
                                   float(factor)))
+
+
+def block_add_many(t, parts, dims, stream=None):
+    """Device block t = ((t + parts[0]) + parts[1]) + ... (addresses)."""
+    arr = (_vp * max(len(parts), 1))(*parts)
+    _check(lib().sipgpu_block_add_many(stream.handle if stream else None, t, arr, len(parts),
+                                       len(dims), _iptr(_ints(dims))))
 
 
 def block_norm2(t, dims, result, stream=None, rank=None):

@@ -124,18 +124,31 @@ class TupleEnergy:
     term_groups, each one GEMM task) into `wbuf` and the fused denominator /
     energy reduction into `out` (device double)."""
 
-    def __init__(self, w, tup, wbuf, out, moduli=None, factor=1.0):
+    def __init__(self, w, tup, wbuf, out, moduli=None, factor=1.0, parts=None):
+        """`parts` (optional): five device buffers of at least W's size.  The
+        first batch then assigns W, batch g > 0 assigns its own products into
+        parts[g - 1], and one pass adds the parts to W in batch order: the
+        same roundings as accumulating every batch into W, but the reads of
+        the partial sums stream through HBM in one kernel instead of waiting
+        in five GEMM epilogues."""
         to_device(w)
         self.dims = tuple_dims(w, tup)
         self.batches = []
+        groups = term_groups()
+        if parts is not None and len(parts) < len(groups) - 1:
+            raise ValueError("need %d partial buffers" % (len(groups) - 1))
+        self.parts = list(parts[:len(groups) - 1]) if parts is not None else None
         first = True
-        for group in term_groups():
+        for g, group in enumerate(groups):
+            dest = wbuf if (self.parts is None or g == 0) else self.parts[g - 1]
+            if self.parts is not None:
+                first = True  # every batch assigns its own buffer
             cs = []
             for ptrn, pairs in tuple_entries(w, tup, group):
                 for ln, lb, rn, rb in pairs:
                     L, R = w[ln], w[rn]
                     cs.append(sg.make_contraction(ptrn, L.addr(lb), L.dims(lb), R.addr(rb),
-                                                  R.dims(rb), wbuf, self.dims,
+                                                  R.dims(rb), dest, self.dims,
                                                   accumulate=not first))
                     first = False
             # entries of two patterns into one block plan on panels (the table
@@ -153,6 +166,8 @@ class TupleEnergy:
     def execute(self, s=None):
         for b in self.batches:
             b.execute(s)
+        if self.parts is not None:
+            sg.block_add_many(self.wbuf, self.parts, self.dims, stream=s)
         sg.energy_sum(self.dims, self.wbuf, self.wbuf, self.eps, W_SIGNS, self.factor, self.out,
                       s=s)
 

@@ -404,6 +404,8 @@ def triples_bench(tuples=((0, 1, 2, 0, 0, 0), (3, 4, 5, 0, 0, 0), (6, 7, 8, 0, 0
     tp.to_device(w)
     nmax = max(int(np.prod(tp.tuple_dims(w, t))) for t in tuples)
     wbuf = sg.DeviceArray(nmax)
+    # each batch assigns its own partial W, summed into W in batch order
+    parts = [sg.DeviceArray(nmax) for _ in range(5)]
     out = sg.DeviceArray(1)
     v, o = sum(w["v_segs"]), sum(w["o_segs"])
     whole = 12.0 * (v ** 3) * (o ** 3) * (v + o)
@@ -411,7 +413,8 @@ def triples_bench(tuples=((0, 1, 2, 0, 0, 0), (3, 4, 5, 0, 0, 0), (6, 7, 8, 0, 0
            "whole_T_flop": whole}
     energies = {}
     for nm in moduli:
-        tes = [tp.TupleEnergy(w, t, wbuf.ptr, out.ptr, moduli=nm) for t in tuples]
+        tes = [tp.TupleEnergy(w, t, wbuf.ptr, out.ptr, moduli=nm, parts=[p.ptr for p in parts])
+               for t in tuples]
         flops = sum(te.flops for te in tes)
         out.zero()
         for te in tes:
@@ -440,6 +443,8 @@ def triples_bench(tuples=((0, 1, 2, 0, 0, 0), (3, 4, 5, 0, 0, 0), (6, 7, 8, 0, 0
                     energies[None])
     wbuf.free()
     out.free()
+    for p in parts:
+        p.free()
     for k in ("V1", "V2", "T"):
         w[k].dev.free()
     return res

@@ -318,6 +318,13 @@ int sipgpu_block_scale(sipgpu_stream_t s, double *t, int rank, const int *dims, 
 /* t0 += t1 * factor (a factor within 1e-13 of one is not applied for rank > 0). */
 int sipgpu_block_add(sipgpu_stream_t s, double *t0, const double *t1, int rank, const int *dims,
                      double factor);
+/* t = ((t + parts[0]) + parts[1]) + ... elementwise, 0..8 sources added in
+ * order: the sequence of roundings of the same tensor_block_add__ calls with
+ * factor one, in one pass over HBM.  No reference counterpart as one call;
+ * a block accumulated from several producers (partial results written by
+ * separate batches) is summed with it. */
+int sipgpu_block_add_many(sipgpu_stream_t s, double *t, const double *const *parts, int nparts,
+                          int rank, const int *dims);
 /* *result (a device double) = sum of squares, in an order fixed by the
  * element count (per-workgroup partials, then one workgroup). */
 int sipgpu_block_norm2(sipgpu_stream_t s, const double *t, int rank, const int *dims,

@@ -230,3 +230,31 @@ def test_native_scalar_add_always_scales():
     want, _ = o.block_add([], np.array([1.0]), np.array([3.0]), s)
     assert np.array_equal(_bits(a.download()), _bits(want))
     assert a.download()[0] != 4.0
+
+
+@pytest.mark.parametrize("nparts,n,misalign", [(1, 1000, 0), (5, 1_000_001, 0), (8, 4097, 1),
+                                               (3, 3, 1)])
+def test_native_add_many_vs_oracle(nparts, n, misalign):
+    """t = ((t + p0) + p1) + ...: the oracle's tensor_block_add with factor one,
+    once per source in order; bit-exact."""
+    print("add_many", nparts, n, misalign, flush=True)
+    rng = np.random.default_rng(nparts * 7 + n)
+    t = rng.standard_normal(n)
+    ps = [rng.standard_normal(n) for _ in range(nparts)]
+    want = t.copy()
+    for p in ps:
+        want, e = o.block_add([n], want, p, 1.0)
+        assert e == 0
+    dt = sg.DeviceArray(n + 1)
+    dt.upload_range(misalign, t)
+    dps = []
+    for p in ps:
+        d = sg.DeviceArray(n + 1)
+        d.upload_range(misalign, p)
+        dps.append(d)
+    sg.block_add_many(dt.at(misalign), [d.at(misalign) for d in dps], [n])
+    assert np.array_equal(_bits(dt.download_range(misalign, n)), _bits(want))
+    with pytest.raises(sg.SipGpuError):
+        sg.block_add_many(dt.ptr, [dps[0].ptr] * 9, [n])
+    for d in [dt] + dps:
+        d.free()
