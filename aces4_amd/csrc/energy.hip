@@ -130,6 +130,43 @@ __global__ __launch_bounds__(kEnergyThreads) void energy_partial_kernel(EnergyAr
     if (threadIdx.x == 0) part[blockIdx.x] = red[0];
 }
 
+// The partial blocks' sum and its energy in one pass: t = ((t + p0) + p1) +
+// ... is stored, and the energy of the summed element is accumulated exactly
+// as energy_partial_kernel does with x = y = t (same elements per thread, same
+// order), so the result is the one of the two separate passes.
+__global__ __launch_bounds__(kEnergyThreads) void energy_parts_kernel(EnergyArgs a, double *t,
+                                                                      AddManyArgs ps,
+                                                                      double *part) {
+    __shared__ double red[kEnergyThreads];
+    const long long e0 = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long S = (long long)gridDim.x * blockDim.x;
+    int idx[kEnergyMaxRank], step[kEnergyMaxRank];
+    decode(a, e0, idx);
+    decode(a, S, step);
+    double v = 0.0;
+    for (long long e = e0; e < a.n; e += S) {
+        double x[kAddManyMax];
+        const double t0 = t[e];
+#pragma unroll
+        for (int q = 0; q < kAddManyMax; ++q)
+            if (q < ps.n) x[q] = ps.p[q][e];
+        double w = t0;
+#pragma unroll
+        for (int q = 0; q < kAddManyMax; ++q)
+            if (q < ps.n) w += x[q];
+        t[e] = w;
+        v += (w / denominator(a, idx)) * w;
+        advance(a, idx, step);
+    }
+    red[threadIdx.x] = v;
+    __syncthreads();
+    for (int s = kEnergyThreads / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
 // One workgroup: thread t sums partials t, t + 256, ... in that order, then
 // a fixed tree over the threads (the same order on every run).
 __global__ __launch_bounds__(kEnergyThreads) void energy_final_kernel(double *out,
@@ -154,6 +191,18 @@ hipError_t launch_denominator_divide(const EnergyArgs &a, double *x, hipStream_t
     const long long blocks = (a.n + kEnergyThreads - 1) / kEnergyThreads;
     hipLaunchKernelGGL(denominator_divide_kernel, dim3((unsigned)(blocks < 16384 ? blocks : 16384)),
                        dim3(kEnergyThreads), 0, s, a, x);
+    return hipGetLastError();
+}
+
+hipError_t launch_energy_sum_parts(const EnergyArgs &a, double *t, const AddManyArgs &ps,
+                                   double factor, double *out, double *part, hipStream_t s) {
+    if (a.n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(energy_parts_kernel, dim3(kEnergyBlocks), dim3(kEnergyThreads), 0, s, a, t,
+                       ps, part);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(energy_final_kernel, dim3(1), dim3(kEnergyThreads), 0, s, out, part,
+                       kEnergyBlocks, factor);
     return hipGetLastError();
 }
 

@@ -133,6 +133,10 @@ struct AddManyArgs {
     int n;
 };
 hipError_t launch_block_add_many(double *t, const AddManyArgs &a, long long n, hipStream_t st);
+// launch_block_add_many, then launch_energy_sum with x = y = t, in one pass
+// (the same result bits as the two calls).
+hipError_t launch_energy_sum_parts(const EnergyArgs &a, double *t, const AddManyArgs &ps,
+                                   double factor, double *out, double *part, hipStream_t s);
 // out[0] = sum x[i]^2 over n >= 0 elements: per-workgroup partials (at most
 // kNormBlocks, in `part`) then one workgroup; the order depends on n only.
 constexpr int kNormBlocks = 1024;

@@ -2345,6 +2345,29 @@ int sipgpu_energy_sum(sipgpu_stream_t s, int rank, const int *dims, const double
     return 0;
 }
 
+int sipgpu_energy_sum_parts(sipgpu_stream_t s, int rank, const int *dims, double *t,
+                            const double *const *parts, int nparts, const double *const *eps,
+                            const int *signs, double factor, double *out) {
+    if (!t || !out || (nparts > 0 && !parts)) return fail(SIPGPU_EARG, "energy: null pointer");
+    if (nparts < 0 || nparts > kAddManyMax) return fail(SIPGPU_EARG, "energy: 0..8 partial blocks");
+    AddManyArgs ps{};
+    ps.n = nparts;
+    for (int q = 0; q < nparts; ++q) {
+        if (!parts[q]) return fail(SIPGPU_EARG, "energy: null partial block");
+        ps.p[q] = parts[q];
+    }
+    Context &c = ctx();
+    std::lock_guard<std::mutex> g(c.mu);
+    int rc = ensure_init(c);
+    if (rc) return rc;
+    EnergyArgs a;
+    if ((rc = energy_args(rank, dims, eps, signs, a))) return rc;
+    sipgpu_stream_s &st = resolve(c, s);
+    if ((rc = ensure_stream_scratch(st))) return rc;
+    HIP_TRY(launch_energy_sum_parts(a, t, ps, factor, out, st.scratch, st.s));
+    return 0;
+}
+
 int sipgpu_fill_random(sipgpu_stream_t s, double *out, long long n, unsigned long long seed,
                        unsigned long long stream, int dist, double a, double b) {
     if (n < 0 || (n > 0 && !out) || (dist != 0 && dist != 1))

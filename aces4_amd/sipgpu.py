@@ -51,6 +51,7 @@ EXPORTED = [
     "sipgpu_block_slice", "sipgpu_block_insert", "sipgpu_plan_slice", "sipgpu_cache_remove",
     "sipgpu_cache_scope_enter", "sipgpu_cache_scope_add", "sipgpu_cache_scope_leave",
     "sipgpu_cache_set_persistent", "sipgpu_cache_end_program", "sipgpu_block_add_many",
+    "sipgpu_energy_sum_parts",
     "sipgpu_cache_create", "sipgpu_cache_destroy", "sipgpu_cache_insert", "sipgpu_cache_acquire",
     "sipgpu_cache_release", "sipgpu_cache_read", "sipgpu_cache_flush", "sipgpu_cache_evict",
     "sipgpu_cache_get_stats", "sipgpu_server_create", "sipgpu_server_submit",
@@ -205,6 +206,8 @@ def lib():
         L.sipgpu_energy_denominator.argtypes = [_vp, ctypes.c_int, _ip, _vp, _pp, _ip]
         L.sipgpu_energy_sum.argtypes = [_vp, ctypes.c_int, _ip, _vp, _vp, _pp, _ip,
                                         ctypes.c_double, _vp]
+        L.sipgpu_energy_sum_parts.argtypes = [_vp, ctypes.c_int, _ip, _vp, _pp, ctypes.c_int, _pp,
+                                              _ip, ctypes.c_double, _vp]
         L.sipgpu_fill_random.argtypes = [_vp, _vp, ctypes.c_longlong, ctypes.c_ulonglong,
                                          ctypes.c_ulonglong, ctypes.c_int, ctypes.c_double,
                                          ctypes.c_double]
@@ -453,6 +456,14 @@ def energy_sum(dims, x, y, eps, signs, factor, out, s=None):
     r, d, e, sg_ = _energy_arrays(dims, eps, signs)
     _check(lib().sipgpu_energy_sum(None if s is None else s.handle, r, d, x, y, e, sg_,
                                    float(factor), out))
+
+
+def energy_sum_parts(dims, t, parts, eps, signs, factor, out, s=None):
+    """t = ((t + parts[0]) + ...) stored, then *out += factor * sum (t / D) * t."""
+    r, d, e, sg_ = _energy_arrays(dims, eps, signs)
+    arr = (ctypes.c_void_p * max(len(parts), 1))(*[int(p) for p in parts])
+    _check(lib().sipgpu_energy_sum_parts(None if s is None else s.handle, r, d, t, arr, len(parts),
+                                         e, sg_, float(factor), out))
 
 
 def measured_clock_mhz():

@@ -127,10 +127,10 @@ class TupleEnergy:
     def __init__(self, w, tup, wbuf, out, moduli=None, factor=1.0, parts=None):
         """`parts` (optional): five device buffers of at least W's size.  The
         first batch then assigns W, batch g > 0 assigns its own products into
-        parts[g - 1], and one pass adds the parts to W in batch order: the
-        same roundings as accumulating every batch into W, but the reads of
-        the partial sums stream through HBM in one kernel instead of waiting
-        in five GEMM epilogues."""
+        parts[g - 1], and one pass adds the parts to W in batch order and
+        reduces the energy: the same roundings as accumulating every batch
+        into W, but the reads of the partial sums stream through HBM in one
+        kernel instead of waiting in five GEMM epilogues."""
         to_device(w)
         self.dims = tuple_dims(w, tup)
         self.batches = []
@@ -167,7 +167,10 @@ class TupleEnergy:
         for b in self.batches:
             b.execute(s)
         if self.parts is not None:
-            sg.block_add_many(self.wbuf, self.parts, self.dims, stream=s)
+            # the parts' sum and the energy in one pass
+            sg.energy_sum_parts(self.dims, self.wbuf, self.parts, self.eps, W_SIGNS, self.factor,
+                                self.out, s=s)
+            return
         sg.energy_sum(self.dims, self.wbuf, self.wbuf, self.eps, W_SIGNS, self.factor, self.out,
                       s=s)
 

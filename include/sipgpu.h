@@ -305,6 +305,14 @@ int sipgpu_energy_denominator(sipgpu_stream_t s, int rank, const int *dims, doub
 int sipgpu_energy_sum(sipgpu_stream_t s, int rank, const int *dims, const double *x,
                       const double *y, const double *const *eps, const int *signs,
                       double factor, double *out);
+/* t = ((t + parts[0]) + parts[1]) + ... (0..8 partial blocks, stored in t),
+ * then *out += factor * sum_e (t[e] / D(e)) * t[e], in one pass: the result
+ * of sipgpu_block_add_many followed by sipgpu_energy_sum(x = y = t), bit for
+ * bit.  A W block whose terms were written as partial blocks is summed and
+ * reduced without reading it back. */
+int sipgpu_energy_sum_parts(sipgpu_stream_t s, int rank, const int *dims, double *t,
+                            const double *const *parts, int nparts, const double *const *eps,
+                            const int *signs, double factor, double *out);
 
 /* ------------------------------------------------------------------------
  * Block operations on device blocks, enqueued on `s` (asynchronous): the

@@ -118,3 +118,31 @@ def test_tuple_with_partial_buffers_is_bitwise_equal(small, moduli):
     (w0, e0), (w1, e1) = results
     assert np.array_equal(w0.view(np.uint64), w1.view(np.uint64))
     assert e0 == e1
+
+
+def test_energy_sum_parts_equals_the_two_passes():
+    """sipgpu_energy_sum_parts = block_add_many then energy_sum(x = y = t), bit
+    for bit, on an odd-sized rank-6 block."""
+    rng = np.random.default_rng(9)
+    dims = [3, 4, 5, 2, 3, 7]
+    n = int(np.prod(dims))
+    signs = [-1, -1, -1, 1, 1, 1]
+    eps = [rng.uniform(0.1, 3, d) if s < 0 else rng.uniform(-2, -0.3, d) for d, s in zip(dims, signs)]
+    ed = [sg.DeviceArray.from_host(e) for e in eps]
+    t0 = rng.standard_normal(n)
+    ps = [rng.standard_normal(n) for _ in range(5)]
+    dps = [sg.DeviceArray.from_host(p) for p in ps]
+    a, b = sg.DeviceArray.from_host(t0), sg.DeviceArray.from_host(t0)
+    oa, ob = sg.DeviceArray(1), sg.DeviceArray(1)
+    oa.zero()
+    ob.zero()
+    sg.block_add_many(a.ptr, [d.ptr for d in dps], dims)
+    sg.energy_sum(dims, a.ptr, a.ptr, [e.ptr for e in ed], signs, 0.5, oa.ptr)
+    sg.energy_sum_parts(dims, b.ptr, [d.ptr for d in dps], [e.ptr for e in ed], signs, 0.5, ob.ptr)
+    sg.synchronize()
+    assert np.array_equal(a.download().view(np.uint64), b.download().view(np.uint64))
+    assert oa.download()[0] == ob.download()[0]
+    want = t0.copy()
+    for p in ps:
+        want = want + p
+    assert np.array_equal(b.download(), want)
