@@ -381,11 +381,15 @@ def graph_bench(reps=20):
     w3 = tp.triples()
     tp.to_device(w3)
     t = (0, 1, 2, 0, 0, 0)
-    wbuf = sg.DeviceArray(int(np.prod(tp.tuple_dims(w3, t))))
-    te = tp.TupleEnergy(w3, t, wbuf.ptr, out.ptr)
+    nw = int(np.prod(tp.tuple_dims(w3, t)))
+    wbuf = sg.DeviceArray(nw)
+    parts = [sg.DeviceArray(nw) for _ in range(5)]
+    te = tp.TupleEnergy(w3, t, wbuf.ptr, out.ptr, parts=[p.ptr for p in parts])
     compare("triples_tuple", lambda st: te.execute(st), te.flops)
     te.destroy()
     wbuf.free()
+    for p in parts:
+        p.free()
     for k in ("V1", "V2", "T"):
         w3[k].dev.free()
     s.destroy()

@@ -45,6 +45,14 @@ def test_graph_replays_match_eager(monkeypatch):
     eps = [sg.DeviceArray.from_host(rng.uniform(0.5, 2.0, d)) for d in dims0]
     out = sg.DeviceArray(1)
     ptrn = [1, -1, -2, 2]  # D(i,j) = L(i,k) R(k,j)
+    # block operations: a slice of the permuted block, its squared norm, and a
+    # two-part sum with its energy
+    sdims = [max(1, d // 2) for d in dims0]
+    Sl = sg.DeviceArray(int(np.prod(sdims)))
+    N2 = sg.DeviceArray(1)
+    Q = sg.DeviceArray.from_host(rng.standard_normal(int(np.prod(dims0))))
+    T2 = sg.DeviceArray(int(np.prod(dims0)))
+    out2 = sg.DeviceArray(1)
 
     def iteration(st):
         b_packed.execute(st)
@@ -55,16 +63,24 @@ def test_graph_replays_match_eager(monkeypatch):
         sg.fill_random(F.ptr, 1000, 11, 3, "normal", 0.5, s=st)
         sg.energy_sum(dims0, w1["D"].addr(db0), P.ptr, [e.ptr for e in eps], [1, -1, 1, -1],
                       0.5, out.ptr, s=st)
+        sg.block_slice(P.ptr, dims0, Sl.ptr, sdims, [1] * len(dims0), stream=st)
+        sg.block_norm2(P.ptr, dims0, N2.ptr, stream=st)
+        sg.block_init(T2.ptr, dims0, 0.0, stream=st)
+        sg.block_add(T2.ptr, P.ptr, dims0, 0.5, stream=st)
+        sg.energy_sum_parts(dims0, T2.ptr, [Q.ptr, P.ptr], [e.ptr for e in eps], [1, -1, 1, -1],
+                            0.25, out2.ptr, s=st)
 
     def reset():
         for w in (w1, w2, w3):
             w["D"].dev.upload(w["D"].host)
         C.zero()
         out.zero()
+        out2.zero()
 
     def results():
         return [w1["D"].dev.download(), w2["D"].dev.download(), w3["D"].dev.download(),
-                C.download(), P.download(), F.download(), out.download()]
+                C.download(), P.download(), F.download(), out.download(), Sl.download(),
+                N2.download(), T2.download(), out2.download()]
 
     reset()
     iteration(s)
