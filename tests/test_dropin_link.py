@@ -16,8 +16,9 @@ LIBDIR = os.path.dirname(sg.LIB_PATH)
 
 def build_caller(tmp_path):
     exe = str(tmp_path / "dropin_caller")
+    # linked by path: the library the tests load, wherever it lives
     subprocess.run(["g++", "-O2", "-std=c++17", "-ffp-contract=off",
-                    os.path.join(HERE, "dropin_caller.cpp"), "-o", exe, "-L", LIBDIR, "-lsipgpu",
+                    os.path.join(HERE, "dropin_caller.cpp"), "-o", exe, sg.LIB_PATH,
                     "-Wl,-rpath," + LIBDIR], check=True)
     return exe
 
