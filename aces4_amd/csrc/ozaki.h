@@ -98,7 +98,7 @@ __device__ __forceinline__ double *oz_dest(const OzTask &t, const OzSub *subs, c
 }
 
 // A GEMM work unit: tile (mt, nt) of a task over stages [s0, s1) of its
-// virtual K (whole pairs of stages).  part < 0: the whole K, the product is
+// virtual K (whole stages).  part < 0: the whole K, the product is
 // stored; part >= 0: one K segment of a split tile, its accumulators go to
 // workspace slot `part` and the fixup pass sums the tile's segments.  In the
 // fixup list, part is the tile's first slot and s0 its number of segments.

@@ -1120,9 +1120,9 @@ int sipgpu::rt::build_panel_plan(const sipgpu_contraction *list, int n, int nmod
             align_panel();
             return base;
         };
-        // virtual K: the pairs' panel chunks, padded to whole pairs of
-        // stages (the GEMM runs two stages per loop trip) with chunks of the
-        // zero bytes
+        // virtual K: the pairs' panel chunks, padded with chunks of the zero
+        // bytes to whole stages (FP64) or whole pairs of stages (the INT8
+        // GEMM runs two stages per loop trip)
         tasks[t].chunk0 = (int)chunks.size();
         const int np = (int)rvec[rs[0]].size();
         for (int p = 0; p < np; ++p) {
@@ -1136,7 +1136,7 @@ int sipgpu::rt::build_panel_plan(const sipgpu_contraction *list, int n, int nmod
                 return fail(SIPGPU_ESIZE, nmod ? "split path: operand panels exceed 4 GiB per modulus"
                                                : "packed path: operand panels exceed 64 GiB");
         }
-        while ((chunks.size() - tasks[t].chunk0) % (2 * kOzStageChunks))
+        while ((chunks.size() - tasks[t].chunk0) % ((nmod ? 2 : 1) * kOzStageChunks))
             chunks.push_back(OzChunkRef{0u, 0u, 0, 0});
         tasks[t].nchunks = (int)chunks.size() - tasks[t].chunk0;
     }
