@@ -137,6 +137,8 @@ class TupleEnergy:
         groups = term_groups()
         if parts is not None and len(parts) < len(groups) - 1:
             raise ValueError("need %d partial buffers" % (len(groups) - 1))
+        if parts is not None and len(set(parts[:len(groups) - 1]) | {wbuf}) != len(groups):
+            raise ValueError("the partial buffers must be distinct from each other and from W")
         self.parts = list(parts[:len(groups) - 1]) if parts is not None else None
         first = True
         for g, group in enumerate(groups):
