@@ -124,13 +124,15 @@ class TupleEnergy:
     term_groups, each one GEMM task) into `wbuf` and the fused denominator /
     energy reduction into `out` (device double)."""
 
-    def __init__(self, w, tup, wbuf, out, moduli=None, factor=1.0, parts=None):
+    def __init__(self, w, tup, wbuf, out, moduli=None, factor=1.0, parts=None, tune=False):
         """`parts` (optional): five device buffers of at least W's size.  The
         first batch then assigns W, batch g > 0 assigns its own products into
         parts[g - 1], and one pass adds the parts to W in batch order and
         reduces the energy: the same roundings as accumulating every batch
         into W, but the reads of the partial sums stream through HBM in one
-        kernel instead of waiting in five GEMM epilogues."""
+        kernel instead of waiting in five GEMM epilogues.  `tune`: plan each
+        batch with the autotuner (its timing runs write scratch blocks only;
+        batches of equal shape share one tuning)."""
         to_device(w)
         self.dims = tuple_dims(w, tup)
         self.batches = []
@@ -155,7 +157,7 @@ class TupleEnergy:
                     first = False
             # entries of two patterns into one block plan on panels (the table
             # kernel takes one pattern per block)
-            self.batches.append(sg.Batch(cs, moduli=moduli))
+            self.batches.append(sg.tune(cs, moduli=moduli)[0] if tune else sg.Batch(cs, moduli=moduli))
         segs = {"v": w["v_segs"], "o": w["o_segs"]}
         self.eps = []
         for lab, s in zip(W_LABELS, tup):

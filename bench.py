@@ -384,7 +384,7 @@ def graph_bench(reps=20):
     nw = int(np.prod(tp.tuple_dims(w3, t)))
     wbuf = sg.DeviceArray(nw)
     parts = [sg.DeviceArray(nw) for _ in range(5)]
-    te = tp.TupleEnergy(w3, t, wbuf.ptr, out.ptr, parts=[p.ptr for p in parts])
+    te = tp.TupleEnergy(w3, t, wbuf.ptr, out.ptr, parts=[p.ptr for p in parts], tune=True)
     compare("triples_tuple", lambda st: te.execute(st), te.flops)
     te.destroy()
     wbuf.free()
@@ -417,7 +417,8 @@ def triples_bench(tuples=((0, 1, 2, 0, 0, 0), (3, 4, 5, 0, 0, 0), (6, 7, 8, 0, 0
            "whole_T_flop": whole}
     energies = {}
     for nm in moduli:
-        tes = [tp.TupleEnergy(w, t, wbuf.ptr, out.ptr, moduli=nm, parts=[p.ptr for p in parts])
+        tes = [tp.TupleEnergy(w, t, wbuf.ptr, out.ptr, moduli=nm, parts=[p.ptr for p in parts],
+                              tune=True)
                for t in tuples]
         flops = sum(te.flops for te in tes)
         out.zero()

@@ -146,3 +146,26 @@ def test_energy_sum_parts_equals_the_two_passes():
     for p in ps:
         want = want + p
     assert np.array_equal(b.download(), want)
+
+
+def test_tuned_tuple_with_parts_vs_oracle(small):
+    """Autotuned batches into partial buffers: W and the energy against the
+    oracle's contract + add sequence."""
+    w, host = small
+    tup = TUPLES[2]
+    dims = tp.tuple_dims(w, tup)
+    n = int(np.prod(dims))
+    print("triples tuned with parts", tup, flush=True)
+    wbuf = sg.DeviceArray(n)
+    out = sg.DeviceArray(1)
+    out.zero()
+    parts = [sg.DeviceArray(n) for _ in range(5)]
+    te = tp.TupleEnergy(w, tup, wbuf.ptr, out.ptr, parts=[p.ptr for p in parts], tune=True)
+    te.execute()
+    sg.synchronize()
+    W_ref, E_ref = oracle_tuple(w, host, tup)
+    assert rel_fro(wbuf.download(), W_ref) <= 1e-13
+    assert abs(out.download()[0] - E_ref) <= 1e-13 * abs(E_ref)
+    te.destroy()
+    with pytest.raises(ValueError):
+        tp.TupleEnergy(w, tup, wbuf.ptr, out.ptr, parts=[wbuf.ptr] + [p.ptr for p in parts[:4]])
