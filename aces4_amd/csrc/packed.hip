@@ -27,57 +27,62 @@ typedef double f64x4 __attribute__((ext_vector_type(4)));
 typedef double f64x2 __attribute__((ext_vector_type(2)));
 typedef const __attribute__((address_space(1))) int8_t gi8_t;
 
-// Persistent workgroups over 32 x 128 tiles of operand blocks: read along
-// the index adjacent in memory, transposed through LDS, written as 128-byte
-// row pieces of the block's panel.
+// Workgroups over 32 x 128 tiles of operand blocks, each done as two halves
+// of 64 k (half the LDS of a whole tile, so twice the workgroups per CU keep
+// their loads in flight): read along the index adjacent in memory,
+// transposed through LDS, written as 128-byte row pieces of the panel.
 __global__ __launch_bounds__(256) void pack_kernel(OzConvArgs a) {
-    __shared__ double S[kOzTileX][kOzTileK + 1];
+    constexpr int HK = kOzTileK / 2;
+    __shared__ __attribute__((aligned(16))) double S[kOzTileX][HK + 2];
     const int tid = threadIdx.x;
     for (int it = blockIdx.x; it < a.nitems; it += gridDim.x) {
         const OzConvTile w = a.items[it];
         const OzBlock b = a.blocks[w.block];
-        double v[16];
-        if (b.xfast) {
-            const int x = w.x0 + (tid & 31);
-            const bool xin = x < b.X;
-            const int xo = xin ? a.tab[b.xtab + x] : 0;
+#pragma unroll 1
+        for (int half = 0; half < 2; ++half) {
+            const int kb = w.k0 + half * HK;
+            double v[8];
+            if (b.xfast) {
+                const int x = w.x0 + (tid & 31);
+                const bool xin = x < b.X;
+                const int xo = xin ? a.tab[b.xtab + x] : 0;
 #pragma unroll
-            for (int j = 0; j < 16; ++j) {
-                const int k = w.k0 + (tid >> 5) + 8 * j;
-                v[j] = (xin && k < b.K) ? b.data[a.tab[b.ktab + k] + xo] : 0.0;
+                for (int j = 0; j < 8; ++j) {
+                    const int k = kb + (tid >> 5) + 8 * j;
+                    v[j] = (xin && k < b.K) ? b.data[a.tab[b.ktab + k] + xo] : 0.0;
+                }
+            } else {
+                const int k = kb + (tid & 63);
+                const bool kin = k < b.K;
+                const int ko = kin ? a.tab[b.ktab + k] : 0;
+#pragma unroll
+                for (int j = 0; j < 8; ++j) {
+                    const int x = w.x0 + (tid >> 6) + 4 * j;
+                    v[j] = (kin && x < b.X) ? b.data[ko + a.tab[b.xtab + x]] : 0.0;
+                }
             }
-        } else {
-            const int k = w.k0 + (tid & 127);
-            const bool kin = k < b.K;
-            const int ko = kin ? a.tab[b.ktab + k] : 0;
+            __syncthreads(); // the previous LDS reads are done
+            if (b.xfast) {
 #pragma unroll
-            for (int j = 0; j < 16; ++j) {
-                const int x = w.x0 + (tid >> 7) + 2 * j;
-                v[j] = (kin && x < b.X) ? b.data[ko + a.tab[b.xtab + x]] : 0.0;
+                for (int j = 0; j < 8; ++j) S[tid & 31][(tid >> 5) + 8 * j] = v[j];
+            } else {
+#pragma unroll
+                for (int j = 0; j < 8; ++j) S[(tid >> 6) + 4 * j][tid & 63] = v[j];
             }
-        }
-        __syncthreads(); // the previous tile's LDS reads are done
-        if (b.xfast) {
+            __syncthreads();
+            // 8 lanes per row, 4 pairs each: a row's 8 lanes cover 128 bytes
+            const int xx = tid >> 3, l = tid & 7;
+            const int x = w.x0 + xx;
+            if (x < b.X) {
+                double *dst = reinterpret_cast<double *>(a.planes + b.panel_off) +
+                              (long long)x * b.Kp + kb;
 #pragma unroll
-            for (int j = 0; j < 16; ++j) S[tid & 31][(tid >> 5) + 8 * j] = v[j];
-        } else {
-#pragma unroll
-            for (int j = 0; j < 16; ++j) S[(tid >> 7) + 2 * j][tid & 127] = v[j];
-        }
-        __syncthreads();
-        // 8 lanes per row; at store j lane l writes the pair k = 2 (8 j + l):
-        // a row's 8 lanes cover 128 contiguous bytes of the panel, and the
-        // LDS reads of 16 consecutive lanes (two rows) fall in distinct banks
-        const int xx = tid >> 3, l = tid & 7;
-        const int x = w.x0 + xx;
-        if (x < b.X) {
-            double *dst =
-                reinterpret_cast<double *>(a.planes + b.panel_off) + (long long)x * b.Kp + w.k0;
-#pragma unroll
-            for (int j = 0; j < 8; ++j) {
-                const int k = 2 * (8 * j + l);
-                if (w.k0 + k < b.Kp) // Kp is even: whole 16-byte pairs
-                    *reinterpret_cast<f64x2 *>(dst + k) = f64x2{S[xx][k], S[xx][k + 1]};
+                for (int j = 0; j < 4; ++j) {
+                    const int k = 2 * (8 * j + l);
+                    if (kb + k < b.Kp) // Kp is even: whole 16-byte pairs
+                        *reinterpret_cast<f64x2 *>(dst + k) =
+                            *reinterpret_cast<const f64x2 *>(&S[xx][k]);
+                }
             }
         }
     }
