@@ -268,3 +268,27 @@ def test_library_exports_the_whole_reference_module():
     assert not missing, missing
     names = declared_functions()
     assert all(n in names for n in REFERENCE_MODULE_SYMBOLS)
+
+
+def test_native_block_operations_fail_loudly_without_gpu():
+    """No CPU fallback: without a device every native block operation returns
+    an error (the fake device address is never touched)."""
+    if sg.device_count() > 0:
+        pytest.skip("a GPU is present")
+    fake = 0x100000
+    calls = [
+        lambda: sg.block_init(fake, [4, 4], 1.0),
+        lambda: sg.block_scale(fake, [4, 4], 2.0),
+        lambda: sg.block_add(fake, fake + 4096, [4, 4], 0.5),
+        lambda: sg.block_norm2(fake, [4, 4], fake + 4096),
+        lambda: sg.block_slice(fake, [4, 4], fake + 4096, [2, 2], [1, 1]),
+        lambda: sg.block_insert(fake, [4, 4], fake + 4096, [2, 2], [1, 1]),
+        lambda: sg.block_add_many(fake, [fake + 4096], [4, 4]),
+    ]
+    for call in calls:
+        with pytest.raises(sg.SipGpuError):
+            call()
+    # argument errors are reported before the device is looked for
+    with pytest.raises(sg.SipGpuError) as ei:
+        sg.block_slice(fake, [4, 4], fake + 4096, [2, 2], [3, 0])
+    assert ei.value.code == -107
