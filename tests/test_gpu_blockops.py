@@ -258,3 +258,40 @@ def test_native_add_many_vs_oracle(nparts, n, misalign):
         sg.block_add_many(dt.ptr, [dps[0].ptr] * 9, [n])
     for d in [dt] + dps:
         d.free()
+
+
+def test_amplitude_update_sequence_bitwise():
+    """The T2 update of rccsd_rhf.sialx (UPDATET2, :2053-2057) as device block
+    operations -- Taibj = T2new(b,j,a,i); Taibj += T2new(a,i,b,j); Taibj *=
+    0.5; energy_denominator_rhf -- against the same sequence in the oracle,
+    bit for bit; then the DIIS error T1 -= Taibj; T1 *= -0.5 (:2030-2031)."""
+    rng = np.random.default_rng(17)
+    dims = [30, 15, 30, 15]  # (a, i, b, j) of one ring block
+    n = int(np.prod(dims))
+    t2new = rng.standard_normal(n)
+    t1 = rng.standard_normal(n)
+    signs = [-1, 1, -1, 1]
+    eps = [rng.uniform(0.1, 3, d) if s < 0 else rng.uniform(-2, -0.3, d) for d, s in zip(dims, signs)]
+    # oracle: (a,i,b,j) -> (b,j,a,i) is dim_transp [1, 3, 4, 1, 2]
+    tr = [1, 3, 4, 1, 2]
+    want, e = o.copy(dims, tr, t2new)
+    assert e == 0
+    want, e = o.block_add(dims, want, t2new, 1.0)
+    want, e = o.block_scale(dims, want, 0.5)
+    want = o.energy_denominator(dims, want, eps, signs)
+    diis, e = o.block_add(dims, t1, want, -1.0)
+    diis, e = o.block_scale(dims, diis, -0.5)
+    # device
+    d_new = sg.DeviceArray.from_host(t2new)
+    d_t = sg.DeviceArray(n)
+    d_t1 = sg.DeviceArray.from_host(t1)
+    ed = [sg.DeviceArray.from_host(x) for x in eps]
+    sg.permute(dims, tr, d_new.ptr, d_t.ptr)
+    sg.block_add(d_t.ptr, d_new.ptr, dims, 1.0)
+    sg.block_scale(d_t.ptr, dims, 0.5)
+    sg.energy_denominator(dims, d_t.ptr, [x.ptr for x in ed], signs)
+    sg.block_add(d_t1.ptr, d_t.ptr, dims, -1.0)
+    sg.block_scale(d_t1.ptr, dims, -0.5)
+    sg.synchronize()
+    assert np.array_equal(_bits(d_t.download()), _bits(want))
+    assert np.array_equal(_bits(d_t1.download()), _bits(diis))
