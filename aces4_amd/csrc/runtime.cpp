@@ -1929,6 +1929,17 @@ int sipgpu_event_record(sipgpu_event_t e, sipgpu_stream_t s) {
     HIP_TRY(hipEventRecord(e->e, resolve(c, s).s));
     return 0;
 }
+int sipgpu_stream_wait_event(sipgpu_stream_t s, sipgpu_event_t e) {
+    if (!e) return fail(SIPGPU_EARG, "null event");
+    Context &c = ctx();
+    {
+        std::lock_guard<std::mutex> g(c.mu);
+        int rc = ensure_init(c);
+        if (rc) return rc;
+    }
+    HIP_TRY(hipStreamWaitEvent(resolve(c, s).s, e->e, 0));
+    return 0;
+}
 int sipgpu_event_synchronize(sipgpu_event_t e) {
     if (!e) return fail(SIPGPU_EARG, "null event");
     HIP_TRY(hipEventSynchronize(e->e));

@@ -51,7 +51,7 @@ EXPORTED = [
     "sipgpu_block_slice", "sipgpu_block_insert", "sipgpu_plan_slice", "sipgpu_cache_remove",
     "sipgpu_cache_scope_enter", "sipgpu_cache_scope_add", "sipgpu_cache_scope_leave",
     "sipgpu_cache_set_persistent", "sipgpu_cache_end_program", "sipgpu_block_add_many",
-    "sipgpu_energy_sum_parts",
+    "sipgpu_energy_sum_parts", "sipgpu_stream_wait_event",
     "sipgpu_cache_create", "sipgpu_cache_destroy", "sipgpu_cache_insert", "sipgpu_cache_acquire",
     "sipgpu_cache_release", "sipgpu_cache_read", "sipgpu_cache_flush", "sipgpu_cache_evict",
     "sipgpu_cache_get_stats", "sipgpu_server_create", "sipgpu_server_submit",
@@ -138,6 +138,7 @@ def lib():
         L.sipgpu_event_create.argtypes = [ctypes.POINTER(_vp)]
         L.sipgpu_event_destroy.argtypes = [_vp]
         L.sipgpu_event_record.argtypes = [_vp, _vp]
+        L.sipgpu_stream_wait_event.argtypes = [_vp, _vp]
         L.sipgpu_event_synchronize.argtypes = [_vp]
         L.sipgpu_event_elapsed_ms.argtypes = [_vp, _vp, ctypes.POINTER(ctypes.c_float)]
         L.sipgpu_permute.argtypes = [_vp, ctypes.c_int, _ip, _ip, _vp, _vp]
@@ -547,6 +548,10 @@ class Stream:
     def synchronize(self):
         _check(lib().sipgpu_stream_synchronize(self.handle))
 
+    def wait(self, event):
+        """Later work on this stream waits for `event` (no host wait)."""
+        _check(lib().sipgpu_stream_wait_event(self.handle, event.handle))
+
     def destroy(self):
         if self.handle:
             _check(lib().sipgpu_stream_destroy(self.handle))
@@ -674,6 +679,12 @@ def plan_slice(tens_ext, slice_ext, ext_beg):
                                    _iptr(_ints(ext_beg)), ctypes.byref(cr), _iptr(ext), st, info))
     r = cr.value
     return [int(x) for x in ext[:r]], [int(x) for x in st[:r]], info[0], info[1], info[2], info[3]
+
+
+def stream_wait(event, stream=None):
+    """Work enqueued on `stream` (None: the library's stream) after this
+    call waits for `event`."""
+    _check(lib().sipgpu_stream_wait_event(stream.handle if stream else None, event.handle))
 
 
 def permute(rank_dims, dim_transp, din, dout, stream=None):

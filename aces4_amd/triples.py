@@ -124,7 +124,8 @@ class TupleEnergy:
     term_groups, each one GEMM task) into `wbuf` and the fused denominator /
     energy reduction into `out` (device double)."""
 
-    def __init__(self, w, tup, wbuf, out, moduli=None, factor=1.0, parts=None, tune=False):
+    def __init__(self, w, tup, wbuf, out, moduli=None, factor=1.0, parts=None, tune=False,
+                 two_streams=False):
         """`parts` (optional): five device buffers of at least W's size.  The
         first batch then assigns W, batch g > 0 assigns its own products into
         parts[g - 1], and one pass adds the parts to W in batch order and
@@ -132,7 +133,12 @@ class TupleEnergy:
         into W, but the reads of the partial sums stream through HBM in one
         kernel instead of waiting in five GEMM epilogues.  `tune`: plan each
         batch with the autotuner (its timing runs write scratch blocks only;
-        batches of equal shape share one tuning)."""
+        batches of equal shape share one tuning).  `two_streams` (with
+        `parts`): the batches alternate between the caller's stream and a
+        second one, joined before the summing pass, so one batch's pack and
+        last partial round of tiles overlap the next batch's GEMM; the
+        batches write distinct buffers and own their panels, so the result
+        is the same."""
         to_device(w)
         self.dims = tuple_dims(w, tup)
         self.batches = []
@@ -166,10 +172,23 @@ class TupleEnergy:
             self.eps.append(base.at(seg_offset(segs[kind], s)))
         self.wbuf, self.out, self.factor = wbuf, out, factor
         self.flops = sum(b.flops for b in self.batches)
+        self.aux = None
+        if two_streams and self.parts is not None:
+            self.aux = sg.Stream()
+            self.fork, self.join = sg.Event(), sg.Event()
 
     def execute(self, s=None):
-        for b in self.batches:
-            b.execute(s)
+        if self.aux is not None:
+            # fork: the second stream starts after the caller's earlier work
+            self.fork.record(s)
+            self.aux.wait(self.fork)
+            for g, b in enumerate(self.batches):
+                b.execute(s if g % 2 == 0 else self.aux)
+            self.join.record(self.aux)
+            sg.stream_wait(self.join, s)
+        else:
+            for b in self.batches:
+                b.execute(s)
         if self.parts is not None:
             # the parts' sum and the energy in one pass
             sg.energy_sum_parts(self.dims, self.wbuf, self.parts, self.eps, W_SIGNS, self.factor,
@@ -182,3 +201,7 @@ class TupleEnergy:
         for b in self.batches:
             b.destroy()
         self.batches = []
+        if self.aux is not None:
+            self.aux.synchronize()
+            self.aux.destroy()
+            self.aux = None

@@ -418,7 +418,7 @@ def triples_bench(tuples=((0, 1, 2, 0, 0, 0), (3, 4, 5, 0, 0, 0), (6, 7, 8, 0, 0
     energies = {}
     for nm in moduli:
         tes = [tp.TupleEnergy(w, t, wbuf.ptr, out.ptr, moduli=nm, parts=[p.ptr for p in parts],
-                              tune=True)
+                              tune=True, two_streams=True)
                for t in tuples]
         flops = sum(te.flops for te in tes)
         out.zero()

@@ -166,6 +166,9 @@ int sipgpu_stream_synchronize(sipgpu_stream_t s);
 int sipgpu_event_create(sipgpu_event_t *e);
 int sipgpu_event_destroy(sipgpu_event_t e);
 int sipgpu_event_record(sipgpu_event_t e, sipgpu_stream_t s);
+/* Work enqueued on `s` after this call waits for the work recorded by `e`
+ * (no host wait): the join of work split over several streams. */
+int sipgpu_stream_wait_event(sipgpu_stream_t s, sipgpu_event_t e);
 int sipgpu_event_synchronize(sipgpu_event_t e);
 int sipgpu_event_elapsed_ms(sipgpu_event_t start, sipgpu_event_t stop, float *ms);
 

@@ -97,27 +97,32 @@ def test_denominator_kernel_bitwise():
 @pytest.mark.parametrize("moduli", [None, 14], ids=["fp64", "split14"])
 def test_tuple_with_partial_buffers_is_bitwise_equal(small, moduli):
     """Batches assigned into separate buffers and summed in batch order give
-    the same W bits and the same energy as batches accumulated into W."""
+    the same W bits and the same energy as batches accumulated into W, on one
+    stream and split over two."""
     w, host = small
     tup = TUPLES[0]
     dims = tp.tuple_dims(w, tup)
     n = int(np.prod(dims))
     print("triples partial buffers", tup, moduli, flush=True)
     results = []
-    for use_parts in (False, True):
+    for use_parts, two in ((False, False), (True, False), (True, True)):
         wbuf = sg.DeviceArray(n)
         out = sg.DeviceArray(1)
         out.zero()
         parts = [sg.DeviceArray(n) for _ in range(5)] if use_parts else None
         te = tp.TupleEnergy(w, tup, wbuf.ptr, out.ptr, moduli=moduli,
-                            parts=[p.ptr for p in parts] if parts else None)
-        te.execute()
-        sg.synchronize()
+                            parts=[p.ptr for p in parts] if parts else None, two_streams=two)
+        s = sg.Stream()
+        for _ in range(2):  # a second run accumulates a second energy
+            te.execute(s)
+        s.synchronize()
         results.append((wbuf.download(), out.download()[0]))
         te.destroy()
-    (w0, e0), (w1, e1) = results
-    assert np.array_equal(w0.view(np.uint64), w1.view(np.uint64))
-    assert e0 == e1
+        s.destroy()
+    (w0, e0), *rest = results
+    for w1, e1 in rest:
+        assert np.array_equal(w0.view(np.uint64), w1.view(np.uint64))
+        assert e0 == e1
 
 
 def test_energy_sum_parts_equals_the_two_passes():
