@@ -292,3 +292,8 @@ def test_native_block_operations_fail_loudly_without_gpu():
     with pytest.raises(sg.SipGpuError) as ei:
         sg.block_slice(fake, [4, 4], fake + 4096, [2, 2], [3, 0])
     assert ei.value.code == -107
+
+
+def test_stream_wait_event_rejects_a_null_event():
+    rc = sg.lib().sipgpu_stream_wait_event(None, None)
+    assert rc == -107
