@@ -132,3 +132,35 @@ def test_capture_refuses_planning_and_host_copies():
     g.destroy()
     b.destroy()
     s.destroy()
+
+
+def test_graph_of_two_stream_triples_tuple():
+    """A triples tuple whose batches alternate over two streams (fork / join
+    with events) captured from the caller's stream: every replay reproduces
+    the eager W and energy bit for bit."""
+    from aces4_amd import triples as tp
+    w = tp.triples(o_segs=(2, 3), v_segs=(4, 3, 5), seed=5)
+    tp.to_device(w)
+    tup = (0, 1, 2, 0, 1, 0)
+    n = int(np.prod(tp.tuple_dims(w, tup)))
+    wbuf = sg.DeviceArray(n)
+    parts = [sg.DeviceArray(n) for _ in range(5)]
+    out = sg.DeviceArray(1)
+    te = tp.TupleEnergy(w, tup, wbuf.ptr, out.ptr, parts=[p.ptr for p in parts], two_streams=True)
+    s = sg.Stream()
+    out.zero()
+    te.execute(s)
+    s.synchronize()
+    want_w, want_e = wbuf.download(), out.download()[0]
+    g = sg.Graph.capture(lambda st: te.execute(st), s)
+    for rep in range(3):
+        wbuf.zero()
+        out.zero()
+        s.synchronize()
+        g.launch(s)
+        s.synchronize()
+        assert np.array_equal(wbuf.download().view(np.uint64), want_w.view(np.uint64)), rep
+        assert out.download()[0] == want_e, rep
+    g.destroy()
+    te.destroy()
+    s.destroy()
