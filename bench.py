@@ -384,7 +384,8 @@ def graph_bench(reps=20):
     nw = int(np.prod(tp.tuple_dims(w3, t)))
     wbuf = sg.DeviceArray(nw)
     parts = [sg.DeviceArray(nw) for _ in range(5)]
-    te = tp.TupleEnergy(w3, t, wbuf.ptr, out.ptr, parts=[p.ptr for p in parts], tune=True)
+    te = tp.TupleEnergy(w3, t, wbuf.ptr, out.ptr, parts=[p.ptr for p in parts], tune=True,
+                        two_streams=True)
     compare("triples_tuple", lambda st: te.execute(st), te.flops)
     te.destroy()
     wbuf.free()
