@@ -297,3 +297,23 @@ def test_native_block_operations_fail_loudly_without_gpu():
 def test_stream_wait_event_rejects_a_null_event():
     rc = sg.lib().sipgpu_stream_wait_event(None, None)
     assert rc == -107
+
+
+def test_slice_plans_beyond_the_sip_rank():
+    """The reference module accepts any rank; canonical forms of up to eight
+    dims run, a slice that needs nine is refused with SIPGPU_ERANK."""
+    te = [3, 1, 4, 1, 5, 2, 1, 3, 2, 2]
+    se = [3, 1, 2, 1, 5, 2, 1, 1, 2, 1]
+    eb = [0, 0, 1, 0, 0, 0, 0, 2, 0, 1]
+    ext, st, base, count, span, tsize = sg.plan_slice(te, se, eb)
+    want = _slice_offsets(te, se, eb)
+    l = np.arange(count, dtype=np.int64)
+    off = np.full(count, base, dtype=np.int64)
+    for e, s in zip(ext, st):
+        off += (l % e) * s
+        l //= e
+    np.testing.assert_array_equal(off, want)
+    # nine dims of extent 2 in a 3^9 block, none mergeable: nine canonical dims
+    with pytest.raises(sg.SipGpuError) as ei:
+        sg.plan_slice([3] * 9, [2] * 9, [0] * 9)
+    assert ei.value.code == -102
