@@ -135,3 +135,26 @@ def test_native_contract_errors():
     with pytest.raises(sg.SipGpuError) as ei:
         sg.contract([1] * 7 + [-1] * 7, dL.ptr, [1] * 7, dR.ptr, [1] * 7, dD.ptr, [1] * 7)
     assert ei.value.code == -102
+
+
+def test_dropin_contract_beyond_the_sip_rank():
+    """The reference module's contraction takes ranks up to 32: a rank-8
+    result from a rank-7 and a rank-3 operand through the drop-in symbol,
+    within 1e-13 of the oracle."""
+    orc = Oracle()
+    # D(a,b,c,d,e,f,g,h) = L(a,b,k,c,d,e,f) R(g,k,h)
+    ext = dict(a=2, b=3, c=2, d=3, e=2, f=2, g=3, h=2, k=5)
+    dl, ll, rl = "abcdefgh", "abkcdef", "gkh"
+    code = {c: i + 1 for i, c in enumerate("abcdefghk")}
+    aces = [code[c] for c in dl + ll + rl]
+    ptrn, e = orc.get_contraction_ptrn(len(dl), len(ll), len(rl), aces)
+    assert e == 0
+    ld, rd, dd = [ext[c] for c in ll], [ext[c] for c in rl], [ext[c] for c in dl]
+    rng = np.random.default_rng(8)
+    L = rng.standard_normal(int(np.prod(ld)))
+    R = rng.standard_normal(int(np.prod(rd)))
+    want, e = orc.contract(ptrn, L, ld, R, rd, dd)
+    assert e == 0
+    got, e = sg.tensor_block_contract(ptrn, L, ld, R, rd, dd)
+    assert e == 0
+    assert rel_fro(got, want) <= 1e-13
