@@ -166,10 +166,18 @@ void gemm_f64_kernel(GemmArgs args) {
     double vb[TileLoader<BN, NT, kBK>::kItems];
 
     for (int t = blockIdx.x; t < args.ntiles; t += gridDim.x) {
+        // a single contraction split along K: segment ks of the tile
+        const bool split = !args.tiles && args.ksplit > 1;
+        const int ks = split ? t / (args.ntiles / args.ksplit) : 0;
+        const int tt = t - ks * (split ? args.ntiles / args.ksplit : 0);
         const GemmTile tile = args.tiles ? args.tiles[t]
-                                         : GemmTile{0, t % args.tilesM, t / args.tilesM, 0};
+                                         : GemmTile{0, tt % args.tilesM, tt / args.tilesM, 0};
         if (tile.task < 0) continue; // padding slot of the XCD schedule
-        const GemmTask task = args.tiles ? args.tasks[tile.task] : args.task0;
+        GemmTask task = args.tiles ? args.tasks[tile.task] : args.task0;
+        if (ks > 0) {
+            task.D = args.ws + (ks - 1) * args.ws_stride;
+            task.beta = 0;
+        }
         const int m0 = tile.mt * BM, n0 = tile.nt * BN;
         // rows / columns of this wave's sub-tile that lie inside the block
         const int mlive = __builtin_amdgcn_readfirstlane(task.M - (m0 + wm * TM));
@@ -182,7 +190,13 @@ void gemm_f64_kernel(GemmArgs args) {
             for (int j = 0; j < FN; ++j) acc[i][j] = f64x4{0.0, 0.0, 0.0, 0.0};
 
         for (int p = task.first; p < task.first + task.npairs; ++p) {
-            const GemmPair pr = args.tiles ? args.pairs[p] : args.pair0;
+            GemmPair pr = args.tiles ? args.pairs[p] : args.pair0;
+            if (split) { // this segment's k range: the tables from k0 on
+                const int k0 = ks * args.kseg;
+                pr.kL += k0;
+                pr.kR += k0;
+                pr.K = min(pr.K - k0, args.kseg);
+            }
             la.pair(pr.flags & 1, pr.L, args.tab + pr.kL, pr.K, args.tab + pr.mL, m0, task.M, tid);
             lb.pair(pr.flags & 2, pr.R, args.tab + pr.kR, pr.K, args.tab + pr.nR, n0, task.N, tid);
             la.lookup(0, tid);

@@ -14,6 +14,8 @@
 struct sipgpu_stream_s {
     hipStream_t s = nullptr;
     double *scratch = nullptr;
+    double *ws = nullptr; // K-segment partial results of split single contractions
+    size_t ws_bytes = 0;  // (grow-only)
 };
 
 namespace sipgpu {

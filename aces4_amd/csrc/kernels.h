@@ -67,6 +67,13 @@ struct GemmArgs {
     int tilesM;
     GemmTask task0;
     GemmPair pair0;
+    // single contraction split along K (ksplit > 1, 64 x 64 kernel only):
+    // tile t computes segment s = t / (ntiles / ksplit) of kseg k values;
+    // segment 0 goes to task0.D, segment s > 0 is assigned to ws + (s - 1) *
+    // ws_stride with task0's destination tables
+    int ksplit, kseg;
+    double *ws;
+    long long ws_stride;
 };
 
 // Tile configurations of the grouped FP64-MFMA kernel (BM x BN, waves, BK):

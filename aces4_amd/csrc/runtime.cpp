@@ -193,6 +193,8 @@ void make_task_pair(const ContractionShape &s, int kL, int mL, int kR, int nR, i
     }
 }
 
+constexpr int64_t kSplitCUs = 256; // MI355X compute units
+
 int pick_config(int64_t M, int64_t N) {
     // SIPGPU_GEMM_CONFIG=<n> pins the tile configuration (tuning runs).
     static const int forced = [] {
@@ -234,6 +236,43 @@ int contract_device(Context &c, sipgpu_stream_s &st, const int *ptrn, const doub
         task.npairs = 1;
         a.task0 = task;
         a.pair0 = pair;
+        // Too few tiles to fill the CUs and a long K: split K into segments
+        // that run side by side, segment s > 0 into the stream's workspace,
+        // then add the segments to D in order (a fixed summation order).
+        const int64_t K = pair.K;
+        int nseg = 1, kseg = (int)K;
+        if (cfg == kGemm64x64 && ntiles * 2 <= kSplitCUs && K >= 256) {
+            const int want = (int)std::min<int64_t>({4, kSplitCUs / ntiles, K / 128});
+            kseg = (int)(((K + want - 1) / want + 15) / 16 * 16);
+            nseg = (int)((K + kseg - 1) / kseg);
+        }
+        if (nseg > 1) {
+            const size_t need = (size_t)(nseg - 1) * (size_t)s.dsize * sizeof(double);
+            if (need > st.ws_bytes) {
+                if (sipgpu::rt::capturing(st.s)) {
+                    nseg = 1; // no allocation during capture: run unsplit
+                } else {
+                    if (st.ws) HIP_TRY(hipFree(st.ws));
+                    st.ws = nullptr;
+                    st.ws_bytes = 0;
+                    HIP_TRY(hipMalloc(&st.ws, need));
+                    st.ws_bytes = need;
+                }
+            }
+        }
+        if (nseg > 1) {
+            a.ksplit = nseg;
+            a.kseg = kseg;
+            a.ws = st.ws;
+            a.ws_stride = s.dsize;
+            a.ntiles = (int)ntiles * nseg;
+            HIP_TRY(launch_gemm(cfg, a, a.ntiles, st.s));
+            AddManyArgs am{};
+            am.n = nseg - 1;
+            for (int q = 0; q < am.n; ++q) am.p[q] = st.ws + (size_t)q * s.dsize;
+            HIP_TRY(launch_block_add_many(d, am, s.dsize, st.s));
+            return 0;
+        }
         HIP_TRY(launch_gemm(cfg, a, a.ntiles, st.s));
         return 0;
     }
@@ -1889,6 +1928,7 @@ int sipgpu_stream_destroy(sipgpu_stream_t s) {
     if (!s) return fail(SIPGPU_EARG, "null stream");
     hipError_t e = hipStreamSynchronize(s->s);
     if (s->scratch) (void)hipFree(s->scratch);
+    if (s->ws) (void)hipFree(s->ws);
     if (e == hipSuccess) e = hipStreamDestroy(s->s);
     delete s;
     if (e != hipSuccess) return hip_fail(e, "hipStreamDestroy");
