@@ -158,3 +158,31 @@ def test_dropin_contract_beyond_the_sip_rank():
     got, e = sg.tensor_block_contract(ptrn, L, ld, R, rd, dd)
     assert e == 0
     assert rel_fro(got, want) <= 1e-13
+
+
+@pytest.mark.parametrize("accumulate", [False, True])
+def test_single_contraction_split_along_k(accumulate):
+    """A lone contraction with one 64 x 64 tile and K = 1000 runs as four K
+    segments summed in order: within 1e-13 of the oracle, assign and
+    accumulate, and bitwise the same on every run."""
+    orc = Oracle()
+    rng = np.random.default_rng(31)
+    ptrn = [-1, 1, -1, 2]  # D(m,n) = sum_k L(k,m) R(k,n)
+    ld, rd, dd = [1000, 60], [1000, 50], [60, 50]
+    L = rng.standard_normal(60000)
+    R = rng.standard_normal(50000)
+    D0 = rng.standard_normal(3000)
+    want, e = orc.contract(ptrn, L, ld, R, rd, dd)
+    assert e == 0
+    if accumulate:
+        want = D0 + want
+    dl, dr = sg.DeviceArray.from_host(L), sg.DeviceArray.from_host(R)
+    outs = []
+    for _ in range(2):
+        dd_ = sg.DeviceArray.from_host(D0)
+        sg.contract(ptrn, dl.ptr, ld, dr.ptr, rd, dd_.ptr, dd, accumulate=accumulate)
+        sg.synchronize()
+        outs.append(dd_.download())
+        dd_.free()
+    assert rel_fro(outs[0], want) <= 1e-13
+    assert np.array_equal(outs[0].view(np.uint64), outs[1].view(np.uint64))
