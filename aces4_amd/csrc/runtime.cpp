@@ -252,11 +252,14 @@ int contract_device(Context &c, sipgpu_stream_s &st, const int *ptrn, const doub
                 if (sipgpu::rt::capturing(st.s)) {
                     nseg = 1; // no allocation during capture: run unsplit
                 } else {
+                    // at least 32 MiB, doubling: a stream of uneven blocks
+                    // reallocates a few times at most
+                    const size_t grow = std::max({need, 2 * st.ws_bytes, (size_t)32 << 20});
                     if (st.ws) HIP_TRY(hipFree(st.ws));
                     st.ws = nullptr;
                     st.ws_bytes = 0;
-                    HIP_TRY(hipMalloc(&st.ws, need));
-                    st.ws_bytes = need;
+                    HIP_TRY(hipMalloc(&st.ws, grow));
+                    st.ws_bytes = grow;
                 }
             }
         }
