@@ -186,3 +186,60 @@ def test_single_contraction_split_along_k(accumulate):
         dd_.free()
     assert rel_fro(outs[0], want) <= 1e-13
     assert np.array_equal(outs[0].view(np.uint64), outs[1].view(np.uint64))
+
+
+SPLIT_SHAPES = [
+    # (M, N, K): the smallest K that splits, one below it, a K whose rounded
+    # segment leaves a short last segment, ragged tile edges, and a grid of
+    # tiles near the split limit
+    (64, 64, 256), (64, 64, 255), (65, 33, 257), (200, 190, 390), (130, 70, 1001),
+    (700, 700, 600),
+]
+
+
+@pytest.mark.parametrize("mnk", SPLIT_SHAPES, ids=lambda s: "x".join(map(str, s)))
+@pytest.mark.parametrize("accumulate", [False, True])
+def test_split_along_k_edges(mnk, accumulate):
+    """Single contractions at the edges of the K split (segment count, a
+    rounded segment length that leaves a short last segment, ragged tiles):
+    within 1e-13 of the oracle, assign and accumulate."""
+    M, N, K = mnk
+    rng = np.random.default_rng(M * 7 + N * 3 + K)
+    ptrn = [-1, 1, -1, 2]
+    ld, rd, dd = [K, M], [K, N], [M, N]
+    L, R, D0 = rng.standard_normal(K * M), rng.standard_normal(K * N), rng.standard_normal(M * N)
+    print("split edges", mnk, accumulate, flush=True)
+    want, e = o.contract(ptrn, L, ld, R, rd, dd)
+    assert e == 0
+    if accumulate:
+        want = D0 + want
+    dl, dr, dd_ = (sg.DeviceArray.from_host(x) for x in (L, R, D0))
+    sg.contract(ptrn, dl.ptr, ld, dr.ptr, rd, dd_.ptr, dd, accumulate=accumulate)
+    sg.synchronize()
+    got = dd_.download()
+    for x in (dl, dr, dd_):
+        x.free()
+    assert rel_fro(got, want) <= TOL
+
+
+def test_split_along_k_two_summed_dims():
+    """The ring's pattern R(a,i,b,j) = sum_{k,c} W(k,b,c,j) T(a,i,c,k) on 8^4
+    result blocks with k = c = 20 (K = 400 over two dims, one 64 x 64 tile):
+    within 1e-13 of the oracle."""
+    A = I = B = J = 8
+    Kd = C = 20
+    ptrn, e = sg.get_contraction_ptrn(4, 4, 4, [1, 2, 3, 4, 5, 3, 6, 4, 1, 2, 6, 5])
+    assert e == 0
+    ld, rd, dd = [Kd, B, C, J], [A, I, C, Kd], [A, I, B, J]
+    rng = np.random.default_rng(77)
+    L, R = rng.standard_normal(int(np.prod(ld))), rng.standard_normal(int(np.prod(rd)))
+    want, e = o.contract(ptrn, L, ld, R, rd, dd)
+    assert e == 0
+    dl, dr = sg.DeviceArray.from_host(L), sg.DeviceArray.from_host(R)
+    dd_ = sg.DeviceArray.from_host(np.zeros(int(np.prod(dd))))
+    sg.contract(ptrn, dl.ptr, ld, dr.ptr, rd, dd_.ptr, dd)
+    sg.synchronize()
+    got = dd_.download()
+    for x in (dl, dr, dd_):
+        x.free()
+    assert rel_fro(got, want) <= TOL
