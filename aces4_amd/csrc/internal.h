@@ -16,6 +16,9 @@ struct sipgpu_stream_s {
     double *scratch = nullptr;
     double *ws = nullptr; // K-segment partial results of split single contractions
     size_t ws_bytes = 0;  // (grow-only)
+    // Earlier, smaller workspaces: a graph captured on this stream may still
+    // address one, so they are freed only with the stream.
+    std::vector<double *> retired;
 };
 
 namespace sipgpu {

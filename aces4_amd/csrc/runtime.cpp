@@ -111,6 +111,9 @@ Context &ctx() {
     return *c;
 }
 
+// the stream scratch holds the partial sums of dots, energies and norms
+static_assert(kNormBlocks <= std::max(kDotBlocks, kEnergyBlocks), "scratch too small for norm2");
+
 int ensure_stream_scratch(sipgpu_stream_s &s) {
     if (s.scratch) return 0;
     HIP_TRY(hipMalloc(&s.scratch, std::max(kDotBlocks, kEnergyBlocks) * sizeof(double)));
@@ -253,9 +256,11 @@ int contract_device(Context &c, sipgpu_stream_s &st, const int *ptrn, const doub
                     nseg = 1; // no allocation during capture: run unsplit
                 } else {
                     // at least 32 MiB, doubling: a stream of uneven blocks
-                    // reallocates a few times at most
+                    // reallocates a few times at most.  The old workspace is
+                    // kept until the stream is destroyed: graphs captured on
+                    // this stream address it.
                     const size_t grow = std::max({need, 2 * st.ws_bytes, (size_t)32 << 20});
-                    if (st.ws) HIP_TRY(hipFree(st.ws));
+                    if (st.ws) st.retired.push_back(st.ws);
                     st.ws = nullptr;
                     st.ws_bytes = 0;
                     HIP_TRY(hipMalloc(&st.ws, grow));
@@ -1932,6 +1937,7 @@ int sipgpu_stream_destroy(sipgpu_stream_t s) {
     hipError_t e = hipStreamSynchronize(s->s);
     if (s->scratch) (void)hipFree(s->scratch);
     if (s->ws) (void)hipFree(s->ws);
+    for (double *w : s->retired) (void)hipFree(w);
     if (e == hipSuccess) e = hipStreamDestroy(s->s);
     delete s;
     if (e != hipSuccess) return hip_fail(e, "hipStreamDestroy");

@@ -476,7 +476,10 @@ int sipgpu_server_destroy(sipgpu_server_t server);
  * replays.  Batches must be planned and single-contraction shapes run once
  * before capture: planning during capture fails with SIPGPU_ECAPTURE and
  * leaves the capture intact.  The captured work clears memory with kernels
- * only (no memset nodes). */
+ * only (no memset nodes).  Single contractions and energy sums in a graph use
+ * the capture stream's scratch and K-split workspace: destroy the graph before
+ * that stream, and do not replay it on another stream while single
+ * contractions run on the capture stream. */
 typedef struct sipgpu_graph_s *sipgpu_graph_t;
 int sipgpu_graph_begin(sipgpu_stream_t s);
 int sipgpu_graph_end(sipgpu_stream_t s, sipgpu_graph_t *graph);
