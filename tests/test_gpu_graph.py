@@ -164,3 +164,39 @@ def test_graph_of_two_stream_triples_tuple():
     g.destroy()
     te.destroy()
     s.destroy()
+
+
+def test_graph_of_split_single_contraction():
+    """A single contraction that runs split along K (one 64 x 64 tile, K =
+    1000: four segments, three in the stream's workspace, then summed in
+    order) captured and replayed three times, with other single contractions
+    on the same stream between the replays: every replay bitwise equal to the
+    eager result, and the graph holds the split GEMM and the summing pass."""
+    s = sg.Stream()
+    rng = np.random.default_rng(41)
+    ptrn = [-1, 1, -1, 2]
+    ld, rd, dd = [1000, 60], [1000, 50], [60, 50]
+    L, R = rng.standard_normal(60000), rng.standard_normal(50000)
+    D0 = rng.standard_normal(3000)
+    dl, dr, d = (sg.DeviceArray.from_host(x) for x in (L, R, D0))
+    other = sg.DeviceArray(3000)
+    sg.contract(ptrn, dl.ptr, ld, dr.ptr, rd, d.ptr, dd, accumulate=True, stream=s)
+    s.synchronize()
+    eager = d.download()
+    sg.Graph.begin(s)
+    sg.contract(ptrn, dl.ptr, ld, dr.ptr, rd, d.ptr, dd, accumulate=True, stream=s)
+    g = sg.Graph.end(s)
+    assert g.nodes >= 2
+    for rep in range(3):
+        print("split graph replay", rep, flush=True)
+        d.upload(D0)
+        # eager split work on the capture stream between replays (it shares
+        # the workspace; the stream orders it)
+        sg.contract(ptrn, dl.ptr, ld, dr.ptr, rd, other.ptr, dd, stream=s)
+        g.launch(s)
+        s.synchronize()
+        assert np.array_equal(d.download().view(np.uint64), eager.view(np.uint64)), rep
+    g.destroy()
+    for x in (dl, dr, d, other):
+        x.free()
+    s.destroy()
