@@ -63,6 +63,8 @@ using sipgpu::rt::BatchImage;
 using sipgpu::rt::fail;
 using sipgpu::rt::hip_fail;
 double g_last_clock_mhz = 0.0; // shader clock seen by the last MFMA calibration
+// Compute units of the library's device (256 on MI355X), read at init.
+int64_t g_device_cus = 256;
 
 #define HIP_TRY(expr)                                                                     \
     do {                                                                                  \
@@ -126,6 +128,9 @@ int init_locked(Context &c, int device) {
     HIP_TRY(hipGetDeviceCount(&n));
     if (device < 0 || device >= n) return fail(SIPGPU_EARG, "no such device");
     HIP_TRY(hipSetDevice(device));
+    int cus = 0;
+    HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device));
+    if (cus > 0) g_device_cus = cus;
     HIP_TRY(hipStreamCreateWithFlags(&c.stream.s, hipStreamNonBlocking));
     int rc = ensure_stream_scratch(c.stream);
     if (rc) return rc;
@@ -196,8 +201,6 @@ void make_task_pair(const ContractionShape &s, int kL, int mL, int kR, int nR, i
     }
 }
 
-constexpr int64_t kSplitCUs = 256; // MI355X compute units
-
 int pick_config(int64_t M, int64_t N) {
     // SIPGPU_GEMM_CONFIG=<n> pins the tile configuration (tuning runs).
     static const int forced = [] {
@@ -205,8 +208,9 @@ int pick_config(int64_t M, int64_t N) {
         return e ? std::atoi(e) : -1;
     }();
     if (forced >= 0 && forced < kGemmConfigs) return forced;
-    // 64x64 tiles at four waves per SIMD measured fastest on every block shape
-    // of the benchmark terms (the larger tiles run at one or two waves).
+    // Config 0 (64x64 tile, four waves of 32x32, no occupancy hint) measured
+    // fastest on every block shape of the benchmark terms; the larger tiles
+    // fit fewer workgroups per CU.
     (void)M;
     (void)N;
     return kGemm64x64;
@@ -244,26 +248,37 @@ int contract_device(Context &c, sipgpu_stream_s &st, const int *ptrn, const doub
         // then add the segments to D in order (a fixed summation order).
         const int64_t K = pair.K;
         int nseg = 1, kseg = (int)K;
-        if (cfg == kGemm64x64 && ntiles * 2 <= kSplitCUs && K >= 256) {
-            const int want = (int)std::min<int64_t>({4, kSplitCUs / ntiles, K / 128});
+        if (cfg == kGemm64x64 && ntiles * 2 <= g_device_cus && K >= 256) {
+            const int want = (int)std::min<int64_t>({4, g_device_cus / ntiles, K / 128});
             kseg = (int)(((K + want - 1) / want + 15) / 16 * 16);
             nseg = (int)((K + kseg - 1) / kseg);
         }
         if (nseg > 1) {
             const size_t need = (size_t)(nseg - 1) * (size_t)s.dsize * sizeof(double);
             if (need > st.ws_bytes) {
-                if (sipgpu::rt::capturing(st.s)) {
-                    nseg = 1; // no allocation during capture: run unsplit
+                // No allocation while this thread captures (on any stream)
+                // or this stream is captured.  Running unsplit instead would
+                // change the summation order, so a replay would differ in
+                // bits from the eager result: refuse, and the caller runs the
+                // contraction once on this stream before capturing.
+                if (sipgpu::rt::thread_captures() > 0 || sipgpu::rt::capturing(st.s))
+                    return fail(SIPGPU_ECAPTURE,
+                                "a contraction whose split workspace must grow (run it once "
+                                "on this stream before capturing)");
+                // at least 32 MiB, doubling: a stream of uneven blocks
+                // reallocates a few times at most.  The old workspace is
+                // kept until the stream is destroyed: graphs captured on
+                // this stream address it.
+                const size_t grow = std::max({need, 2 * st.ws_bytes, (size_t)32 << 20});
+                double *fresh = nullptr;
+                if (hipMalloc(&fresh, grow) != hipSuccess) {
+                    // HBM taken (by a block cache, say): the unsplit
+                    // contraction needs no workspace, so run it that way
+                    (void)hipGetLastError();
+                    nseg = 1;
                 } else {
-                    // at least 32 MiB, doubling: a stream of uneven blocks
-                    // reallocates a few times at most.  The old workspace is
-                    // kept until the stream is destroyed: graphs captured on
-                    // this stream address it.
-                    const size_t grow = std::max({need, 2 * st.ws_bytes, (size_t)32 << 20});
                     if (st.ws) st.retired.push_back(st.ws);
-                    st.ws = nullptr;
-                    st.ws_bytes = 0;
-                    HIP_TRY(hipMalloc(&st.ws, grow));
+                    st.ws = fresh;
                     st.ws_bytes = grow;
                 }
             }

@@ -136,6 +136,16 @@ int make_event(hipEvent_t *e) {
     return 0;
 }
 
+// Cache calls are host-side bookkeeping around uploads, spills, fences and
+// synchronisation: none of it may be recorded into a graph (a replay would
+// also reuse arena offsets that a later eviction hands to another block).
+int refuse_capture(hipStream_t s) {
+    if (int rc = sipgpu::rt::refuse_in_capture("a block cache call")) return rc;
+    if (s && sipgpu::rt::capturing(s))
+        return fail(SIPGPU_ECAPTURE, "a block cache call is not allowed on a capturing stream");
+    return 0;
+}
+
 // Write back (if needed) and drop the device copy of an unpinned block.
 // The arena range is reusable once the returned fence (if any) completes.
 int evict_locked(sipgpu_cache_s &c, unsigned long long key, CBlock &b) {
@@ -253,7 +263,9 @@ int acquire_locked(sipgpu_cache_s &c, hipStream_t stream, unsigned long long key
             }
         } else if (!b.initialised) {
             if (b.init == SIPGPU_CACHE_ZERO) {
-                if (need_value) SIPGPU_HIP_TRY(hipMemsetAsync(d, 0, bytes, stream));
+                if (need_value)
+                    SIPGPU_HIP_TRY(sipgpu::launch_block_op(sipgpu::kOpFill, d, nullptr,
+                                                           (long long)b.elems, 0.0, stream));
             } else if (b.init == SIPGPU_CACHE_UNIFORM || b.init == SIPGPU_CACHE_NORMAL) {
                 SIPGPU_HIP_TRY(sipgpu::launch_fill_random(
                     d, (long long)b.elems, b.seed, b.stream,
@@ -318,6 +330,7 @@ int sipgpu_cache_create(size_t budget_bytes, sipgpu_cache_t *cache) {
 }
 
 int sipgpu_cache_destroy(sipgpu_cache_t c) {
+    if (int rc = refuse_capture(nullptr)) return rc;
     if (!c) return fail(SIPGPU_EARG, "null cache");
     SIPGPU_HIP_TRY(hipDeviceSynchronize());
     for (auto &kv : c->blocks) {
@@ -334,6 +347,7 @@ int sipgpu_cache_destroy(sipgpu_cache_t c) {
 int sipgpu_cache_insert(sipgpu_cache_t c, unsigned long long key, size_t elems,
                         const double *host, int init, unsigned long long seed,
                         unsigned long long stream, double a, double b) {
+    if (int rc = refuse_capture(nullptr)) return rc;
     if (!c || elems == 0) return fail(SIPGPU_EARG, "cache: bad arguments");
     if (init < SIPGPU_CACHE_UNDEFINED || init > SIPGPU_CACHE_NORMAL)
         return fail(SIPGPU_EARG, "cache: unknown initialisation");
@@ -357,6 +371,7 @@ int sipgpu_cache_insert(sipgpu_cache_t c, unsigned long long key, size_t elems,
 
 int sipgpu_cache_acquire(sipgpu_cache_t c, sipgpu_stream_t s, unsigned long long key, int mode,
                          double **dev) {
+    if (int rc = refuse_capture(stream_of(s))) return rc;
     if (!c || !dev || mode < SIPGPU_CACHE_READ || mode > SIPGPU_CACHE_READWRITE)
         return fail(SIPGPU_EARG, "cache: bad arguments");
     std::lock_guard<std::mutex> g(c->mu);
@@ -364,12 +379,14 @@ int sipgpu_cache_acquire(sipgpu_cache_t c, sipgpu_stream_t s, unsigned long long
 }
 
 int sipgpu_cache_release(sipgpu_cache_t c, sipgpu_stream_t s, unsigned long long key) {
+    if (int rc = refuse_capture(stream_of(s))) return rc;
     if (!c) return fail(SIPGPU_EARG, "null cache");
     std::lock_guard<std::mutex> g(c->mu);
     return release_locked(*c, stream_of(s), key);
 }
 
 int sipgpu_cache_read(sipgpu_cache_t c, unsigned long long key, double *out) {
+    if (int rc = refuse_capture(nullptr)) return rc;
     if (!c || !out) return fail(SIPGPU_EARG, "cache: bad arguments");
     std::lock_guard<std::mutex> g(c->mu);
     auto it = c->blocks.find(key);
@@ -388,6 +405,7 @@ int sipgpu_cache_read(sipgpu_cache_t c, unsigned long long key, double *out) {
 }
 
 int sipgpu_cache_flush(sipgpu_cache_t c, unsigned long long key, int all) {
+    if (int rc = refuse_capture(nullptr)) return rc;
     if (!c) return fail(SIPGPU_EARG, "null cache");
     std::lock_guard<std::mutex> g(c->mu);
     SIPGPU_HIP_TRY(hipDeviceSynchronize());
@@ -410,6 +428,7 @@ int sipgpu_cache_flush(sipgpu_cache_t c, unsigned long long key, int all) {
 }
 
 int sipgpu_cache_evict(sipgpu_cache_t c, unsigned long long key) {
+    if (int rc = refuse_capture(nullptr)) return rc;
     if (!c) return fail(SIPGPU_EARG, "null cache");
     std::lock_guard<std::mutex> g(c->mu);
     auto it = c->blocks.find(key);
@@ -420,6 +439,7 @@ int sipgpu_cache_evict(sipgpu_cache_t c, unsigned long long key) {
 }
 
 int sipgpu_cache_remove(sipgpu_cache_t c, unsigned long long key) {
+    if (int rc = refuse_capture(nullptr)) return rc;
     if (!c) return fail(SIPGPU_EARG, "null cache");
     if (int rc = sipgpu::rt::refuse_in_capture("removing a cached block")) return rc;
     std::lock_guard<std::mutex> g(c->mu);
@@ -446,6 +466,7 @@ int sipgpu_cache_scope_add(sipgpu_cache_t c, unsigned long long key) {
 }
 
 int sipgpu_cache_scope_leave(sipgpu_cache_t c) {
+    if (int rc = refuse_capture(nullptr)) return rc;
     if (!c) return fail(SIPGPU_EARG, "null cache");
     if (int rc = sipgpu::rt::refuse_in_capture("leaving a cache scope")) return rc;
     std::lock_guard<std::mutex> g(c->mu);
@@ -474,6 +495,7 @@ int sipgpu_cache_set_persistent(sipgpu_cache_t c, unsigned long long key, int pe
 }
 
 int sipgpu_cache_end_program(sipgpu_cache_t c, unsigned long long *kept) {
+    if (int rc = refuse_capture(nullptr)) return rc;
     if (!c) return fail(SIPGPU_EARG, "null cache");
     if (int rc = sipgpu::rt::refuse_in_capture("ending a program")) return rc;
     std::lock_guard<std::mutex> g(c->mu);

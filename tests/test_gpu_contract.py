@@ -191,9 +191,10 @@ def test_single_contraction_split_along_k(accumulate):
 SPLIT_SHAPES = [
     # (M, N, K): the smallest K that splits, one below it, a K whose rounded
     # segment leaves a short last segment, ragged tile edges, and a grid of
-    # tiles near the split limit
+    # tiles near the split limit; (63, 61, 1000): four segments of an odd
+    # block size, so the workspace segments are not 16-byte aligned
     (64, 64, 256), (64, 64, 255), (65, 33, 257), (200, 190, 390), (130, 70, 1001),
-    (700, 700, 600),
+    (700, 700, 600), (63, 61, 1000),
 ]
 
 

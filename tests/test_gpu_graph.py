@@ -200,3 +200,92 @@ def test_graph_of_split_single_contraction():
     for x in (dl, dr, d, other):
         x.free()
     s.destroy()
+
+
+def test_split_workspace_growth_refused_during_capture():
+    """While this thread captures stream A, a K-split contraction on a fresh
+    stream B (whose workspace would have to grow) is refused with
+    SIPGPU_ECAPTURE instead of allocating or running unsplit; A's capture
+    stays usable, and after it the same contraction on B runs split and
+    matches the oracle.  A shape warmed on A and captured on a fresh stream
+    is refused the same way (a replay could not reproduce the eager bits)."""
+    from checker import Oracle
+    from cases import rel_fro
+    a, b = sg.Stream(), sg.Stream()
+    rng = np.random.default_rng(5)
+    ptrn = [-1, 1, -1, 2]
+    ld, rd, dd = [1000, 60], [1000, 50], [60, 50]
+    L, R = rng.standard_normal(60000), rng.standard_normal(50000)
+    dl, dr = sg.DeviceArray.from_host(L), sg.DeviceArray.from_host(R)
+    d = sg.DeviceArray.from_host(np.zeros(3000))
+    w = wl.ring(o_segs=(5,), v_segs=(6, 4))
+    bt = wl.batch_for(w)
+    sg.contract(ptrn, dl.ptr, ld, dr.ptr, rd, d.ptr, dd, stream=a)  # warm the shape on A
+    a.synchronize()
+    sg.Graph.begin(a)
+    bt.execute(a)
+    with pytest.raises(sg.SipGpuError) as ei:
+        sg.contract(ptrn, dl.ptr, ld, dr.ptr, rd, d.ptr, dd, stream=b)
+    assert ei.value.code == -111
+    g = sg.Graph.end(a)
+    assert g.nodes >= 1
+    w["D"].dev.zero()
+    g.launch(a)
+    a.synchronize()
+    ref = w["D"].dev.download()
+    w["D"].dev.zero()
+    bt.execute(a)
+    a.synchronize()
+    assert np.array_equal(w["D"].dev.download(), ref)
+    # capture on a fresh stream C of a shape warmed on A only
+    c = sg.Stream()
+    sg.Graph.begin(c)
+    bt.execute(c)
+    with pytest.raises(sg.SipGpuError) as ei:
+        sg.contract(ptrn, dl.ptr, ld, dr.ptr, rd, d.ptr, dd, stream=c)
+    assert ei.value.code == -111
+    sg.Graph.end(c).destroy()
+    # eager on B: grows B's workspace, runs split, matches the oracle
+    sg.contract(ptrn, dl.ptr, ld, dr.ptr, rd, d.ptr, dd, stream=b)
+    b.synchronize()
+    want, e = Oracle().contract(ptrn, L, ld, R, rd, dd)
+    assert e == 0 and rel_fro(d.download(), want) <= 1e-13
+    g.destroy()
+    bt.destroy()
+    for x in (dl, dr, d):
+        x.free()
+    for s in (a, b, c):
+        s.destroy()
+
+
+def test_cache_calls_refused_during_capture():
+    """Acquire, release, read and flush of the block cache inside a capture
+    fail with SIPGPU_ECAPTURE (an upload, memset or fence recorded into a
+    graph would bake in an arena offset a later eviction reuses); the
+    capture stays usable and the cache works after it.  A ZERO block is
+    cleared by a kernel, not a memset."""
+    s = sg.Stream()
+    cache = sg.Cache(1 << 20)
+    cache.insert(1, 1000, init="zero")
+    w = wl.ring(o_segs=(5,), v_segs=(6, 4))
+    bt = wl.batch_for(w)
+    sg.Graph.begin(s)
+    bt.execute(s)
+    for call in (lambda: cache.acquire(1, "rw", s), lambda: cache.acquire(1, "r"),
+                 lambda: cache.read(1, 1000), lambda: cache.flush()):
+        with pytest.raises(sg.SipGpuError) as ei:
+            call()
+        assert ei.value.code == -111
+    g = sg.Graph.end(s)
+    assert g.nodes >= 1
+    g.launch(s)
+    s.synchronize()
+    dev = cache.acquire(1, "r", s)
+    assert dev
+    cache.release(1, s)
+    s.synchronize()
+    assert np.array_equal(cache.read(1, 1000), np.zeros(1000))
+    g.destroy()
+    bt.destroy()
+    cache.destroy()
+    s.destroy()
