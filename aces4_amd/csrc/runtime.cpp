@@ -113,6 +113,27 @@ Context &ctx() {
     return *c;
 }
 
+// Device memory of sipgpu_contract_noalloc: the K-segment workspace of a
+// split single contraction, then the contract-to-scalar partials.  One copy
+// per device, held until the process ends.  On the host `scratch` names a
+// stand-in, not device memory: init resolves each device's address with
+// hipGetSymbolAddress, and host code only ever passes that address on.
+// A split contraction has at most g_device_cus / 2 tiles of 64 x 64 and
+// four K segments, so its three workspace segments hold at most
+// 3 * 128 * 4096 doubles (12 MiB) on 256 CUs.
+constexpr size_t kNoAllocWsDoubles = (size_t)2 << 20; // 16 MiB
+constexpr size_t kNoAllocPartDoubles = (size_t)std::max(kDotBlocks, kEnergyBlocks);
+static __device__ __align__(256) double scratch[kNoAllocWsDoubles + kNoAllocPartDoubles];
+constexpr int kMaxDevices = 64;
+double *g_scratch_dev[kMaxDevices] = {}; // hipGetSymbolAddress of `scratch`, per device
+
+// Workspace and partials a contraction uses instead of its stream's own.
+struct FixedScratch {
+    double *ws;      // K segments 1.. of a split contraction
+    size_t ws_bytes;
+    double *part;    // contract-to-scalar partials (kDotBlocks doubles)
+};
+
 // the stream scratch holds the partial sums of dots, energies and norms
 static_assert(kNormBlocks <= std::max(kDotBlocks, kEnergyBlocks), "scratch too small for norm2");
 
@@ -131,6 +152,11 @@ int init_locked(Context &c, int device) {
     int cus = 0;
     HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device));
     if (cus > 0) g_device_cus = cus;
+    if (device < kMaxDevices && !g_scratch_dev[device]) {
+        void *p = nullptr;
+        HIP_TRY(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        g_scratch_dev[device] = static_cast<double *>(p);
+    }
     HIP_TRY(hipStreamCreateWithFlags(&c.stream.s, hipStreamNonBlocking));
     int rc = ensure_stream_scratch(c.stream);
     if (rc) return rc;
@@ -158,10 +184,13 @@ std::string shape_key(const int *ptrn, int lr, const int *ld, int rr, const int 
 // Looks up or builds the device tables of a contraction shape.  Returns the
 // reference's ierr (1, -1) or a SIPGPU_E* code.
 int get_shape(Context &c, const int *ptrn, int lr, const int *ld, int rr, const int *rd,
-              int dr, const int *dd, DeviceShape **out) {
+              int dr, const int *dd, DeviceShape **out, bool may_plan = true) {
     const std::string key = shape_key(ptrn, lr, ld, rr, rd, dr, dd);
     auto it = c.shapes.find(key);
     if (it != c.shapes.end()) { *out = it->second.get(); return 0; }
+    if (!may_plan)
+        return fail(SIPGPU_ENOTFOUND, "no-allocation contraction of a shape that is not planned "
+                                      "(run sipgpu_contract with it once)");
     // a new shape uploads its tables: run it once before capturing
     if (int rc = sipgpu::rt::refuse_in_capture("a contraction of a new shape")) return rc;
     auto ds = std::make_unique<DeviceShape>();
@@ -217,11 +246,13 @@ int pick_config(int64_t M, int64_t N) {
 }
 
 // One contraction on device pointers (no locking; caller holds the lock).
+// `fx` (may be null) replaces the stream's workspace and partials; with it
+// the call allocates nothing and refuses shapes it would have to plan.
 int contract_device(Context &c, sipgpu_stream_s &st, const int *ptrn, const double *l, int lr,
                     const int *ld, const double *r, int rr, const int *rd, double *d, int dr,
-                    const int *dd, int accumulate) {
+                    const int *dd, int accumulate, const FixedScratch *fx = nullptr) {
     DeviceShape *ds = nullptr;
-    int rc = get_shape(c, ptrn, lr, ld, rr, rd, dr, dd, &ds);
+    int rc = get_shape(c, ptrn, lr, ld, rr, rd, dr, dd, &ds, fx == nullptr);
     if (rc) return rc;
     const ContractionShape &s = ds->shape;
     if (s.dsize == 0) return 0;
@@ -255,7 +286,13 @@ int contract_device(Context &c, sipgpu_stream_s &st, const int *ptrn, const doub
         }
         if (nseg > 1) {
             const size_t need = (size_t)(nseg - 1) * (size_t)s.dsize * sizeof(double);
-            if (need > st.ws_bytes) {
+            if (fx) {
+                if (need > fx->ws_bytes)
+                    return fail(SIPGPU_ESIZE, "no-allocation contraction: its K segments need " +
+                                                  std::to_string(need) + " bytes, more than the " +
+                                                  std::to_string(fx->ws_bytes) +
+                                                  " of the static scratch");
+            } else if (need > st.ws_bytes) {
                 // No allocation while this thread captures (on any stream)
                 // or this stream is captured.  Running unsplit instead would
                 // change the summation order, so a replay would differ in
@@ -284,15 +321,16 @@ int contract_device(Context &c, sipgpu_stream_s &st, const int *ptrn, const doub
             }
         }
         if (nseg > 1) {
+            double *ws = fx ? fx->ws : st.ws;
             a.ksplit = nseg;
             a.kseg = kseg;
-            a.ws = st.ws;
+            a.ws = ws;
             a.ws_stride = s.dsize;
             a.ntiles = (int)ntiles * nseg;
             HIP_TRY(launch_gemm(cfg, a, a.ntiles, st.s));
             AddManyArgs am{};
             am.n = nseg - 1;
-            for (int q = 0; q < am.n; ++q) am.p[q] = st.ws + (size_t)q * s.dsize;
+            for (int q = 0; q < am.n; ++q) am.p[q] = ws + (size_t)q * s.dsize;
             HIP_TRY(launch_block_add_many(d, am, s.dsize, st.s));
             return 0;
         }
@@ -301,9 +339,12 @@ int contract_device(Context &c, sipgpu_stream_s &st, const int *ptrn, const doub
     }
     if (accumulate) return fail(SIPGPU_EARG, "accumulate is only defined for block results");
     if (dr == 0 && lr > 0 && rr > 0) {
-        rc = ensure_stream_scratch(st);
-        if (rc) return rc;
-        HIP_TRY(launch_dot(d, l, tab + ds->kL, r, tab + ds->kR, (int)s.K, st.scratch, st.s));
+        if (!fx) {
+            rc = ensure_stream_scratch(st);
+            if (rc) return rc;
+        }
+        HIP_TRY(launch_dot(d, l, tab + ds->kL, r, tab + ds->kR, (int)s.K,
+                           fx ? fx->part : st.scratch, st.s));
     } else if (dr > 0 && lr > 0 && rr == 0) {
         HIP_TRY(launch_scale_scatter(d, tab + ds->mD, l, tab + ds->mL, r, (int)s.M, st.s));
     } else if (dr > 0 && lr == 0 && rr > 0) {
@@ -2048,6 +2089,30 @@ int sipgpu_contract(sipgpu_stream_t s, const int *ptrn, const double *l, int lra
     if (rc) return rc;
     rc = contract_device(c, resolve(c, s), ptrn, l, lrank, ldims, r, rrank, rdims, d, drank,
                          ddims, accumulate);
+    if (rc == 1 || rc == -1) return fail(SIPGPU_EPATTERN, "contraction pattern rejected");
+    return rc;
+}
+
+int sipgpu_contract_noalloc(sipgpu_stream_t s, const int *ptrn, const double *l, int lrank,
+                            const int *ldims, const double *r, int rrank, const int *rdims,
+                            double *d, int drank, const int *ddims, int accumulate) {
+    if (!l || !r || !d || (lrank + rrank > 0 && !ptrn)) return fail(SIPGPU_EARG, "null argument");
+    if (lrank < 0 || lrank > SIPGPU_MAX_RANK || rrank < 0 || rrank > SIPGPU_MAX_RANK ||
+        drank < 0 || drank > SIPGPU_MAX_RANK)
+        return fail(SIPGPU_ERANK, "rank outside 0..6");
+    Context &c = ctx();
+    std::lock_guard<std::mutex> g(c.mu);
+    // initialisation creates the library's stream and scratch: set-up, done
+    // by sipgpu_init (or any other call) beforehand
+    if (!c.ready)
+        return fail(SIPGPU_EARG, "no-allocation contraction before the library is initialised "
+                                 "(call sipgpu_init)");
+    if (c.device >= kMaxDevices || !g_scratch_dev[c.device])
+        return fail(SIPGPU_EDEVICE, "no static scratch on this device");
+    double *base = g_scratch_dev[c.device];
+    const FixedScratch fx{base, kNoAllocWsDoubles * sizeof(double), base + kNoAllocWsDoubles};
+    int rc = contract_device(c, resolve(c, s), ptrn, l, lrank, ldims, r, rrank, rdims, d, drank,
+                             ddims, accumulate, &fx);
     if (rc == 1 || rc == -1) return fail(SIPGPU_EPATTERN, "contraction pattern rejected");
     return rc;
 }

@@ -23,7 +23,7 @@ ERRORS = {
     -101: "SIPGPU_EPATTERN", -102: "SIPGPU_ERANK", -103: "SIPGPU_EPERM",
     -104: "SIPGPU_ESIZE", -105: "SIPGPU_EDEVICE", -106: "SIPGPU_ENOMEM",
     -107: "SIPGPU_EARG", -108: "SIPGPU_EHAZARD", -109: "SIPGPU_EBUDGET",
-    -110: "SIPGPU_ENOTFOUND",
+    -110: "SIPGPU_ENOTFOUND", -111: "SIPGPU_ECAPTURE",
 }
 
 _ip = ctypes.POINTER(ctypes.c_int)
@@ -56,7 +56,7 @@ EXPORTED = [
     "sipgpu_cache_release", "sipgpu_cache_read", "sipgpu_cache_flush", "sipgpu_cache_evict",
     "sipgpu_cache_get_stats", "sipgpu_server_create", "sipgpu_server_submit",
     "sipgpu_server_wait", "sipgpu_server_replay", "sipgpu_server_get_stats",
-    "sipgpu_server_destroy",
+    "sipgpu_server_destroy", "sipgpu_contract_noalloc",
 ]
 
 
@@ -143,6 +143,8 @@ def lib():
         L.sipgpu_event_elapsed_ms.argtypes = [_vp, _vp, ctypes.POINTER(ctypes.c_float)]
         L.sipgpu_permute.argtypes = [_vp, ctypes.c_int, _ip, _ip, _vp, _vp]
         L.sipgpu_contract.argtypes = [_vp, _ip, _vp, ctypes.c_int, _ip, _vp, ctypes.c_int, _ip,
+                                      _vp, ctypes.c_int, _ip, ctypes.c_int]
+        L.sipgpu_contract_noalloc.argtypes = [_vp, _ip, _vp, ctypes.c_int, _ip, _vp, ctypes.c_int, _ip,
                                       _vp, ctypes.c_int, _ip, ctypes.c_int]
         L.sipgpu_batch_plan.argtypes = [ctypes.POINTER(Contraction), ctypes.c_int,
                                         ctypes.POINTER(_vp)]
@@ -702,6 +704,16 @@ def contract(ptrn, l, ldims, r, rdims, d, ddims, accumulate=False, stream=None):
     _check(lib().sipgpu_contract(stream.handle if stream else None, _iptr(p), l, len(ldims),
                                  _iptr(ld), r, len(rdims), _iptr(rd), d, len(ddims), _iptr(dd),
                                  1 if accumulate else 0))
+
+
+def contract_noalloc(ptrn, l, ldims, r, rdims, d, ddims, accumulate=False, stream=None):
+    """sipgpu_contract_noalloc: contract() with the library's static scratch
+    (no device allocation; the shape must have run through contract() once)."""
+    p = _ints(ptrn)
+    ld, rd, dd = _ints(ldims), _ints(rdims), _ints(ddims)
+    _check(lib().sipgpu_contract_noalloc(stream.handle if stream else None, _iptr(p), l,
+                                         len(ldims), _iptr(ld), r, len(rdims), _iptr(rd), d,
+                                         len(ddims), _iptr(dd), 1 if accumulate else 0))
 
 
 def make_contraction(ptrn, l, ldims, r, rdims, d, ddims, accumulate=False):

@@ -186,6 +186,22 @@ int sipgpu_contract(sipgpu_stream_t s, const int *ptrn, const double *l,
                     const int *rdims, double *d, int drank, const int *ddims,
                     int accumulate);
 
+/* sipgpu_contract that allocates nothing.  No counterpart in the reference:
+ * it is for callers that cannot allocate device memory (code under graph
+ * capture, tight serving loops).  Same arguments and results, bit for bit;
+ * the K-segment workspace of a split contraction and the contract-to-scalar
+ * partials live in a static device array of 16 MiB + 32 KiB per device,
+ * held until the process ends.  Preconditions: the library is initialised
+ * (SIPGPU_EARG otherwise) and the shape (pattern and extents) has been run
+ * once through sipgpu_contract, which uploads its offset tables
+ * (SIPGPU_ENOTFOUND otherwise); a workspace need beyond 16 MiB is refused
+ * with SIPGPU_ESIZE.  All calls on one device share that array: calls that
+ * overlap on the device (other streams, other threads) race. */
+int sipgpu_contract_noalloc(sipgpu_stream_t s, const int *ptrn, const double *l,
+                            int lrank, const int *ldims, const double *r,
+                            int rrank, const int *rdims, double *d, int drank,
+                            const int *ddims, int accumulate);
+
 /* One contraction of a batch.  Entries that name the same destination block
  * (same pointer, extents and pattern destination order) form one task: their
  * products are summed in list order inside one accumulator and written once;
