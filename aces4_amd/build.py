@@ -11,7 +11,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libsipgpu.so")
-SOURCES = ["plan.cpp", "permute.hip", "contract.hip", "calib.hip", "ozaki.hip", "ozaki_big.hip", "packed.hip", "synth.hip", "energy.hip", "blockops.hip", "runtime.cpp", "serve.cpp", "graph.cpp"]
+SOURCES = ["plan.cpp", "permute.hip", "contract.hip", "calib.hip", "ozaki.hip", "ozaki_big.hip", "packed.hip", "synth.hip", "energy.hip", "blockops.hip", "runtime.cpp", "serve.cpp", "graph.cpp", "dispatch.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 FLAGS = [

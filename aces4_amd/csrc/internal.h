@@ -36,6 +36,8 @@ bool capturing(hipStream_t s);
 int &thread_captures();
 // SIPGPU_ECAPTURE when the calling thread is capturing, else 0.
 int refuse_in_capture(const char *what);
+// Elements of a cached block (*elems = 0 when the key is unknown).
+int cache_block_elems(sipgpu_cache_t c, unsigned long long key, size_t *elems);
 // The library's device, and the HIP stream of `s` (null: the library's own).
 int device_id();
 hipStream_t stream_of(sipgpu_stream_t s);

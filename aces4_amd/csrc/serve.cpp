@@ -9,21 +9,26 @@
 // host memory the cache allocates for it.  Eviction is per block (the
 // reference evicts whole arrays), and never touches a block in use.
 //
-// Server: requests (one block contraction each, blocks named by cache
-// keys) are queued by any thread; a scheduler thread takes everything that
-// has arrived (up to a batch limit and half the cache budget of pinned
+// Server: requests (one block contraction or permute each, blocks named by
+// cache keys) are queued by any thread; a scheduler thread takes everything
+// that has arrived (up to a batch limit and half the cache budget of pinned
 // blocks, stopping at the first request that would read a block an earlier
-// one in the batch writes, or vice versa),
-// makes its blocks resident, and launches the batch as one grouped FP64
-// MFMA launch, its work list uploaded through a small ring of descriptor
-// slots (no allocation per batch).  A completion thread waits for each
-// batch's event and stamps its requests' completion times.
+// one in the batch writes, or vice versa; a permute is a batch of its own),
+// makes its blocks resident, and launches the batch as one grouped launch,
+// its work list uploaded through a small ring of descriptor slots (no
+// allocation per batch).  Batches alternate over two device streams
+// ("lanes"), so a batch admitted while another runs starts beside it; a
+// batch that touches a block the other lane's unfinished batches write (or
+// writes one they touch) first waits for them on the device.  One
+// completion thread per lane stamps each batch's requests when its event
+// completes.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 #include <atomic>
 #include <chrono>
 #include <condition_variable>
+#include <cstdlib>
 #include <cstring>
 #include <deque>
 #include <list>
@@ -108,8 +113,10 @@ struct CBlock {
     double a = 0.0, b = 0.0;
     bool initialised = false;      // contents defined (by upload, fill or a write)
     int pins = 0;
-    hipEvent_t last_use = nullptr; // completes when no launched work uses the device copy
-    bool used_on_device = false;
+    // Per stream that used the device copy since it became resident, an
+    // event recorded at its last release there: the copy is unused once all
+    // of them complete (work on several streams may read one block).
+    std::vector<std::pair<hipStream_t, hipEvent_t>> uses;
     std::list<unsigned long long>::iterator lru_it;
     bool in_lru = false;
 };
@@ -134,6 +141,17 @@ namespace {
 int make_event(hipEvent_t *e) {
     SIPGPU_HIP_TRY(hipEventCreateWithFlags(e, hipEventDisableTiming));
     return 0;
+}
+
+// `s` waits until no launched work uses the block's device copy.
+int wait_uses(hipStream_t s, const CBlock &b) {
+    for (const auto &u : b.uses) SIPGPU_HIP_TRY(hipStreamWaitEvent(s, u.second, 0));
+    return 0;
+}
+
+void drop_uses(CBlock &b) {
+    for (auto &u : b.uses) (void)hipEventDestroy(u.second);
+    b.uses.clear();
 }
 
 // Cache calls are host-side bookkeeping around uploads, spills, fences and
@@ -161,7 +179,7 @@ int evict_locked(sipgpu_cache_s &c, unsigned long long key, CBlock &b) {
             if (e != hipSuccess) return fail(SIPGPU_ENOMEM, "cache: pinned spill buffer");
             b.home_owned = true;
         }
-        if (b.used_on_device) SIPGPU_HIP_TRY(hipStreamWaitEvent(c.copy, b.last_use, 0));
+        if (int rc = wait_uses(c.copy, b)) return rc;
         SIPGPU_HIP_TRY(hipMemcpyAsync(b.home, c.arena.base + b.off, bytes, hipMemcpyDeviceToHost,
                                       c.copy));
         hipEvent_t f;
@@ -171,11 +189,11 @@ int evict_locked(sipgpu_cache_s &c, unsigned long long key, CBlock &b) {
         b.home_valid = true;
         c.st.spills += 1;
         c.st.bytes_d2h += bytes;
-    } else if (b.used_on_device) {
+    } else if (!b.uses.empty()) {
         // a later upload into this range must follow the last kernel reading it
         hipEvent_t f;
         if (int rc = make_event(&f)) return rc;
-        SIPGPU_HIP_TRY(hipStreamWaitEvent(c.copy, b.last_use, 0));
+        if (int rc = wait_uses(c.copy, b)) return rc;
         SIPGPU_HIP_TRY(hipEventRecord(f, c.copy));
         c.fences.push_back(f);
     }
@@ -198,10 +216,10 @@ int remove_locked(sipgpu_cache_s &c, unsigned long long key) {
         b.in_lru = false;
     }
     if (b.off != kNotResident) {
-        if (b.used_on_device) {
+        if (!b.uses.empty()) {
             hipEvent_t f;
             if (int rc = make_event(&f)) return rc;
-            SIPGPU_HIP_TRY(hipStreamWaitEvent(c.copy, b.last_use, 0));
+            if (int rc = wait_uses(c.copy, b)) return rc;
             SIPGPU_HIP_TRY(hipEventRecord(f, c.copy));
             c.fences.push_back(f);
         }
@@ -212,7 +230,7 @@ int remove_locked(sipgpu_cache_s &c, unsigned long long key) {
         SIPGPU_HIP_TRY(hipStreamSynchronize(c.copy));
         SIPGPU_HIP_TRY(hipHostFree(b.home));
     }
-    if (b.last_use) (void)hipEventDestroy(b.last_use);
+    drop_uses(b);
     c.blocks.erase(it);
     c.persistent.erase(key);
     return 0;
@@ -295,10 +313,16 @@ int release_locked(sipgpu_cache_s &c, hipStream_t stream, unsigned long long key
     if (it == c.blocks.end()) return fail(SIPGPU_ENOTFOUND, "cache: unknown block key");
     CBlock &b = it->second;
     if (b.pins <= 0) return fail(SIPGPU_EARG, "cache: release without acquire");
-    if (!b.last_use)
-        if (int rc = make_event(&b.last_use)) return rc;
-    SIPGPU_HIP_TRY(hipEventRecord(b.last_use, stream));
-    b.used_on_device = true;
+    hipEvent_t *ev = nullptr;
+    for (auto &u : b.uses)
+        if (u.first == stream) ev = &u.second;
+    if (!ev) {
+        hipEvent_t e;
+        if (int rc = make_event(&e)) return rc;
+        b.uses.emplace_back(stream, e);
+        ev = &b.uses.back().second;
+    }
+    SIPGPU_HIP_TRY(hipEventRecord(*ev, stream));
     if (--b.pins == 0) {
         c.lru.push_back(key);
         b.lru_it = std::prev(c.lru.end());
@@ -308,6 +332,14 @@ int release_locked(sipgpu_cache_s &c, hipStream_t stream, unsigned long long key
 }
 
 } // namespace
+
+int sipgpu::rt::cache_block_elems(sipgpu_cache_t c, unsigned long long key, size_t *elems) {
+    if (!c || !elems) return fail(SIPGPU_EARG, "cache: bad arguments");
+    std::lock_guard<std::mutex> g(c->mu);
+    auto it = c->blocks.find(key);
+    *elems = it == c->blocks.end() ? 0 : it->second.elems;
+    return 0;
+}
 
 extern "C" {
 
@@ -335,7 +367,7 @@ int sipgpu_cache_destroy(sipgpu_cache_t c) {
     SIPGPU_HIP_TRY(hipDeviceSynchronize());
     for (auto &kv : c->blocks) {
         if (kv.second.home_owned) (void)hipHostFree(kv.second.home);
-        if (kv.second.last_use) (void)hipEventDestroy(kv.second.last_use);
+        drop_uses(kv.second);
     }
     for (hipEvent_t f : c->fences) (void)hipEventDestroy(f);
     (void)hipStreamDestroy(c->copy);
@@ -534,10 +566,25 @@ namespace {
 
 constexpr int kSlots = 4;
 constexpr size_t kSlotBytes = 32u << 20;
+constexpr int kLanes = 2; // device streams batches alternate over
+
+enum { kKindContract = 0, kKindPermute = 1 };
+
+// A permute request: dims are the source block's extents, transp the
+// reference's dim_transp ([0] = 1, then the 1-based new position of each
+// old dimension).
+struct PermuteReq {
+    int rank = 0;
+    int dims[SIPGPU_MAX_RANK] = {};
+    int transp[SIPGPU_MAX_RANK + 1] = {};
+    unsigned long long src = 0, dst = 0;
+};
 
 struct Pending {
     unsigned long long ticket;
-    sipgpu_request req;
+    int kind;
+    sipgpu_request req; // kind == kKindContract
+    PermuteReq perm;    // kind == kKindPermute
     Clock::time_point arrival;
 };
 
@@ -546,12 +593,19 @@ struct Done {
     int status = 0;
 };
 
+// Batches of each lane that last wrote / accessed a block (0: none).
+struct KeyUse {
+    unsigned long long write[kLanes] = {};
+    unsigned long long access[kLanes] = {};
+};
+
 } // namespace
 
 struct sipgpu_server_s {
     sipgpu_cache_t cache = nullptr;
     int max_batch = 0;
-    hipStream_t stream = nullptr;
+    int lanes = kLanes;
+    hipStream_t stream[kLanes] = {};
     void *slot_dev[kSlots] = {};
     void *slot_host[kSlots] = {};
     hipEvent_t slot_free[kSlots] = {};
@@ -568,18 +622,23 @@ struct sipgpu_server_s {
 
     struct Flight {
         hipEvent_t ev;
+        unsigned long long seq;
         std::vector<std::pair<unsigned long long, Clock::time_point>> reqs;
         int status;
     };
-    std::deque<Flight> inflight;
+    std::deque<Flight> inflight[kLanes];
     std::vector<hipEvent_t> event_pool;
-
-    std::thread sched, compl_;
+    // end events of batches not yet stamped complete, by batch number (mu)
+    std::unordered_map<unsigned long long, hipEvent_t> live;
+    std::thread sched, compl_[kLanes];
     sipgpu_server_stats st{};
-
-    // split-precision batches: planes, result planes and workspace
-    void *scratch = nullptr;
-    size_t scratch_bytes = 0;
+    // scheduler thread only
+    std::unordered_map<unsigned long long, KeyUse> key_use;
+    unsigned long long next_seq = 1;
+    unsigned long long cross_waits = 0; // batches that waited on the other lane
+    // split batches: planes, result planes and workspace, per lane
+    void *scratch[kLanes] = {};
+    size_t scratch_bytes[kLanes] = {};
 };
 
 namespace {
@@ -596,44 +655,110 @@ int take_slot(sipgpu_server_s &S, int &slot) {
 }
 
 // A panel batch (split precision, or packed FP64): its descriptors through a
-// slot, its planes in the scratch buffer.  Batches run in stream order, so
-// one scratch buffer serves them all; it grows (after the stream drains)
-// when a batch needs more.
-int server_panels(sipgpu_server_s &S, const sipgpu::rt::PanelPlan &plan,
+// slot, its planes in the lane's scratch buffer.  Batches of a lane run in
+// stream order, so one scratch buffer per lane serves them all; it grows
+// (after the lane's stream drains) when a batch needs more.
+int server_panels(sipgpu_server_s &S, int lane, const sipgpu::rt::PanelPlan &plan,
                   const std::vector<unsigned char> &desc) {
     if (plan.ntiles == 0) return 0;
+    const hipStream_t st = S.stream[lane];
     auto up = [](size_t v) { return (v + 255) & ~(size_t)255; };
     const size_t need = up(plan.plane_total()) + up(plan.res_total()) + up(plan.work_total());
-    if (need > S.scratch_bytes) {
-        SIPGPU_HIP_TRY(hipStreamSynchronize(S.stream));
-        if (S.scratch) SIPGPU_HIP_TRY(hipFree(S.scratch));
-        S.scratch = nullptr;
-        S.scratch_bytes = 0;
+    if (need > S.scratch_bytes[lane]) {
+        SIPGPU_HIP_TRY(hipStreamSynchronize(st));
+        if (S.scratch[lane]) SIPGPU_HIP_TRY(hipFree(S.scratch[lane]));
+        S.scratch[lane] = nullptr;
+        S.scratch_bytes[lane] = 0;
         const size_t bytes = std::max(need, (size_t)256 << 20);
-        SIPGPU_HIP_TRY(hipMalloc(&S.scratch, bytes));
-        S.scratch_bytes = bytes;
+        SIPGPU_HIP_TRY(hipMalloc(&S.scratch[lane], bytes));
+        S.scratch_bytes[lane] = bytes;
     }
     int slot = 0;
     if (int rc = take_slot(S, slot)) return rc;
-    int8_t *planes = static_cast<int8_t *>(S.scratch);
+    int8_t *planes = static_cast<int8_t *>(S.scratch[lane]);
     uint8_t *res = reinterpret_cast<uint8_t *>(planes + up(plan.plane_total()));
     double *work = plan.nslots ? reinterpret_cast<double *>(res + up(plan.res_total())) : nullptr;
     std::memcpy(S.slot_host[slot], desc.data(), desc.size());
     hipError_t e = hipMemcpyAsync(S.slot_dev[slot], S.slot_host[slot], desc.size(),
-                                  hipMemcpyHostToDevice, S.stream);
+                                  hipMemcpyHostToDevice, st);
     S.slot_used[slot] = true;
     if (e == hipSuccess)
-        e = sipgpu::oz_zero_heads(planes, plan.plane_bytes, plan.nmod ? plan.nmod : 1, S.stream);
+        e = sipgpu::oz_zero_heads(planes, plan.plane_bytes, plan.nmod ? plan.nmod : 1, st);
     if (e != hipSuccess) return hip_fail(e, "server: split batch");
     sipgpu::rt::SplitData sd;
     sipgpu::rt::bind_panel_plan(plan, S.slot_dev[slot], planes, res, work, sd);
-    int rc = sipgpu::rt::execute_panels(sd, plan.nmod > 0, SIPGPU_STAGE_ALL, S.stream);
-    e = hipEventRecord(S.slot_free[slot], S.stream);
+    int rc = sipgpu::rt::execute_panels(sd, plan.nmod > 0, SIPGPU_STAGE_ALL, st);
+    e = hipEventRecord(S.slot_free[slot], st);
     if (!rc && e != hipSuccess) rc = hip_fail(e, "server: split batch");
     return rc;
 }
 
-int server_batch(sipgpu_server_s &S, std::vector<Pending> &batch) {
+size_t elems_of(int rank, const int *dims) {
+    size_t n = 1;
+    for (int i = 0; i < rank; ++i) n *= (size_t)dims[i];
+    return n;
+}
+
+// The cache's size of a block (0 when the key is unknown).
+size_t cache_elems(sipgpu_cache_s *c, unsigned long long key) {
+    std::lock_guard<std::mutex> g(c->mu);
+    auto it = c->blocks.find(key);
+    return it == c->blocks.end() ? 0 : it->second.elems;
+}
+
+// Every block a request names holds exactly its extents' elements: a
+// mismatch would let a kernel read or write outside the block.
+int check_sizes(sipgpu_cache_s *c, const Pending &p) {
+    if (p.kind == kKindPermute) {
+        const size_t n = elems_of(p.perm.rank, p.perm.dims);
+        const size_t a = cache_elems(c, p.perm.src), b = cache_elems(c, p.perm.dst);
+        if (!a || !b) return fail(SIPGPU_ENOTFOUND, "server: unknown block key");
+        if (a != n || b != n)
+            return fail(SIPGPU_EARG, "server: permute extents do not match the cached blocks");
+        return 0;
+    }
+    const sipgpu_request &r = p.req;
+    const unsigned long long keys[3] = {r.lkey, r.rkey, r.dkey};
+    const size_t want[3] = {elems_of(r.lrank, r.ldims), elems_of(r.rrank, r.rdims),
+                            elems_of(r.drank, r.ddims)};
+    for (int q = 0; q < 3; ++q) {
+        const size_t have = cache_elems(c, keys[q]);
+        if (!have) return fail(SIPGPU_ENOTFOUND, "server: unknown block key");
+        if (have != want[q])
+            return fail(SIPGPU_EARG, "server: request extents do not match the cached blocks");
+    }
+    return 0;
+}
+
+int server_permute(sipgpu_server_s &S, int lane, const PermuteReq &p) {
+    if (int rc = check_sizes(S.cache, Pending{0, kKindPermute, {}, p, {}})) return rc;
+    sipgpu_stream_s sw;
+    sw.s = S.stream[lane];
+    double *src = nullptr, *dst = nullptr;
+    int rc = sipgpu_cache_acquire(S.cache, &sw, p.src, SIPGPU_CACHE_READ, &src);
+    if (rc) return rc;
+    rc = sipgpu_cache_acquire(S.cache, &sw, p.dst, SIPGPU_CACHE_WRITE, &dst);
+    if (!rc) {
+        rc = sipgpu_permute(&sw, p.rank, p.dims, p.transp, src, dst);
+        const int r2 = sipgpu_cache_release(S.cache, &sw, p.dst);
+        if (!rc) rc = r2;
+    }
+    const int r3 = sipgpu_cache_release(S.cache, &sw, p.src);
+    if (!rc) rc = r3;
+    if (!rc) {
+        S.st.batches += 1;
+        S.st.requests += 1;
+        S.st.max_batch = std::max<unsigned long long>(S.st.max_batch, 1);
+    }
+    return rc;
+}
+
+int server_batch(sipgpu_server_s &S, int lane, std::vector<Pending> &batch) {
+    if (batch.size() == 1 && batch[0].kind == kKindPermute)
+        return server_permute(S, lane, batch[0].perm);
+    for (const Pending &p : batch)
+        if (int rc = check_sizes(S.cache, p)) return rc;
+    const hipStream_t stream = S.stream[lane];
     // blocks of the batch, each acquired once: destinations for write
     // (read-write if their first entry accumulates), operands for read
     std::unordered_map<unsigned long long, double *> dptr;
@@ -644,7 +769,7 @@ int server_batch(sipgpu_server_s &S, std::vector<Pending> &batch) {
             dmode[p.req.dkey] = p.req.accumulate ? SIPGPU_CACHE_READWRITE : SIPGPU_CACHE_WRITE;
     int rc = 0;
     sipgpu_stream_s sw;
-    sw.s = S.stream;
+    sw.s = stream;
     for (const Pending &p : batch) {
         const unsigned long long keys[3] = {p.req.lkey, p.req.rkey, p.req.dkey};
         for (int q = 0; q < 3 && !rc; ++q) {
@@ -694,18 +819,18 @@ int server_batch(sipgpu_server_s &S, std::vector<Pending> &batch) {
     const std::vector<unsigned char> &desc = panels ? plan.host : img.host;
     if (!rc && desc.size() > kSlotBytes)
         rc = fail(SIPGPU_ESIZE, "server: batch work list exceeds its descriptor slot");
-    if (!rc && panels) rc = server_panels(S, plan, desc);
+    if (!rc && panels) rc = server_panels(S, lane, plan, desc);
     if (!rc && !panels && img.ntiles > 0) {
         int slot = 0;
         rc = take_slot(S, slot);
         if (!rc) {
             std::memcpy(S.slot_host[slot], img.host.data(), img.host.size());
             hipError_t e = hipMemcpyAsync(S.slot_dev[slot], S.slot_host[slot], img.host.size(),
-                                          hipMemcpyHostToDevice, S.stream);
+                                          hipMemcpyHostToDevice, stream);
             if (e == hipSuccess)
                 e = sipgpu::launch_gemm(img.cfg, img.args_at(S.slot_dev[slot]), img.ntiles,
-                                        S.stream);
-            if (e == hipSuccess) e = hipEventRecord(S.slot_free[slot], S.stream);
+                                        stream);
+            if (e == hipSuccess) e = hipEventRecord(S.slot_free[slot], stream);
             if (e != hipSuccess) rc = hip_fail(e, "server: launch");
             S.slot_used[slot] = true;
         }
@@ -723,8 +848,72 @@ int server_batch(sipgpu_server_s &S, std::vector<Pending> &batch) {
     return rc;
 }
 
+// Blocks a batch reads and writes.
+void batch_keys(const std::vector<Pending> &batch, std::vector<unsigned long long> &reads,
+                std::vector<unsigned long long> &writes) {
+    for (const Pending &p : batch) {
+        if (p.kind == kKindPermute) {
+            reads.push_back(p.perm.src);
+            writes.push_back(p.perm.dst);
+        } else {
+            reads.push_back(p.req.lkey);
+            reads.push_back(p.req.rkey);
+            writes.push_back(p.req.dkey);
+            if (p.req.accumulate) reads.push_back(p.req.dkey);
+        }
+    }
+}
+
+// Orders a batch about to run on `lane` after the other lanes' batches that
+// it conflicts with: a block it reads after their last write of it, a block
+// it writes after their last access.  Batches of one lane are ordered by
+// their stream.  Returns whether it had to wait.
+bool order_after_other_lanes(sipgpu_server_s &S, int lane,
+                             const std::vector<unsigned long long> &reads,
+                             const std::vector<unsigned long long> &writes) {
+    std::vector<unsigned long long> after;
+    for (unsigned long long k : reads) {
+        auto it = S.key_use.find(k);
+        if (it == S.key_use.end()) continue;
+        for (int o = 0; o < S.lanes; ++o)
+            if (o != lane && it->second.write[o]) after.push_back(it->second.write[o]);
+    }
+    for (unsigned long long k : writes) {
+        auto it = S.key_use.find(k);
+        if (it == S.key_use.end()) continue;
+        for (int o = 0; o < S.lanes; ++o)
+            if (o != lane && it->second.access[o]) after.push_back(it->second.access[o]);
+    }
+    if (after.empty()) return false;
+    std::sort(after.begin(), after.end());
+    after.erase(std::unique(after.begin(), after.end()), after.end());
+    bool waited = false;
+    // Only this thread records the pooled events, so an event found live
+    // here still marks the end of that batch.
+    std::lock_guard<std::mutex> lk(S.mu);
+    for (unsigned long long seq : after) {
+        auto it = S.live.find(seq);
+        if (it == S.live.end()) continue; // already complete
+        (void)hipStreamWaitEvent(S.stream[lane], it->second, 0);
+        waited = true;
+    }
+    return waited;
+}
+
+void note_use(sipgpu_server_s &S, int lane, unsigned long long seq,
+              const std::vector<unsigned long long> &reads,
+              const std::vector<unsigned long long> &writes) {
+    for (unsigned long long k : reads) S.key_use[k].access[lane] = seq;
+    for (unsigned long long k : writes) {
+        KeyUse &u = S.key_use[k];
+        u.write[lane] = seq;
+        u.access[lane] = seq;
+    }
+}
+
 void scheduler_loop(sipgpu_server_s *S) {
     (void)hipSetDevice(sipgpu::rt::device_id());
+    int lane = 0;
     for (;;) {
         std::vector<Pending> batch;
         {
@@ -736,30 +925,38 @@ void scheduler_loop(sipgpu_server_s *S) {
             const size_t pin_limit = S->cache->arena.size / 2;
             size_t pinned = 0;
             while (!S->queue.empty() && (int)batch.size() < S->max_batch) {
-                const sipgpu_request &r = S->queue.front().req;
+                const Pending &p = S->queue.front();
+                // a permute is a batch of its own
+                if (p.kind == kKindPermute) {
+                    if (batch.empty()) {
+                        batch.push_back(p);
+                        S->queue.pop_front();
+                    }
+                    break;
+                }
+                const sipgpu_request &r = p.req;
                 // keep program order: no block both written and read in one batch
                 if (writes.count(r.lkey) || writes.count(r.rkey) || reads.count(r.dkey)) break;
                 // one precision per batch
                 if (!batch.empty() && r.moduli != batch[0].req.moduli) break;
                 size_t add = 0;
-                auto bytes_of = [](int rank, const int *dims) {
-                    size_t n = 8;
-                    for (int i = 0; i < rank; ++i) n *= (size_t)dims[i];
-                    return n;
-                };
-                if (!reads.count(r.lkey)) add += bytes_of(r.lrank, r.ldims);
-                if (!reads.count(r.rkey)) add += bytes_of(r.rrank, r.rdims);
-                if (!writes.count(r.dkey)) add += bytes_of(r.drank, r.ddims);
+                if (!reads.count(r.lkey)) add += 8 * elems_of(r.lrank, r.ldims);
+                if (!reads.count(r.rkey)) add += 8 * elems_of(r.rrank, r.rdims);
+                if (!writes.count(r.dkey)) add += 8 * elems_of(r.drank, r.ddims);
                 if (!batch.empty() && pinned + add > pin_limit) break;
                 pinned += add;
                 reads.insert(r.lkey);
                 reads.insert(r.rkey);
                 writes.insert(r.dkey);
-                batch.push_back(S->queue.front());
+                batch.push_back(p);
                 S->queue.pop_front();
             }
         }
-        const int rc = server_batch(*S, batch);
+        std::vector<unsigned long long> rkeys, wkeys;
+        batch_keys(batch, rkeys, wkeys);
+        const unsigned long long seq = S->next_seq++;
+        if (order_after_other_lanes(*S, lane, rkeys, wkeys)) S->cross_waits += 1;
+        const int rc = server_batch(*S, lane, batch);
         hipEvent_t ev = nullptr;
         {
             std::lock_guard<std::mutex> lk(S->mu);
@@ -769,29 +966,35 @@ void scheduler_loop(sipgpu_server_s *S) {
             }
         }
         if (!ev) (void)hipEventCreateWithFlags(&ev, hipEventDisableTiming);
-        (void)hipEventRecord(ev, S->stream);
+        (void)hipEventRecord(ev, S->stream[lane]);
+        note_use(*S, lane, seq, rkeys, wkeys);
         {
             std::lock_guard<std::mutex> lk(S->mu);
+            S->live[seq] = ev;
             sipgpu_server_s::Flight f;
             f.ev = ev;
+            f.seq = seq;
             f.status = rc;
             for (const Pending &p : batch) f.reqs.emplace_back(p.ticket, p.arrival);
-            S->inflight.push_back(std::move(f));
+            S->inflight[lane].push_back(std::move(f));
         }
-        S->cv_inflight.notify_one();
+        S->cv_inflight.notify_all();
+        lane = (lane + 1) % S->lanes;
     }
 }
 
-void completion_loop(sipgpu_server_s *S) {
+// One completion thread per lane: a lane's batches end in stream order, so
+// each is stamped as soon as its own event completes.
+void completion_loop(sipgpu_server_s *S, int lane) {
     (void)hipSetDevice(sipgpu::rt::device_id());
     for (;;) {
         sipgpu_server_s::Flight f;
         {
             std::unique_lock<std::mutex> lk(S->mu);
-            S->cv_inflight.wait(lk, [&] { return !S->inflight.empty() || S->sched_done; });
-            if (S->inflight.empty()) return; // scheduler finished and all batches stamped
-            f = std::move(S->inflight.front());
-            S->inflight.pop_front();
+            S->cv_inflight.wait(lk, [&] { return !S->inflight[lane].empty() || S->sched_done; });
+            if (S->inflight[lane].empty()) return; // scheduler finished and all batches stamped
+            f = std::move(S->inflight[lane].front());
+            S->inflight[lane].pop_front();
         }
         const hipError_t e = hipEventSynchronize(f.ev);
         const Clock::time_point t = Clock::now();
@@ -803,6 +1006,7 @@ void completion_loop(sipgpu_server_s *S) {
             d.status = f.status ? f.status : (e == hipSuccess ? 0 : SIPGPU_EDEVICE);
             S->done[r.first] = d;
         }
+        S->live.erase(f.seq);
         S->event_pool.push_back(f.ev);
         S->cv_done.notify_all();
     }
@@ -819,18 +1023,27 @@ int check_request(const sipgpu_request &r) {
     return 0;
 }
 
-int submit_at(sipgpu_server_s *S, const sipgpu_request *r, Clock::time_point arrival,
-              unsigned long long *ticket) {
-    if (int rc = check_request(*r)) return rc;
+int enqueue(sipgpu_server_s *S, Pending p, unsigned long long *ticket) {
+    if (int rc = check_sizes(S->cache, p)) return rc;
     {
         std::lock_guard<std::mutex> lk(S->mu);
         if (S->stop) return fail(SIPGPU_EARG, "server: stopped");
-        const unsigned long long t = S->next_ticket++;
-        S->queue.push_back(Pending{t, *r, arrival});
-        if (ticket) *ticket = t;
+        p.ticket = S->next_ticket++;
+        if (ticket) *ticket = p.ticket;
+        S->queue.push_back(p);
     }
     S->cv_queue.notify_one();
     return 0;
+}
+
+int submit_at(sipgpu_server_s *S, const sipgpu_request *r, Clock::time_point arrival,
+              unsigned long long *ticket) {
+    if (int rc = check_request(*r)) return rc;
+    Pending p{};
+    p.kind = kKindContract;
+    p.req = *r;
+    p.arrival = arrival;
+    return enqueue(S, p, ticket);
 }
 
 int wait_ticket(sipgpu_server_s *S, unsigned long long ticket, double *latency_ms) {
@@ -849,19 +1062,24 @@ int wait_ticket(sipgpu_server_s *S, unsigned long long ticket, double *latency_m
 extern "C" {
 
 int sipgpu_server_create(sipgpu_cache_t cache, int max_batch, sipgpu_server_t *server) {
+    if (int rc = sipgpu::rt::refuse_in_capture("creating a server")) return rc;
     if (!cache || !server || max_batch < 1) return fail(SIPGPU_EARG, "server: bad arguments");
     if (int rc = sipgpu::rt::ensure_ready()) return rc;
     auto S = std::make_unique<sipgpu_server_s>();
     S->cache = cache;
     S->max_batch = max_batch;
-    SIPGPU_HIP_TRY(hipStreamCreateWithFlags(&S->stream, hipStreamNonBlocking));
+    // SIPGPU_SERVER_LANES=1 runs every batch on one stream (experiments)
+    if (const char *e = std::getenv("SIPGPU_SERVER_LANES"))
+        S->lanes = std::max(1, std::min(kLanes, std::atoi(e)));
+    for (int l = 0; l < S->lanes; ++l)
+        SIPGPU_HIP_TRY(hipStreamCreateWithFlags(&S->stream[l], hipStreamNonBlocking));
     for (int i = 0; i < kSlots; ++i) {
         SIPGPU_HIP_TRY(hipMalloc(&S->slot_dev[i], kSlotBytes));
         SIPGPU_HIP_TRY(hipHostMalloc(&S->slot_host[i], kSlotBytes, hipHostMallocDefault));
         SIPGPU_HIP_TRY(hipEventCreateWithFlags(&S->slot_free[i], hipEventDisableTiming));
     }
     S->sched = std::thread(scheduler_loop, S.get());
-    S->compl_ = std::thread(completion_loop, S.get());
+    for (int l = 0; l < S->lanes; ++l) S->compl_[l] = std::thread(completion_loop, S.get(), l);
     *server = S.release();
     return 0;
 }
@@ -869,6 +1087,34 @@ int sipgpu_server_create(sipgpu_cache_t cache, int max_batch, sipgpu_server_t *s
 int sipgpu_server_submit(sipgpu_server_t S, const sipgpu_request *r, unsigned long long *ticket) {
     if (!S || !r) return fail(SIPGPU_EARG, "server: bad arguments");
     return submit_at(S, r, Clock::now(), ticket);
+}
+
+int sipgpu_server_submit_permute(sipgpu_server_t S, int rank, const int *dims,
+                                 const int *dim_transp, unsigned long long src_key,
+                                 unsigned long long dst_key, unsigned long long *ticket) {
+    if (!S || !dims || !dim_transp) return fail(SIPGPU_EARG, "server: bad arguments");
+    if (rank < 1 || rank > SIPGPU_MAX_RANK)
+        return fail(SIPGPU_ERANK, "server: permutes of rank 1..6");
+    if (src_key == dst_key) return fail(SIPGPU_EHAZARD, "server: a permute writes the block it reads");
+    Pending p{};
+    p.kind = kKindPermute;
+    p.perm.rank = rank;
+    p.perm.src = src_key;
+    p.perm.dst = dst_key;
+    std::memcpy(p.perm.dims, dims, sizeof(int) * rank);
+    std::memcpy(p.perm.transp, dim_transp, sizeof(int) * (rank + 1));
+    // a permutation of 1..rank (the kernel's plan checks it again)
+    unsigned seen = 0;
+    for (int i = 1; i <= rank; ++i) {
+        const int t = dim_transp[i];
+        if (t < 1 || t > rank || (seen >> t) & 1u)
+            return fail(SIPGPU_EPERM, "server: dim_transp is not a permutation");
+        seen |= 1u << t;
+    }
+    for (int i = 0; i < rank; ++i)
+        if (dims[i] < 1) return fail(SIPGPU_EARG, "server: permute extents must be positive");
+    p.arrival = Clock::now();
+    return enqueue(S, p, ticket);
 }
 
 int sipgpu_server_wait(sipgpu_server_t S, unsigned long long ticket, double *latency_ms) {
@@ -909,7 +1155,15 @@ int sipgpu_server_get_stats(sipgpu_server_t S, sipgpu_server_stats *out) {
     return 0;
 }
 
+int sipgpu_server_lanes(sipgpu_server_t S, int *lanes, unsigned long long *cross_waits) {
+    if (!S) return fail(SIPGPU_EARG, "null server");
+    if (lanes) *lanes = S->lanes;
+    if (cross_waits) *cross_waits = S->cross_waits;
+    return 0;
+}
+
 int sipgpu_server_destroy(sipgpu_server_t S) {
+    if (int rc = sipgpu::rt::refuse_in_capture("destroying a server")) return rc;
     if (!S) return fail(SIPGPU_EARG, "null server");
     {
         std::lock_guard<std::mutex> lk(S->mu);
@@ -922,16 +1176,18 @@ int sipgpu_server_destroy(sipgpu_server_t S) {
         S->sched_done = true;
     }
     S->cv_inflight.notify_all();
-    S->compl_.join();
-    (void)hipStreamSynchronize(S->stream);
+    for (int l = 0; l < S->lanes; ++l) S->compl_[l].join();
+    for (int l = 0; l < S->lanes; ++l) (void)hipStreamSynchronize(S->stream[l]);
     for (int i = 0; i < kSlots; ++i) {
         (void)hipFree(S->slot_dev[i]);
         (void)hipHostFree(S->slot_host[i]);
         (void)hipEventDestroy(S->slot_free[i]);
     }
     for (hipEvent_t e : S->event_pool) (void)hipEventDestroy(e);
-    if (S->scratch) (void)hipFree(S->scratch);
-    (void)hipStreamDestroy(S->stream);
+    for (int l = 0; l < S->lanes; ++l) {
+        if (S->scratch[l]) (void)hipFree(S->scratch[l]);
+        (void)hipStreamDestroy(S->stream[l]);
+    }
     delete S;
     return 0;
 }

@@ -56,7 +56,12 @@ EXPORTED = [
     "sipgpu_cache_release", "sipgpu_cache_read", "sipgpu_cache_flush", "sipgpu_cache_evict",
     "sipgpu_cache_get_stats", "sipgpu_server_create", "sipgpu_server_submit",
     "sipgpu_server_wait", "sipgpu_server_replay", "sipgpu_server_get_stats",
-    "sipgpu_server_destroy", "sipgpu_contract_noalloc",
+    "sipgpu_server_destroy", "sipgpu_contract_noalloc", "sipgpu_server_submit_permute",
+    "sipgpu_server_lanes", "sipgpu_worker_create", "sipgpu_worker_destroy",
+    "sipgpu_worker_push_selector", "sipgpu_worker_stack_depth", "sipgpu_worker_contract",
+    "sipgpu_worker_permute", "sipgpu_worker_contract_to_scalar", "sipgpu_worker_read",
+    "sipgpu_worker_write", "sipgpu_worker_flush", "sipgpu_worker_get_stats",
+    "sipgpu_permute_rhs_to_lhs",
 ]
 
 
@@ -99,6 +104,15 @@ class CacheStats(ctypes.Structure):
     _fields_ = [(n, ctypes.c_ulonglong) for n in
                 ("hits", "misses", "evictions", "spills", "bytes_h2d", "bytes_d2h",
                  "resident_bytes", "budget_bytes", "blocks")]
+
+    def as_dict(self):
+        return {n: int(getattr(self, n)) for n, _ in self._fields_}
+
+
+class WorkerStats(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_ulonglong) for n in
+                ("contractions", "permutes", "scalars", "reads", "writes", "waits", "flushes",
+                 "queued")]
 
     def as_dict(self):
         return {n: int(getattr(self, n)) for n, _ in self._fields_}
@@ -205,6 +219,22 @@ def lib():
                                            ctypes.POINTER(ctypes.c_double)]
         L.sipgpu_server_get_stats.argtypes = [_vp, ctypes.POINTER(ServerStats)]
         L.sipgpu_server_destroy.argtypes = [_vp]
+        L.sipgpu_server_submit_permute.argtypes = [_vp, ctypes.c_int, _ip, _ip, _u64, _u64,
+                                                   ctypes.POINTER(_u64)]
+        L.sipgpu_server_lanes.argtypes = [_vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(_u64)]
+        L.sipgpu_worker_create.argtypes = [_vp, _vp, ctypes.POINTER(_vp)]
+        L.sipgpu_worker_destroy.argtypes = [_vp]
+        L.sipgpu_worker_push_selector.argtypes = [_vp, _u64, ctypes.c_int, _ip, _ip]
+        L.sipgpu_worker_stack_depth.argtypes = [_vp, ctypes.POINTER(ctypes.c_int)]
+        L.sipgpu_worker_contract.argtypes = [_vp, _u64, ctypes.c_int, _ip, _ip, ctypes.c_int,
+                                             ctypes.c_int, ctypes.POINTER(_u64)]
+        L.sipgpu_worker_permute.argtypes = [_vp, _ip, ctypes.POINTER(_u64)]
+        L.sipgpu_worker_contract_to_scalar.argtypes = [_vp, ctypes.POINTER(ctypes.c_double)]
+        L.sipgpu_worker_read.argtypes = [_vp, _u64, _vp]
+        L.sipgpu_worker_write.argtypes = [_vp, _u64, ctypes.c_int, _ip, _vp]
+        L.sipgpu_worker_flush.argtypes = [_vp, ctypes.c_int]
+        L.sipgpu_worker_get_stats.argtypes = [_vp, ctypes.POINTER(WorkerStats)]
+        L.sipgpu_permute_rhs_to_lhs.argtypes = [ctypes.c_int, _ip, _ip, _ip]
         _pp = ctypes.POINTER(ctypes.c_void_p)
         L.sipgpu_energy_denominator.argtypes = [_vp, ctypes.c_int, _ip, _vp, _pp, _ip]
         L.sipgpu_energy_sum.argtypes = [_vp, ctypes.c_int, _ip, _vp, _vp, _pp, _ip,
@@ -931,10 +961,24 @@ class Server:
         _check(lib().sipgpu_server_submit(self.handle, ctypes.byref(req), ctypes.byref(t)))
         return t.value
 
+    def submit_permute(self, dims, dim_transp, src_key, dst_key):
+        """Queues tensor_block_copy__(dims, dim_transp) of block src into dst."""
+        d, t = _ints(dims), _ints(dim_transp)
+        tk = ctypes.c_ulonglong(0)
+        _check(lib().sipgpu_server_submit_permute(self.handle, len(dims), _iptr(d), _iptr(t),
+                                                  src_key, dst_key, ctypes.byref(tk)))
+        return tk.value
+
     def wait(self, ticket):
         lat = ctypes.c_double(0)
         _check(lib().sipgpu_server_wait(self.handle, ticket, ctypes.byref(lat)))
         return lat.value
+
+    def lanes(self):
+        """(device streams, batches that waited on a conflicting batch of another stream)"""
+        n, w = ctypes.c_int(0), ctypes.c_ulonglong(0)
+        _check(lib().sipgpu_server_lanes(self.handle, ctypes.byref(n), ctypes.byref(w)))
+        return n.value, int(w.value)
 
     def replay(self, reqs, arrival_s):
         n = len(reqs)
@@ -955,4 +999,84 @@ class Server:
     def destroy(self):
         if self.handle:
             _check(lib().sipgpu_server_destroy(self.handle))
+            self.handle = None
+
+
+def permute_rhs_to_lhs(lhs_ids, rhs_ids):
+    """Interpreter::permute_rhs_to_lhs's 0-based old-to-new vector (host
+    only): six entries, -1 beyond the rank.  Raises SIPGPU_EPERM for an
+    illegal transpose."""
+    lhs = _ints(list(lhs_ids) + [-1] * (MAX_RANK - len(lhs_ids)))
+    rhs = _ints(list(rhs_ids) + [-1] * (MAX_RANK - len(rhs_ids)))
+    out = _ints([0] * MAX_RANK)
+    _check(lib().sipgpu_permute_rhs_to_lhs(len(lhs_ids), _iptr(lhs), _iptr(rhs), _iptr(out)))
+    return [int(x) for x in out]
+
+
+class Worker:
+    """The SIP worker's contraction / permute dispatch over a Server and its
+    Cache, driven in the interpreter's terms: push block selectors, then
+    contract (pops right, then left), permute (pops the left-hand side, then
+    the right-hand side) or contract to a scalar."""
+
+    def __init__(self, cache, server):
+        p = _vp()
+        _check(lib().sipgpu_worker_create(cache.handle, server.handle, ctypes.byref(p)))
+        self.handle = p.value
+
+    def push(self, key, index_ids, extents):
+        ids, ext = _ints(index_ids), _ints(extents)
+        _check(lib().sipgpu_worker_push_selector(self.handle, key, len(index_ids), _iptr(ids),
+                                                 _iptr(ext)))
+
+    def depth(self):
+        d = ctypes.c_int(0)
+        _check(lib().sipgpu_worker_stack_depth(self.handle, ctypes.byref(d)))
+        return d.value
+
+    def contract(self, dkey, dindex_ids, dextents, accumulate=False, moduli=0):
+        ids, ext = _ints(dindex_ids), _ints(dextents)
+        t = ctypes.c_ulonglong(0)
+        _check(lib().sipgpu_worker_contract(self.handle, dkey, len(dindex_ids), _iptr(ids),
+                                            _iptr(ext), 1 if accumulate else 0, int(moduli),
+                                            ctypes.byref(t)))
+        return t.value
+
+    def permute(self, compiler_perm=None):
+        t = ctypes.c_ulonglong(0)
+        cp = None
+        if compiler_perm is not None:
+            cp = _ints(list(compiler_perm) + [-1] * (MAX_RANK - len(compiler_perm)))
+        _check(lib().sipgpu_worker_permute(self.handle, _iptr(cp) if cp is not None else None,
+                                           ctypes.byref(t)))
+        return t.value
+
+    def contract_to_scalar(self):
+        v = ctypes.c_double(0)
+        _check(lib().sipgpu_worker_contract_to_scalar(self.handle, ctypes.byref(v)))
+        return v.value
+
+    def read(self, key, elems):
+        out = np.empty(int(elems))
+        _check(lib().sipgpu_worker_read(self.handle, key, out.ctypes.data))
+        return out
+
+    def write(self, key, extents, host):
+        host = np.ascontiguousarray(host, dtype=np.float64)
+        ext = _ints(extents)
+        assert host.size == int(np.prod(extents))
+        _check(lib().sipgpu_worker_write(self.handle, key, len(extents), _iptr(ext),
+                                         host.ctypes.data))
+
+    def flush(self, writeback=False):
+        _check(lib().sipgpu_worker_flush(self.handle, 1 if writeback else 0))
+
+    def stats(self):
+        st = WorkerStats()
+        _check(lib().sipgpu_worker_get_stats(self.handle, ctypes.byref(st)))
+        return st.as_dict()
+
+    def destroy(self):
+        if self.handle:
+            _check(lib().sipgpu_worker_destroy(self.handle))
             self.handle = None

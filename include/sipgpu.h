@@ -442,17 +442,24 @@ int sipgpu_cache_end_program(sipgpu_cache_t cache, unsigned long long *kept);
 
 /* ------------------------------------------------------------------------
  * Contraction server: continuous batching of single block contractions
- * whose blocks live in a cache.  A scheduler thread takes every request that
- * has arrived (up to `max_batch`, stopping at the first one that would read
- * a block an earlier request of the batch writes, or the reverse, and at a
- * change of precision), makes its blocks resident and launches the batch:
- * one grouped FP64 launch, or the split-precision stages with their planes
- * in a server-owned scratch buffer (grown, after the device drains, to the
- * largest batch's need); a completion thread stamps each request's
- * completion.  A batch pins at most
- * half the cache budget (one request beyond that alone).  Requests on the same
- * blocks take effect in submission order.  No reference counterpart: the
- * SIP's worker runs one contraction at a time (interpreter.cpp:1210-1262). */
+ * (and permutes) whose blocks live in a cache.  A scheduler thread takes
+ * every request that has arrived (up to `max_batch`, stopping at the first
+ * one that would read a block an earlier request of the batch writes, or the
+ * reverse, and at a change of precision; a permute is a batch of its own),
+ * makes its blocks resident and launches the batch: one grouped FP64
+ * launch, or the split-precision stages with their planes in a server-owned
+ * scratch buffer per stream (grown, after that stream drains, to the
+ * largest batch's need).  Batches alternate over two device streams, so a
+ * batch admitted while another runs starts beside it on the device; a batch
+ * that reads a block an unfinished batch of the other stream writes (or
+ * writes one it reads or writes) waits for it on the device first.  One
+ * completion thread per stream stamps each request's completion.  A batch
+ * pins at most half the cache budget (one request beyond that alone).
+ * Requests on the same blocks take effect in submission order.  Every block
+ * a request names must be in the cache with exactly its extents' elements
+ * (SIPGPU_ENOTFOUND / SIPGPU_EARG at submission otherwise).  No reference
+ * counterpart: the SIP's worker runs one contraction at a time
+ * (interpreter.cpp:1210-1262). */
 typedef struct sipgpu_server_s *sipgpu_server_t;
 
 typedef struct {
@@ -472,6 +479,17 @@ typedef struct {
 int sipgpu_server_create(sipgpu_cache_t cache, int max_batch, sipgpu_server_t *server);
 int sipgpu_server_submit(sipgpu_server_t server, const sipgpu_request *req,
                          unsigned long long *ticket);
+/* Queues a permute of block `src_key` into block `dst_key` (the permute
+ * super-instruction, tensor_block_copy__ semantics: `dims` are the source's
+ * extents, dim_transp[0] = 1 and dim_transp[i] the new position of old
+ * dimension i).  It runs as a batch of its own, in submission order with
+ * the contractions around it. */
+int sipgpu_server_submit_permute(sipgpu_server_t server, int rank, const int *dims,
+                                 const int *dim_transp, unsigned long long src_key,
+                                 unsigned long long dst_key, unsigned long long *ticket);
+/* Device streams the server's batches alternate over, and how many batches
+ * had to wait on the device for a conflicting batch of another stream. */
+int sipgpu_server_lanes(sipgpu_server_t server, int *lanes, unsigned long long *cross_waits);
 /* Waits for a request; latency = completion - submission, milliseconds. */
 int sipgpu_server_wait(sipgpu_server_t server, unsigned long long ticket, double *latency_ms);
 /* Submits reqs[i] at arrival_s[i] seconds after the call (an arrival
@@ -481,6 +499,71 @@ int sipgpu_server_replay(sipgpu_server_t server, const sipgpu_request *reqs,
                          const double *arrival_s, int n, double *latency_ms, double *wall_s);
 int sipgpu_server_get_stats(sipgpu_server_t server, sipgpu_server_stats *stats);
 int sipgpu_server_destroy(sipgpu_server_t server);
+
+/* ------------------------------------------------------------------------
+ * Worker dispatch: the SIP interpreter's contraction and permute
+ * instructions, taken in the interpreter's own terms and queued on a server.
+ *
+ * sipgpu_worker_push_selector pushes a block selector (the block's cache key,
+ * its rank, the SIAL index ids of the selector and the block's extents) as
+ * the interpreter's block_selector_stack_ does.  Then:
+ *  - sipgpu_worker_contract replaces the block_contract_op case and
+ *    Interpreter::handle_contraction (interpreter.cpp:734-741, :1210-1262):
+ *    it pops the RIGHT operand first, then the left, builds the ACES pattern
+ *    [destination ids | left ids | right ids], converts it with
+ *    get_contraction_ptrn_ (SIPGPU_EPATTERN on a non-zero ierr, which the
+ *    interpreter turns into a failure at :1243) and queues the contraction
+ *    (accumulate = 1: the block_contract_accumulate_op semantics; moduli 0 or
+ *    6..14 as in sipgpu_request).  A destination missing from the cache is
+ *    created zeroed, as get_block_for_writing creates it;
+ *  - sipgpu_worker_permute replaces block_permute_op, permute_rhs_to_lhs and
+ *    Block::transpose_copy (interpreter.cpp:656-679, :2049-2084;
+ *    block.cpp:216-255): it pops the LEFT-hand side first, builds the 0-based
+ *    old-to-new vector (-1 in unused slots), compares it with the compiler's
+ *    vector when one is given (SIPGPU_EPERM if they differ, as the
+ *    interpreter's check fails), and queues the permute of the right-hand
+ *    block, [1, perm + 1] over its extents, into the left-hand block;
+ *  - sipgpu_worker_contract_to_scalar replaces block_contract_to_scalar_op
+ *    (interpreter.cpp:748-757): pops right then left, waits for the queued
+ *    writers of both blocks, and returns the scalar the interpreter would
+ *    push onto its expression stack.
+ * Host-visible uses: sipgpu_worker_read waits for the queued writer of a
+ * block, then reads it; sipgpu_worker_write (a super-instruction's output)
+ * waits for every queued request on the block, then stores the host data;
+ * sipgpu_worker_flush waits for everything queued (a barrier, a checkpoint,
+ * the end of a program) and, with `writeback`, writes dirty blocks back to
+ * their host homes.  A queued request's failure is reported by the flush.
+ * One thread drives a worker, as one interpreter thread drives the SIP. */
+typedef struct sipgpu_worker_s *sipgpu_worker_t;
+
+typedef struct {
+    unsigned long long contractions, permutes, scalars, reads, writes;
+    unsigned long long waits, flushes, queued;
+} sipgpu_worker_stats;
+
+int sipgpu_worker_create(sipgpu_cache_t cache, sipgpu_server_t server, sipgpu_worker_t *worker);
+int sipgpu_worker_destroy(sipgpu_worker_t worker);
+int sipgpu_worker_push_selector(sipgpu_worker_t worker, unsigned long long key, int rank,
+                                const int *index_ids, const int *extents);
+int sipgpu_worker_stack_depth(sipgpu_worker_t worker, int *depth);
+int sipgpu_worker_contract(sipgpu_worker_t worker, unsigned long long dkey, int drank,
+                           const int *dindex_ids, const int *dextents, int accumulate,
+                           int moduli, unsigned long long *ticket);
+int sipgpu_worker_permute(sipgpu_worker_t worker, const int *compiler_perm,
+                          unsigned long long *ticket);
+int sipgpu_worker_contract_to_scalar(sipgpu_worker_t worker, double *value);
+int sipgpu_worker_read(sipgpu_worker_t worker, unsigned long long key, double *out);
+int sipgpu_worker_write(sipgpu_worker_t worker, unsigned long long key, int rank,
+                        const int *extents, const double *host);
+int sipgpu_worker_flush(sipgpu_worker_t worker, int writeback);
+int sipgpu_worker_get_stats(sipgpu_worker_t worker, sipgpu_worker_stats *stats);
+
+/* Interpreter::permute_rhs_to_lhs's vector (interpreter.cpp:2049-2084), host
+ * only: perm[j] = i where the right-hand side's index id j (of
+ * SIPGPU_MAX_RANK, unused ids -1) is the left-hand side's id i; slots from
+ * lrank on are -1.  SIPGPU_EPERM for an id of the left-hand side that the
+ * right-hand side's first lrank ids lack ("illegal transpose"). */
+int sipgpu_permute_rhs_to_lhs(int lrank, const int *lhs_ids, const int *rhs_ids, int *perm);
 
 /* ------------------------------------------------------------------------
  * Graphs of CC iterations (no reference counterpart: the SIP re-issues every
