@@ -11,7 +11,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libsipgpu.so")
-SOURCES = ["plan.cpp", "permute.hip", "contract.hip", "calib.hip", "ozaki.hip", "ozaki_big.hip", "packed.hip", "synth.hip", "energy.hip", "blockops.hip", "runtime.cpp", "serve.cpp", "graph.cpp", "dispatch.cpp"]
+SOURCES = ["plan.cpp", "permute.hip", "contract.hip", "calib.hip", "ozaki.hip", "ozaki_big.hip", "packed.hip", "synth.hip", "energy.hip", "blockops.hip", "runtime.cpp", "serve.cpp", "graph.cpp", "dispatch.cpp", "triples.hip", "triples.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 FLAGS = [
@@ -28,7 +28,7 @@ FLAGS = [
 # between register banks around loops nor splits the budget between banks.
 VGPR_FORM = ["-mllvm", "-amdgpu-mfma-vgpr-form=1"]
 PER_FILE = {"calib.hip": VGPR_FORM, "contract.hip": VGPR_FORM, "ozaki.hip": VGPR_FORM,
-            "packed.hip": VGPR_FORM}
+            "packed.hip": VGPR_FORM, "triples.hip": VGPR_FORM}
 NOISE = "is not a recognized feature for this target"
 
 

@@ -162,4 +162,52 @@ SliceArgs make_slice_args(const SlicePlan &p);
 hipError_t launch_slice(const SliceArgs &a, bool insert, double *tens, double *slc,
                         hipStream_t st);
 
+// ---------------------------------------------------------------- triples --
+// The (T) energy with W kept on chip (triples.hip).  Indices are cut into
+// 5-blocks within their segments; a box is one 5-block per W index.
+constexpr int kTriplesBox = 5;
+constexpr int kTriplesSums = 4096; // partial sums of the final reduction
+struct TriplesBlk {
+    int seg;    // segment
+    int off;    // offset of the 5-block inside the segment
+    int ext;    // extent, 1..5
+    int gstart; // global orbital index of its first element
+};
+// Packs PA (side 0: sets (x, y1, y2) over vblk x oblk x oblk) or PB (side 1:
+// sets (x2, x3, y3) over vblk x vblk x oblk) from the blocks of T(a,d,i,j),
+// V1(b,c,d,k) and V2(l,c,j,k), given as device pointer tables in segment
+// order (first index slowest).
+struct TriplesPackArgs {
+    double *out;
+    long long total; // doubles of the panel array
+    int side, nkp, kd, kl;
+    int n1, n2;      // 5-block counts of the second and third set index
+    const TriplesBlk *blk0, *blk1, *blk2;
+    int nv, no;      // segment counts
+    const int *vext, *oext;             // segment extents
+    const int *vseg_of, *oseg_of;       // segment of each global index
+    const int *vseg_start, *oseg_start; // global index of each segment's start
+    const double *const *T, *const *V1, *const *V2;
+};
+struct TriplesArgs {
+    const double *PA, *PB;
+    int nkp, nvb, nob;
+    const TriplesBlk *vblk, *oblk;
+    const double *e_vir, *e_occ;
+    double *partial;
+    // global mode: linear box ids [box0, box0 + nbox) of the super-box order
+    long long box0, nbox, per_xcd;
+    int super[6], nblk[6];
+    // tuple mode: the boxes first[d] .. first[d] + count[d] - 1
+    int tuple_mode;
+    int first[6], count[6];
+    double *wout;   // tuple mode, optional: W of the tuple (column-major)
+    int wdims[6];
+};
+hipError_t launch_triples_pack(const TriplesPackArgs &a, hipStream_t s);
+hipError_t launch_triples_boxes(const TriplesArgs &a, long long grid, hipStream_t s);
+// out[0] += factor * sum of partial[0 .. n) in a fixed order (sums: kTriplesSums doubles)
+hipError_t launch_triples_reduce(const double *partial, long long n, double *sums, double factor,
+                                 double *out, hipStream_t s);
+
 } // namespace sipgpu

@@ -61,7 +61,9 @@ EXPORTED = [
     "sipgpu_worker_push_selector", "sipgpu_worker_stack_depth", "sipgpu_worker_contract",
     "sipgpu_worker_permute", "sipgpu_worker_contract_to_scalar", "sipgpu_worker_read",
     "sipgpu_worker_write", "sipgpu_worker_flush", "sipgpu_worker_get_stats",
-    "sipgpu_permute_rhs_to_lhs",
+    "sipgpu_permute_rhs_to_lhs", "sipgpu_triples_create", "sipgpu_triples_info",
+    "sipgpu_triples_pack", "sipgpu_triples_run", "sipgpu_triples_energy", "sipgpu_triples_tuple",
+    "sipgpu_triples_destroy",
 ]
 
 
@@ -235,6 +237,16 @@ def lib():
         L.sipgpu_worker_flush.argtypes = [_vp, ctypes.c_int]
         L.sipgpu_worker_get_stats.argtypes = [_vp, ctypes.POINTER(WorkerStats)]
         L.sipgpu_permute_rhs_to_lhs.argtypes = [ctypes.c_int, _ip, _ip, _ip]
+        L.sipgpu_triples_create.argtypes = [ctypes.c_int, _ip, ctypes.c_int, _ip, _vp, _vp, _vp,
+                                            _vp, _vp, ctypes.POINTER(_vp)]
+        L.sipgpu_triples_info.argtypes = [_vp, ctypes.POINTER(ctypes.c_longlong),
+                                          ctypes.POINTER(ctypes.c_double),
+                                          ctypes.POINTER(ctypes.c_size_t)]
+        L.sipgpu_triples_pack.argtypes = [_vp, _vp]
+        L.sipgpu_triples_run.argtypes = [_vp, ctypes.c_longlong, ctypes.c_longlong, _vp]
+        L.sipgpu_triples_energy.argtypes = [_vp, ctypes.c_double, _vp, _vp]
+        L.sipgpu_triples_tuple.argtypes = [_vp, _ip, ctypes.c_double, _vp, _vp, _vp]
+        L.sipgpu_triples_destroy.argtypes = [_vp]
         _pp = ctypes.POINTER(ctypes.c_void_p)
         L.sipgpu_energy_denominator.argtypes = [_vp, ctypes.c_int, _ip, _vp, _pp, _ip]
         L.sipgpu_energy_sum.argtypes = [_vp, ctypes.c_int, _ip, _vp, _vp, _pp, _ip,

@@ -205,3 +205,68 @@ class TupleEnergy:
             self.aux.synchronize()
             self.aux.destroy()
             self.aux = None
+
+
+class FusedTriples:
+    """The whole unrestricted (T) sum with W kept on chip
+    (sipgpu_triples_*): one workgroup per 5^6 box of W runs the six GEMMs of
+    the box on FP64 MFMA, adds them in LDS, and reduces the box's energy;
+    W is never written.  The operands are packed once into two panel arrays.
+
+        ft = FusedTriples(w)        # plan + pack (~4.3 GB at o = 30, v = 240)
+        ft.run()                    # all box slots, or ft.run(box0, nbox)
+        ft.energy(out_ptr)          # out += factor * sum of the partials
+    """
+
+    def __init__(self, w, stream=None):
+        import ctypes
+        to_device(w)
+        self.w = w
+        nv, no = len(w["v_segs"]), len(w["o_segs"])
+
+        def table(name, counts):
+            arr = w[name]
+            ptrs = [arr.addr(t) for t in itertools.product(*[range(c) for c in counts])]
+            return (ctypes.c_void_p * len(ptrs))(*ptrs)
+
+        self._T = table("T", (nv, nv, no, no))
+        self._V1 = table("V1", (nv, nv, nv, no))
+        self._V2 = table("V2", (no, nv, no, no))
+        vs, os_ = sg._ints(w["v_segs"]), sg._ints(w["o_segs"])
+        p = ctypes.c_void_p()
+        sg._check(sg.lib().sipgpu_triples_create(
+            nv, sg._iptr(vs), no, sg._iptr(os_), ctypes.cast(self._T, ctypes.c_void_p),
+            ctypes.cast(self._V1, ctypes.c_void_p), ctypes.cast(self._V2, ctypes.c_void_p),
+            w["e_occ_dev"].ptr, w["e_vir_dev"].ptr, ctypes.byref(p)))
+        self.handle = p.value
+        slots, fpb, pbytes = ctypes.c_longlong(0), ctypes.c_double(0), ctypes.c_size_t(0)
+        sg._check(sg.lib().sipgpu_triples_info(self.handle, ctypes.byref(slots), ctypes.byref(fpb),
+                                               ctypes.byref(pbytes)))
+        self.slots, self.mfma_flop_per_box, self.panel_bytes = slots.value, fpb.value, pbytes.value
+        v, o = sum(w["v_segs"]), sum(w["o_segs"])
+        # the algorithmic work of the whole sum: 12 terms of 2 x v^3 o^3 (v or o)
+        self.flops = 12.0 * (v ** 3) * (o ** 3) * (v + o)
+        self.pack(stream)
+
+    def pack(self, stream=None):
+        sg._check(sg.lib().sipgpu_triples_pack(self.handle, stream.handle if stream else None))
+
+    def run(self, box0=0, nbox=None, stream=None):
+        if nbox is None:
+            nbox = self.slots - box0
+        sg._check(sg.lib().sipgpu_triples_run(self.handle, int(box0), int(nbox),
+                                              stream.handle if stream else None))
+
+    def energy(self, out, factor=1.0, stream=None):
+        sg._check(sg.lib().sipgpu_triples_energy(self.handle, float(factor), out,
+                                                 stream.handle if stream else None))
+
+    def tuple(self, tup, out, w_out=None, factor=1.0, stream=None):
+        t = sg._ints(tup)
+        sg._check(sg.lib().sipgpu_triples_tuple(self.handle, sg._iptr(t), float(factor), w_out,
+                                                out, stream.handle if stream else None))
+
+    def destroy(self):
+        if self.handle:
+            sg._check(sg.lib().sipgpu_triples_destroy(self.handle))
+            self.handle = None

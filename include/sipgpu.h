@@ -566,6 +566,46 @@ int sipgpu_worker_get_stats(sipgpu_worker_t worker, sipgpu_worker_stats *stats);
 int sipgpu_permute_rhs_to_lhs(int lrank, const int *lhs_ids, const int *rhs_ids, int *perm);
 
 /* ------------------------------------------------------------------------
+ * The (T) triples energy with W kept on chip.  Replaces the per-tuple
+ * sequence of rccsdpt_aaa.sialx (:2164-2174: the W block built from its
+ * twelve contractions, energy_denominator_rhf, then the contraction of the
+ * divided block with the undivided one into esum) for every block tuple
+ * (A, B, C, I, J, K) of the unrestricted sum:
+ *   W(a,b,c,i,j,k) = sum over the 6 permutations P of the pairs (a,i),
+ *     (b,j), (c,k) of  sum_d V1(x2,x3,d,y3) T(x1,d,y1,y2)
+ *                    + sum_l T(x3,x2,y3,l) V2(l,x1,y2,y1)
+ *   E += factor * sum W * (W / D),  D = e_i + e_j + e_k - e_a - e_b - e_c.
+ * T(a,d,i,j), V1(b,c,d,k) and V2(l,c,j,k) (V2 holding -V) are given as
+ * tables of device block pointers in segment order, first index slowest
+ * (T: nv*nv*no*no entries; V1: nv*nv*nv*no; V2: no*nv*no*no); e_occ and
+ * e_vir are device arrays of all occupied / virtual orbital energies.  W is
+ * computed in 5^6 boxes held in LDS and is never written to HBM.
+ *
+ * sipgpu_triples_create plans and allocates the packed operand panels
+ * (sipgpu_triples_info: their bytes, ~4.3 GB for o = 30, v = 240) and one
+ * energy partial per box slot; sipgpu_triples_pack (re)packs them from the
+ * blocks and zeroes the partials; sipgpu_triples_run computes the partials of
+ * box slots [box0, box0 + nbox) (all `slots` slots cover the whole sum);
+ * sipgpu_triples_energy adds factor * (sum of all partials, in slot order)
+ * to *out (a device double).  sipgpu_triples_tuple computes one tuple's
+ * energy into *out (+=) and, if w_out is not null, writes its W block
+ * (column-major a, b, c, i, j, k).  All calls on one plan must be ordered
+ * (one stream). */
+typedef struct sipgpu_triples_s *sipgpu_triples_t;
+int sipgpu_triples_create(int nv, const int *v_seg, int no, const int *o_seg,
+                          const double *const *T, const double *const *V1,
+                          const double *const *V2, const double *e_occ, const double *e_vir,
+                          sipgpu_triples_t *plan);
+int sipgpu_triples_info(sipgpu_triples_t plan, long long *slots, double *flops_per_box,
+                        size_t *panel_bytes);
+int sipgpu_triples_pack(sipgpu_triples_t plan, sipgpu_stream_t s);
+int sipgpu_triples_run(sipgpu_triples_t plan, long long box0, long long nbox, sipgpu_stream_t s);
+int sipgpu_triples_energy(sipgpu_triples_t plan, double factor, double *out, sipgpu_stream_t s);
+int sipgpu_triples_tuple(sipgpu_triples_t plan, const int *tuple, double factor, double *w_out,
+                         double *out, sipgpu_stream_t s);
+int sipgpu_triples_destroy(sipgpu_triples_t plan);
+
+/* ------------------------------------------------------------------------
  * Graphs of CC iterations (no reference counterpart: the SIP re-issues every
  * super-instruction each iteration).  Everything the library enqueues on `s`
  * between begin and end -- batch executions and their stages, single

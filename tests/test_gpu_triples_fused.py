@@ -1,0 +1,174 @@
+"""GPU parity of the (T) energy with W kept on chip (sipgpu_triples_*).
+
+Against the oracle's per-tuple sequence (every contraction of W assigned into
+a temporary and added, energy_denominator_rhf, then fcontract of the divided
+block with W): each tuple's W (written out for the test) within 1e-13
+relative Frobenius error, each tuple's energy and the whole unrestricted sum
+within 1e-13.  Segments of 4, 3, 5 and 2, 3 leave partial 5-blocks, so the
+box masking is exercised.  At the full size of the triples configuration
+(segments of 20 and 10) one 20^6 tuple's energy is checked against the
+twelve-batch FP64 path, and W on sampled elements against direct dot
+products of the twelve terms.
+"""
+import itertools
+
+import numpy as np
+import pytest
+
+from cases import rel_fro
+from checker import Oracle
+
+from aces4_amd import sipgpu as sg
+from aces4_amd import triples as tp
+
+pytestmark = pytest.mark.gpu
+o = Oracle()
+
+
+@pytest.fixture(scope="module")
+def small():
+    w = tp.triples(o_segs=(2, 3), v_segs=(4, 3, 5), seed=5)
+    tp.to_device(w)
+    host = {k: w[k].dev.download() for k in ("V1", "V2", "T")}
+    ft = tp.FusedTriples(w)
+    yield w, host, ft
+    ft.destroy()
+
+
+def oracle_tuple(w, host, tup):
+    dims = tp.tuple_dims(w, tup)
+    W = np.zeros(int(np.prod(dims)))
+    for ptrn, pairs in tp.tuple_entries(w, tup):
+        for ln, lb, rn, rb in pairs:
+            L, R = w[ln], w[rn]
+            tmp, e = o.contract(ptrn, L.download_block(lb, host[ln]), L.dims(lb),
+                                R.download_block(rb, host[rn]), R.dims(rb), dims)
+            assert e == 0
+            W = W + tmp
+    segs = {"v": w["v_segs"], "o": w["o_segs"]}
+    eps = []
+    for lab, s in zip(tp.W_LABELS, tup):
+        kind = w["kind"][lab]
+        base = w["e_vir"] if kind == "v" else w["e_occ"]
+        off = tp.seg_offset(segs[kind], s)
+        eps.append(base[off:off + segs[kind][s]])
+    t = o.energy_denominator(dims, W, eps, tp.W_SIGNS)
+    return W, o.fcontract(t, W)
+
+
+TUPLES = [(0, 1, 2, 0, 1, 0), (1, 1, 1, 1, 1, 1), (2, 0, 1, 1, 0, 1), (2, 2, 2, 0, 0, 0)]
+
+
+@pytest.mark.parametrize("tup", TUPLES, ids=[str(t) for t in TUPLES])
+def test_fused_tuple_w_and_energy_vs_oracle(small, tup):
+    w, host, ft = small
+    print("fused tuple", tup, flush=True)
+    dims = tp.tuple_dims(w, tup)
+    wbuf = sg.DeviceArray(int(np.prod(dims)))
+    out = sg.DeviceArray(1)
+    out.zero()
+    ft.tuple(tup, out.ptr, w_out=wbuf.ptr)
+    sg.synchronize()
+    W_ref, E_ref = oracle_tuple(w, host, tup)
+    assert rel_fro(wbuf.download(), W_ref) <= 1e-13
+    E = out.download()[0]
+    assert abs(E - E_ref) <= 1e-13 * abs(E_ref)
+    wbuf.free()
+    out.free()
+
+
+def test_fused_whole_sum_vs_oracle(small):
+    """The unrestricted sum over all 216 tuples through the box slots, split
+    into two runs, equals the sum of the oracle's tuple energies; a second
+    identical sweep gives the same bits (fixed reduction order)."""
+    w, host, ft = small
+    nv, no = len(w["v_segs"]), len(w["o_segs"])
+    E_ref = 0.0
+    for tup in itertools.product(range(nv), range(nv), range(nv), range(no), range(no), range(no)):
+        E_ref += oracle_tuple(w, host, tup)[1]
+    outs = []
+    for _ in range(2):
+        out = sg.DeviceArray(1)
+        out.zero()
+        ft.pack()
+        half = ft.slots // 3
+        ft.run(0, half)
+        ft.run(half, ft.slots - half)
+        ft.energy(out.ptr)
+        sg.synchronize()
+        outs.append(out.download()[0])
+        out.free()
+    assert abs(outs[0] - E_ref) <= 1e-13 * abs(E_ref)
+    assert outs[0] == outs[1]
+
+
+def test_fused_full_size_tuple():
+    """One 20^6 tuple of the triples configuration (o = 30 in 20, 10; v = 240
+    in 12 x 20): its energy against the twelve-batch FP64 path (oracle-pinned
+    at small sizes) within 1e-13, and 64 sampled W elements against direct
+    dot products of the twelve terms from the blocks."""
+    w = tp.triples()
+    tp.to_device(w)
+    ft = tp.FusedTriples(w)
+    tup = (0, 1, 2, 0, 0, 0)
+    dims = tp.tuple_dims(w, tup)
+    n = int(np.prod(dims))
+    wbuf = sg.DeviceArray(n)
+    out, out2 = sg.DeviceArray(1), sg.DeviceArray(1)
+    out.zero()
+    out2.zero()
+    print("fused full tuple", flush=True)
+    ft.tuple(tup, out.ptr, w_out=wbuf.ptr)
+    sg.synchronize()
+    # sample W before the batch path overwrites the buffer
+    rng = np.random.default_rng(9)
+    picks = [tuple(int(rng.integers(0, d)) for d in dims) for _ in range(64)]
+    flat = [int(np.ravel_multi_index(p, dims, order="F")) for p in picks]
+    got = np.array([wbuf.download_range(f, 1)[0] for f in flat])
+    te = tp.TupleEnergy(w, tup, wbuf.ptr, out2.ptr)
+    te.execute()
+    sg.synchronize()
+    E, E2 = out.download()[0], out2.download()[0]
+    assert abs(E - E2) <= 1e-13 * abs(E2)
+    te.destroy()
+    ft.destroy()
+    # direct dot products of the twelve terms
+    segs = {"v": w["v_segs"], "o": w["o_segs"]}
+    cache = {}
+
+    def elem(name, glob):
+        """Element of array `name` at global indices (one per dimension)."""
+        arr = w[name]
+        blk, loc = [], []
+        for d, g in enumerate(glob):
+            s = 0
+            while g >= arr.segs[d][s]:
+                g -= arr.segs[d][s]
+                s += 1
+            blk.append(s)
+            loc.append(g)
+        key = (name, tuple(blk))
+        if key not in cache:
+            cache[key] = arr.download_block(tuple(blk)).reshape(arr.dims(tuple(blk)), order="F")
+        return cache[key][tuple(loc)]
+
+    base = [tp.seg_offset(segs[w["kind"][lab]], s) for lab, s in zip(tp.W_LABELS, tup)]
+    want = []
+    v, o_ = sum(w["v_segs"]), sum(w["o_segs"])
+    for p in picks:
+        g = {lab: base[q] + p[q] for q, lab in enumerate(tp.W_LABELS)}
+        tot = 0.0
+        for perm in itertools.permutations(range(3)):
+            (x1, y1), (x2, y2), (x3, y3) = (tp.PAIRS[q] for q in perm)
+            t1 = sum(elem("V1", (g[x2], g[x3], d, g[y3])) * elem("T", (g[x1], d, g[y1], g[y2]))
+                     for d in range(v))
+            t2 = sum(elem("V2", (l, g[x3], g[y2], g[y3])) * elem("T", (g[x1], g[x2], g[y1], l))
+                     for l in range(o_))
+            tot += t1 + t2
+        want.append(tot)
+    assert rel_fro(got, np.array(want)) <= 1e-13
+    wbuf.free()
+    out.free()
+    out2.free()
+    for k in ("V1", "V2", "T"):
+        w[k].dev.free()
