@@ -77,59 +77,71 @@ __global__ __launch_bounds__(kPermThreads) void permute_kernel(PermArgs a) {
     const double *__restrict__ in = a.in;
     double *__restrict__ out = a.out;
     double val[kPermItems];
+    // Element e = tid + i * kPermThreads of the tile, as (v, u) = (e / tu,
+    // e % tu) or (u, v) = (e / tv, e % tv): one division per thread, then
+    // the stride kPermThreads is added in mixed radix (no division per item).
+    const int qu = kPermThreads / tu, ru = kPermThreads - qu * tu;
+    const int vu0 = tid / tu, uu0 = tid - vu0 * tu;
     if (a.row_mode) {
         // u is contiguous in both the input and the output.
+        int vv = vu0, uu = uu0;
 #pragma unroll
         for (int i = 0; i < kPermItems; ++i) {
-            const int e = tid + i * kPermThreads;
-            if (e < n) {
-                const int vv = e / tu, uu = e - vv * tu;
-                val[i] = in[u0 + uu + sh_inV[vv]];
-            }
+            if (tid + i * kPermThreads < n) val[i] = in[u0 + uu + sh_inV[vv]];
+            uu += ru;
+            vv += qu;
+            if (uu >= tu) { uu -= tu; ++vv; }
         }
+        vv = vu0;
+        uu = uu0;
 #pragma unroll
         for (int i = 0; i < kPermItems; ++i) {
-            const int e = tid + i * kPermThreads;
-            if (e < n) {
-                const int vv = e / tu, uu = e - vv * tu;
-                out[u0 + uu + sh_outV[vv]] = val[i];
-            }
+            if (tid + i * kPermThreads < n) out[u0 + uu + sh_outV[vv]] = val[i];
+            uu += ru;
+            vv += qu;
+            if (uu >= tu) { uu -= tu; ++vv; }
         }
         return;
     }
     const int pitch = a.pitch;
+    {
+        int vv = vu0, uu = uu0;
 #pragma unroll
-    for (int i = 0; i < kPermItems; ++i) {
-        const int e = tid + i * kPermThreads;
-        if (e < n) {
-            const int vv = e / tu, uu = e - vv * tu;
-            val[i] = in[u0 + uu + sh_inV[vv]];
+        for (int i = 0; i < kPermItems; ++i) {
+            if (tid + i * kPermThreads < n) val[i] = in[u0 + uu + sh_inV[vv]];
+            uu += ru;
+            vv += qu;
+            if (uu >= tu) { uu -= tu; ++vv; }
         }
-    }
+        vv = vu0;
+        uu = uu0;
 #pragma unroll
-    for (int i = 0; i < kPermItems; ++i) {
-        const int e = tid + i * kPermThreads;
-        if (e < n) {
-            const int vv = e / tu, uu = e - vv * tu;
-            tile[vv * pitch + uu] = val[i];
+        for (int i = 0; i < kPermItems; ++i) {
+            if (tid + i * kPermThreads < n) tile[vv * pitch + uu] = val[i];
+            uu += ru;
+            vv += qu;
+            if (uu >= tu) { uu -= tu; ++vv; }
         }
     }
     __syncthreads();
+    const int qv = kPermThreads / tv, rv = kPermThreads - qv * tv;
+    const int uv0 = tid / tv, vv0 = tid - uv0 * tv;
+    int uu = uv0, vv = vv0;
 #pragma unroll
     for (int i = 0; i < kPermItems; ++i) {
-        const int e = tid + i * kPermThreads;
-        if (e < n) {
-            const int uu = e / tv, vv = e - uu * tv;
-            val[i] = tile[vv * pitch + uu];
-        }
+        if (tid + i * kPermThreads < n) val[i] = tile[vv * pitch + uu];
+        vv += rv;
+        uu += qv;
+        if (vv >= tv) { vv -= tv; ++uu; }
     }
+    uu = uv0;
+    vv = vv0;
 #pragma unroll
     for (int i = 0; i < kPermItems; ++i) {
-        const int e = tid + i * kPermThreads;
-        if (e < n) {
-            const int uu = e / tv, vv = e - uu * tv;
-            out[sh_outU[uu] + sh_outV[vv]] = val[i];
-        }
+        if (tid + i * kPermThreads < n) out[sh_outU[uu] + sh_outV[vv]] = val[i];
+        vv += rv;
+        uu += qv;
+        if (vv >= tv) { vv -= tv; ++uu; }
     }
 }
 

@@ -180,29 +180,35 @@ __global__ __launch_bounds__(kThreads) void triples_box_kernel(TriplesArgs a) {
     }
 
     const int nkp = a.nkp;
-    for (int g = 0; g < 6; ++g) {
-        // P = the g-th permutation of (0, 1, 2)
+    // operand pointers of permutation g: its row set (x1, y1, y2) in PA and
+    // column set (x2, x3, y3) in PB, at this wave's first 16-row block
+    auto panels = [&](int g, const f64x2 *&pa, const f64x2 *&pb) {
         const int p0 = kPerm[g][0], p1 = kPerm[g][1], p2 = kPerm[g][2];
-        const long long rowset =
-            ((long long)bc[p0] * a.nob + bc[3 + p0]) * a.nob + bc[3 + p1];
-        const long long colset =
-            ((long long)bc[p1] * a.nvb + bc[p2]) * a.nob + bc[3 + p2];
-        const f64x2 *pa = reinterpret_cast<const f64x2 *>(a.PA) +
-                          ((rowset * 8 + wm * 4) * nkp) * 64 + lane;
-        const f64x2 *pb = reinterpret_cast<const f64x2 *>(a.PB) +
-                          ((colset * 8 + wn * 4) * nkp) * 64 + lane;
+        const long long rowset = ((long long)bc[p0] * a.nob + bc[3 + p0]) * a.nob + bc[3 + p1];
+        const long long colset = ((long long)bc[p1] * a.nvb + bc[p2]) * a.nob + bc[3 + p2];
+        pa = reinterpret_cast<const f64x2 *>(a.PA) + ((rowset * 8 + wm * 4) * nkp) * 64 + lane;
+        pb = reinterpret_cast<const f64x2 *>(a.PB) + ((colset * 8 + wn * 4) * nkp) * 64 + lane;
+    };
+    f64x2 fa[2][4], fb[2][4];
+    auto load = [&](int st, const f64x2 *pa, const f64x2 *pb, int kp) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) fa[st][i] = pa[((long long)i * nkp + kp) * 64];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) fb[st][j] = pb[((long long)j * nkp + kp) * 64];
+    };
+    const f64x2 *pa, *pb;
+    panels(0, pa, pb);
+    load(0, pa, pb, 0);
+    for (int g = 0; g < 6; ++g) {
+        // the next permutation's operands (the last one reloads its own,
+        // which nothing reads)
+        const f64x2 *na, *nb;
+        panels(g < 5 ? g + 1 : g, na, nb);
         f64x4 acc[4][4];
 #pragma unroll
         for (int i = 0; i < 4; ++i)
 #pragma unroll
             for (int j = 0; j < 4; ++j) acc[i][j] = f64x4{0.0, 0.0, 0.0, 0.0};
-        f64x2 fa[2][4], fb[2][4];
-        auto load = [&](int st, int kp) {
-#pragma unroll
-            for (int i = 0; i < 4; ++i) fa[st][i] = pa[((long long)i * nkp + kp) * 64];
-#pragma unroll
-            for (int j = 0; j < 4; ++j) fb[st][j] = pb[((long long)j * nkp + kp) * 64];
-        };
         auto mma = [&](int st) {
 #pragma unroll
             for (int h = 0; h < 2; ++h)
@@ -214,18 +220,23 @@ __global__ __launch_bounds__(kThreads) void triples_box_kernel(TriplesArgs a) {
                                                                          acc[i][j], 0, 0, 0);
         };
         // nkp is even: two k-pairs per trip, the next one's loads in flight
-        // while the current one's MFMAs run (the last trip loads kp = 0
-        // again, which nothing reads)
-        load(0, 0);
+        // while the current one's MFMAs run; the last trip fetches the next
+        // permutation's first k-pair, which then lands during the adds below
         for (int kp = 0; kp < nkp; kp += 2) {
-            load(1, kp + 1);
+            load(1, pa, pb, kp + 1);
             mma(0);
-            load(0, kp + 2 < nkp ? kp + 2 : 0);
+            if (kp + 2 < nkp)
+                load(0, pa, pb, kp + 2);
+            else
+                load(0, na, nb, 0);
             mma(1);
         }
+        pa = na;
+        pb = nb;
         // the previous P's adds into the box are complete
         __syncthreads();
         // add the product into the box: row (x1, y1, y2), column (x2, x3, y3)
+        const int p0 = kPerm[g][0], p1 = kPerm[g][1], p2 = kPerm[g][2];
         const int sx1 = box_stride(p0), sy1 = box_stride(3 + p0), sy2 = box_stride(3 + p1);
         const int sx2 = box_stride(p1), sx3 = box_stride(p2), sy3 = box_stride(3 + p2);
         int coff[4];

@@ -1,19 +1,25 @@
 """Benchmark: FP64-equivalent TFLOP/s of the SIP block-contraction hot path.
 
-Workload (one step): the ring term R(a,i,b,j) += sum_{k,c} W(k,b,c,j) T(a,i,c,k)
-with o = 30 (segments 15,15) and v = 150 (segments 30,30,30,30,15,15): 144
-result blocks, 1728 uneven block pairs, 1.8225e11 flop, all operands and
-results resident in HBM.  A step is one planned batch: a packing pass that
-lays every operand block out K-contiguous (the index permutations applied
-once per block), then one grouped FP64-MFMA launch over the panels with the
-result permutation fused into its scatter.
+Headline workload: the whole unrestricted (T) triples sum of the triples
+configuration -- o = 30 (segments 20, 10), v = 240 (12 segments of 20), every
+block tuple (A, B, C, I, J, K), 1.2093e15 flop of W contractions -- with W
+kept on chip: one workgroup per 5^6 box of W runs the twelve contractions of
+the box (six GEMMs over d and l) on FP64 MFMA, adds them in LDS, divides by
+the orbital-energy denominators and reduces the energy; W is never written.
+The box slots are cut into `--steps` equal launches, so the K timed steps are
+exactly one sweep of the whole sum (no sampling); the operands are packed
+once before the timed region (its time is reported beside).
+
+Beside it, in the same JSON line: the split-precision path against the plain
+path on the triples tuples (each shape class), the ring term with its fused
+permute-in-load batch (the earlier headline), the block-sparse ladder, the
+serving stream, permutes, block operations, graphs, the autotuner, the
+single-contraction latency, and the reference's CPU path timed on bounded
+samples of the triples and ring workloads.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W]
 
-Prints one JSON line.  Besides the throughput it reports the GEMM kernel's
-roofline position (its average launch duration, the product stage launched
-alone between HIP events on the launch stream), per-contraction latency of single unbatched contractions, and the
-reference's CPU path timed on a bounded sample of the same block pairs.
+Prints one JSON line.
 """
 import argparse
 import json
@@ -33,13 +39,15 @@ from aces4_amd import workloads as wl  # noqa: E402
 FP64_PEAK_TFLOPS = 78.6  # MI355X dense FP64 matrix peak
 HBM_PEAK_GBS = 8000.0    # MI355X HBM3E peak
 PROFILE_TRAFFIC = os.path.join(ROOT, "profiles", "ring_gemm_traffic.txt")
+PROFILE_TRIPLES_TRAFFIC = os.path.join(ROOT, "profiles", "triples_box_traffic.txt")
 
 
 def parse():
     p = argparse.ArgumentParser()
     p.add_argument("--gpus", type=int, default=1)
-    p.add_argument("--steps", type=int, default=50)
-    p.add_argument("--warmup", type=int, default=5)
+    p.add_argument("--steps", type=int, default=20)
+    p.add_argument("--warmup", type=int, default=2)
+    p.add_argument("--ring-steps", type=int, default=20)
     p.add_argument("--cpu-seconds", type=float, default=12.0,
                    help="budget for the CPU baseline sample")
     p.add_argument("--no-cpu", action="store_true")
@@ -116,11 +124,11 @@ def cpu_baseline(w, budget_s, gpu_result, steps_run):
                            "nthreads=8" if ref else "C restatement, 1 thread"))
 
 
-def traffic_from_profile():
+def traffic_from_profile(path=PROFILE_TRAFFIC):
     """HBM bytes per launch measured by rocprofv3 --pmc (FETCH_SIZE +
     WRITE_SIZE, scaled by the copy-kernel calibration), if recorded."""
     try:
-        for line in open(PROFILE_TRAFFIC):
+        for line in open(path):
             if line.startswith("hbm_bytes_per_launch:"):
                 return float(line.split(":", 1)[1])
     except OSError:
@@ -408,37 +416,96 @@ def graph_bench(reps=20):
     return res
 
 
-def triples_bench(tuples=((0, 1, 2, 0, 0, 0), (3, 4, 5, 0, 0, 0), (6, 7, 8, 0, 0, 0)),
-                  reps=2, moduli=(None, 14)):
-    """Triples (o = 30 in segments 20,10; v = 240 in 12 x 20): full-size W
-    blocks (20^6 elements, 512 MB) from the twelve permuted contractions,
-    then the fused denominator / energy reduction; W is never divided or
-    written back.  FP64-equivalent TFLOP/s over the sampled tuples, and the
-    time the whole unrestricted (T) sum would take at that rate."""
+def triples_fused_bench(steps, warmup):
+    """The headline: the whole unrestricted (T) sum with W on chip.  The box
+    slots are cut into `steps` equal launches, timed between device
+    synchronisations (value) and each between HIP events on its stream
+    (roofline); `warmup` launches of the same chunks run first.  The
+    operand packing runs once before the timed region and is timed apart."""
     from aces4_amd import triples as tp
     w = tp.triples()
     tp.to_device(w)
+    ft = tp.FusedTriples(w)
+    out = sg.DeviceArray(1)
+    e0, e1 = sg.Event(), sg.Event()
+    e0.record()
+    ft.pack()
+    e1.record()
+    e1.synchronize()
+    pack_ms = e0.elapsed_ms(e1)
+    chunk = -(-ft.slots // steps)
+    ranges = [(i * chunk, max(0, min(chunk, ft.slots - i * chunk))) for i in range(steps)]
+    for i in range(warmup):
+        ft.run(*ranges[i % steps])
+    sg.synchronize()
+    evs = [sg.Event() for _ in range(steps + 1)]
+    t0 = time.perf_counter()
+    evs[0].record()
+    for i in range(steps):
+        ft.run(*ranges[i])
+        evs[i + 1].record()
+    sg.synchronize()
+    wall = time.perf_counter() - t0
+    launch_ms = np.array([evs[i].elapsed_ms(evs[i + 1]) for i in range(steps)])
+    out.zero()
+    ft.energy(out.ptr)
+    sg.synchronize()
+    energy = float(out.download()[0])
+    v, o = sum(w["v_segs"]), sum(w["o_segs"])
+    flops = ft.flops  # 12 terms x 2 v^3 o^3 (v or o): every box is full here
+    per_launch = flops * chunk / ft.slots
+    res = {"wall_s": wall, "flops": flops, "energy": energy, "pack_ms": pack_ms,
+           "slots": ft.slots, "boxes_per_launch": chunk,
+           "launch_ms": launch_ms, "flop_per_launch": per_launch,
+           "panel_bytes": ft.panel_bytes,
+           "mfma_flop_issued_per_box": ft.mfma_flop_per_box,
+           "algorithmic_flop_per_box": flops / ft.slots,
+           "v_segments": w["v_segs"], "o_segments": w["o_segs"], "o": o, "v": v}
+    return w, ft, res
+
+
+def triples_tuple_bench(w, ft, tuples, moduli=(14,), reps=2):
+    """Per tuple, for one tuple of every occupied-segment shape class: the
+    fused kernel's energy against the twelve-batch FP64 path (each batch one
+    GEMM task; oracle-pinned at small sizes), and the twelve-batch path's
+    split-precision form (INT8 MFMA over `moduli` moduli) timed against the
+    fused FP64 tuple, with its energy error."""
+    from aces4_amd import triples as tp
     nmax = max(int(np.prod(tp.tuple_dims(w, t))) for t in tuples)
     wbuf = sg.DeviceArray(nmax)
-    # each batch assigns its own partial W, summed into W in batch order
     parts = [sg.DeviceArray(nmax) for _ in range(5)]
     out = sg.DeviceArray(1)
-    v, o = sum(w["v_segs"]), sum(w["o_segs"])
-    whole = 12.0 * (v ** 3) * (o ** 3) * (v + o)
-    res = {"tuples": [list(t) for t in tuples], "w_block_elems": nmax,
-           "whole_T_flop": whole}
-    energies = {}
-    for nm in moduli:
-        tes = [tp.TupleEnergy(w, t, wbuf.ptr, out.ptr, moduli=nm, parts=[p.ptr for p in parts],
-                              tune=True, two_streams=True)
-               for t in tuples]
-        flops = sum(te.flops for te in tes)
+    flops = 0.0
+    fused_e, fused_ms = [], 0.0
+    for t in tuples:
+        # six GEMMs of 2 |W| (v + o) flop each
+        flops += 12.0 * np.prod(tp.tuple_dims(w, t)) * (sum(w["v_segs"]) + sum(w["o_segs"]))
+    # fused tuples (tuple mode)
+    for t in tuples:
         out.zero()
-        for te in tes:
-            te.execute()
+        ft.tuple(t, out.ptr)
         sg.synchronize()
-        energies[nm] = out.download()[0]
-        e0, e1 = sg.Event(), sg.Event()
+        fused_e.append(float(out.download()[0]))
+    e0, e1 = sg.Event(), sg.Event()
+    e0.record()
+    for _ in range(reps):
+        for t in tuples:
+            ft.tuple(t, out.ptr)
+    e1.record()
+    e1.synchronize()
+    fused_ms = e0.elapsed_ms(e1) / reps
+    res = {"tuples": [list(t) for t in tuples], "flop": flops,
+           "fused_fp64": {"ms": fused_ms, "tflops": flops / (fused_ms * 1e-3) / 1e12}}
+    for nm in (None,) + tuple(moduli):
+        tes = [tp.TupleEnergy(w, t, wbuf.ptr, out.ptr, moduli=nm, parts=[p.ptr for p in parts],
+                              tune=True, two_streams=True) for t in tuples]
+        errs = []
+        for te, ef in zip(tes, fused_e):
+            out.zero()
+            te.execute()
+            sg.synchronize()
+            e = float(out.download()[0])
+            errs.append(abs(e - ef) / abs(ef))
         e0.record()
         for _ in range(reps):
             for te in tes:
@@ -446,25 +513,78 @@ def triples_bench(tuples=((0, 1, 2, 0, 0, 0), (3, 4, 5, 0, 0, 0), (6, 7, 8, 0, 0
         e1.record()
         e1.synchronize()
         ms = e0.elapsed_ms(e1) / reps
-        key = "fp64" if nm is None else "split_%d" % nm
-        res[key] = {"ms_per_tuple": ms / len(tes), "tflops_fp64_equiv": flops / (ms * 1e-3) / 1e12,
-                    "flop_per_tuple": flops / len(tes),
-                    "whole_T_seconds_at_this_rate": whole / (flops / (ms * 1e-3)),
-                    "energy_sampled": energies[nm]}
+        key = "batched_fp64" if nm is None else "batched_split_%d" % nm
+        res[key] = {"ms": ms, "tflops": flops / (ms * 1e-3) / 1e12,
+                    "max_tuple_energy_rel_err_vs_fused_fp64": max(errs),
+                    "vs_fused_fp64": fused_ms / ms}
         for te in tes:
             te.destroy()
-    if None in energies and len(energies) > 1:
-        for nm, e in energies.items():
-            if nm is not None:
-                res["split_%d" % nm]["energy_rel_err_vs_fp64"] = abs(e - energies[None]) / abs(
-                    energies[None])
     wbuf.free()
     out.free()
     for p in parts:
         p.free()
-    for k in ("V1", "V2", "T"):
-        w[k].dev.free()
     return res
+
+
+def triples_cpu_baseline(w, budget_s):
+    """The reference's CPU path on a bounded sample of the triples workload:
+    the fourteen contractions of one 20^6 tuple's first permutation
+    (term 1 over the 12 d segments, term 2 over the 2 l segments), each
+    TMP = A*B then W += TMP, with the reference's own kernels (oracle/_ref,
+    -DNO_BLAS, 8 threads) and with the reference's BLAS build restated on
+    the host (oracle/blas_path.py: its permutes, numpy's OpenBLAS dgemm) at 1
+    and at all host threads; each leg stops after `budget_s` seconds."""
+    from aces4_amd import triples as tp
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    from checker import Oracle, RefLib
+    import blas_path
+    o = Oracle()
+    tup = (0, 1, 2, 0, 0, 0)
+    dims = tp.tuple_dims(w, tup)
+    specs = tp.term_labels()[:2]
+    pairs = []
+    for ptrn, prs in tp.tuple_entries(w, tup, specs):
+        for ln, lb, rn, rb in prs:
+            L, R = w[ln], w[rn]
+            pairs.append((ptrn, L.download_block(lb), L.dims(lb), R.download_block(rb), R.dims(rb)))
+    ncpu = os.cpu_count() or 1
+    mn = float(np.prod(dims))
+
+    def flop(ad, bd):
+        # |A| |B| = M K N K and M N = |W|
+        return 2.0 * mn * np.sqrt(float(np.prod(ad)) * float(np.prod(bd)) / mn)
+
+    total = sum(flop(ad, bd) for _, _, ad, _, bd in pairs)
+    out = {"sample": "the %d contractions of tuple %s's first permutation (%.3g flop), "
+                     "TMP = A*B then W += TMP each" % (len(pairs), list(tup), total)}
+
+    def leg(name, contract, add, cores):
+        W = np.zeros(int(mn))
+        t0 = time.perf_counter()
+        fl, n = 0.0, 0
+        for ptrn, A, ad, B, bd in pairs:
+            tmp = contract(ptrn, A, ad, B, bd)
+            add(W, tmp)
+            fl += flop(ad, bd)
+            n += 1
+            if time.perf_counter() - t0 > budget_s:
+                break
+        dt = time.perf_counter() - t0
+        out[name] = {"value": fl / dt / 1e12, "unit": "TFLOP/s", "cores": cores,
+                     "contractions": n, "seconds": dt}
+
+    if RefLib.available():
+        ref = RefLib()
+        leg("reference_no_blas",
+            lambda p, A, ad, B, bd: ref.contract(p, A, ad, B, bd, dims, nthreads=8)[0],
+            lambda W, t: ref.add(dims, W, t), 8)
+    for th in (1, ncpu):
+        with blas_path.blas_threads(th):
+            leg("blas_path_%d_threads" % th,
+                lambda p, A, ad, B, bd: blas_path.contract(o, p, A, ad, B, bd, dims),
+                lambda W, t: np.add(W, t, out=W), th)
+    return out
 
 
 def tune_summary(res):
@@ -492,61 +612,56 @@ PATH_NAME = {"tables": "plain FP64 MFMA, grouped batch, permutations fused into 
                        "step (pack_kernel), then one grouped GEMM over the panels"}
 
 
-def main():
-    a = parse()
-    if a.gpus != 1:
-        print("bench.py measures one GPU: this build runs the contraction path on a single "
-              "MI355X (multi-GPU sharding is a design note only).", file=sys.stderr)
-        sys.exit(2)
-    sg.init(0)
-    # the ladder of the CPU-runnable configuration, tuned first: its choice
-    # is remembered for the graph leg
-    tuning = {"ladder_cfg1": tune_report(wl.ladder())}
+def ring_bench(steps, warmup, cpu_seconds, no_cpu, no_split, latency_samples):
+    """The ring term (o = 30 in 15, 15; v = 150 in 30, 30, 30, 30, 15, 15):
+    the tuned grouped batch timed per step, its GEMM alone per launch, the
+    split path on the same batch, single-contraction latency, and the
+    reference's CPU path on the same block pairs."""
     w = wl.ring()
-    # the headline batch is planned by the autotuner (a planning step, not
-    # part of the timed region)
+    # the batch is planned by the autotuner (a planning step, not timed)
     batch, tres = sg.tune(wl.contractions_for(w))
-    tuning["ring"] = tune_summary(tres)
     flops = batch.flops
-    stream = None
-    for _ in range(a.warmup):
-        batch.execute(stream)
+    for _ in range(warmup):
+        batch.execute()
     sg.synchronize()
-
-    evs = [sg.Event() for _ in range(a.steps + 1)]
-    sg.synchronize()
+    evs = [sg.Event() for _ in range(steps + 1)]
     t0 = time.perf_counter()
-    evs[0].record(stream)
-    for i in range(a.steps):
-        batch.execute(stream)
-        evs[i + 1].record(stream)
+    evs[0].record()
+    for i in range(steps):
+        batch.execute()
+        evs[i + 1].record()
     sg.synchronize()
     wall = time.perf_counter() - t0
-    step_ms = np.array([evs[i].elapsed_ms(evs[i + 1]) for i in range(a.steps)])
+    step_ms = np.array([evs[i].elapsed_ms(evs[i + 1]) for i in range(steps)])
     got = w["D"].dev.download()
-
-    # The GEMM kernel by itself: operands prepared once, then `steps`
-    # launches of the product stage alone, each bracketed by events on the
-    # stream it is launched on.
-    batch.execute_stages(sg.STAGE_PREPARE, stream)
-    batch.execute_stages(sg.STAGE_PRODUCT, stream)
+    # the GEMM kernel by itself: operands prepared once, then `steps`
+    # launches of the product stage alone, each between events on its stream
+    batch.execute_stages(sg.STAGE_PREPARE)
+    batch.execute_stages(sg.STAGE_PRODUCT)
     sg.synchronize()
-    pevs = [sg.Event() for _ in range(a.steps + 1)]
-    pevs[0].record(stream)
-    for i in range(a.steps):
-        batch.execute_stages(sg.STAGE_PRODUCT, stream)
-        pevs[i + 1].record(stream)
+    pevs = [sg.Event() for _ in range(steps + 1)]
+    pevs[0].record()
+    for i in range(steps):
+        batch.execute_stages(sg.STAGE_PRODUCT)
+        pevs[i + 1].record()
     sg.synchronize()
-    launch_ms = np.array([pevs[i].elapsed_ms(pevs[i + 1]) for i in range(a.steps)])
-    avg_launch_s = float(launch_ms.mean()) / 1e3
-    achieved = flops / avg_launch_s / 1e12
-    steps_run = a.warmup + a.steps
+    launch_ms = np.array([pevs[i].elapsed_ms(pevs[i + 1]) for i in range(steps)])
+    achieved = flops / (float(launch_ms.mean()) * 1e-3) / 1e12
     sys.path.insert(0, os.path.join(ROOT, "tests"))
-
-    split = split_bench(w, batch) if not a.no_split else None
-    if split:
-        # fastest split configuration within the 1e-12 error target, against
-        # the plain path's whole step (pack + GEMM) and its GEMM alone
+    res = {"workload": "ring R(a,i,b,j) += W(k,b,c,j) T(a,i,c,k)", "o_segments": [15, 15],
+           "v_segments": [30, 30, 30, 30, 15, 15], "block_pairs": batch.npairs,
+           "result_blocks": batch.ntasks, "flop_per_step": flops, "path": PATH_NAME[batch.kind],
+           "steps": steps, "tflops": flops * steps / wall / 1e12,
+           "ms_per_step": wall / steps * 1e3,
+           "roofline": {"bound": "mfma", "kernel": GEMM_KERNEL[batch.kind],
+                        "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
+                        "frac": achieved / FP64_PEAK_TFLOPS,
+                        "avg_launch_ms": float(launch_ms.mean()),
+                        "avg_step_ms": float(step_ms.mean()),
+                        "traffic": traffic_from_profile()},
+           "tuning": tune_summary(tres)}
+    if not no_split:
+        split = split_bench(w, batch)
         ok = [v for v in split.values() if v["max_block_rel_err_vs_fp64"] <= 1e-12]
         best = max(ok, key=lambda v: v["tflops_fp64_equiv"]) if ok else None
         step_rate = flops / (float(step_ms.mean()) * 1e-3) / 1e12
@@ -557,8 +672,45 @@ def main():
         bt.destroy()
         split["speedup_vs_plain"] = best["tflops_fp64_equiv"] / step_rate if best else None
         split["speedup_vs_plain_gemm_only"] = best["tflops_fp64_equiv"] / achieved if best else None
+        res["split"] = split
+    if latency_samples > 0:
+        res["latency"] = latency_sample(w, latency_samples)
+    if not no_cpu:
+        res["cpu_baseline"] = cpu_baseline(w, cpu_seconds, got, warmup + steps)
+    batch.destroy()
+    for k in ("D", "L", "R"):
+        w[k].dev.free()
+    return res
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print("bench.py measures one GPU: this build runs the contraction path on a single "
+              "MI355X (multi-GPU sharding is a design note only).", file=sys.stderr)
+        sys.exit(2)
+    sg.init(0)
+    # the ladder of the CPU-runnable configuration, tuned first: its choice
+    # is remembered for the graph leg
+    tuning = {"ladder_cfg1": tune_report(wl.ladder())}
+
+    # headline: the whole (T) sum with W on chip
+    w5, ft, tri = triples_fused_bench(a.steps, a.warmup)
+    launch_ms = tri.pop("launch_ms")
+    avg_launch_s = float(launch_ms.mean()) / 1e3
+    achieved = tri["flop_per_launch"] / avg_launch_s / 1e12
+    per_box = traffic_from_profile(PROFILE_TRIPLES_TRAFFIC)
+    traffic = per_box * tri["boxes_per_launch"] if per_box else None
+    shape_classes = [(0, 1, 2, i, j, k) for i in (0, 1) for j in (0, 1) for k in (0, 1)]
+    tri_tuples = None if a.no_triples else triples_tuple_bench(w5, ft, shape_classes)
+    tri_cpu = None if a.no_cpu else triples_cpu_baseline(w5, a.cpu_seconds / 2)
+    ft.destroy()
+    for k in ("V1", "V2", "T"):
+        w5[k].dev.free()
+
+    ring = ring_bench(a.ring_steps, 5, a.cpu_seconds, a.no_cpu, a.no_split, a.latency_samples)
+    tuning["ring"] = ring.pop("tuning")
     sparse = None if a.no_sparse else sparse_bench()
-    trip = None if a.no_triples else triples_bench()
     serving = None
     if not a.no_serve:
         serving = [serving_bench("resident (HBM budget 200 GB)", 200.0, 3000, 2000.0, 8, 64),
@@ -567,50 +719,62 @@ def main():
                    serving_bench("LRU spill (budget 60% of working set)", 8.5, 1500, 4000.0,
                                  8, 40)]
     graphs = None if a.no_graphs else graph_bench()
-    lat = latency_sample(w, a.latency_samples) if a.latency_samples > 0 else None
     perm = permute_bench()
     blockops = blockops_bench()
     cores, model = cpu_info()
-    cpu = None if a.no_cpu else cpu_baseline(w, a.cpu_seconds, got, steps_run)
-    if cpu:
-        cpu["host"] = "%s, %d logical cores" % (model, cores)
-
-    traffic = traffic_from_profile()
     mfma_meas, copy_meas = sg.measure_peaks(1 << 30)
     measured = {"mfma_f64_tflops": mfma_meas, "clock_mhz": sg.measured_clock_mhz(),
                 "copy_gbs": copy_meas, "copy_bytes": 2 * (1 << 30),
                 "note": "MFMA-only FP64 loop over all CUs; streaming copy of 1 GiB (read+write)"}
+    cpu = None
+    if tri_cpu:
+        ref = tri_cpu.get("reference_no_blas")
+        pick = ref or tri_cpu["blas_path_%d_threads" % (os.cpu_count() or 1)]
+        cpu = {"value": pick["value"], "unit": "TFLOP/s", "cores": pick["cores"],
+               "kind": "reference" if ref else "port", "sample": tri_cpu["sample"],
+               "host": "%s, %d logical cores" % (model, cores), "legs": tri_cpu}
     res = {
         "metric": "FP64-equiv TFLOP/s, split vs plain FP64; p50/p99 per-contraction latency",
-        "value": flops * a.steps / wall / 1e12,
+        "value": tri["flops"] / tri["wall_s"] / 1e12,
         "unit": "TFLOP/s",
         "n_gpus": 1,
         "steps": a.steps,
         "warmup": a.warmup,
-        "ms_per_step": wall / a.steps * 1e3,
+        "ms_per_step": tri["wall_s"] / a.steps * 1e3,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
-        "data": "synthetic (SplitMix64-seeded W ~ U(-0.5,0.5), T ~ N(0,1e-2^2))",
-        "config": {"workload": "ring R(a,i,b,j) += W(k,b,c,j) T(a,i,c,k)", "o": 30, "v": 150,
-                   "o_segments": [15, 15], "v_segments": [30, 30, 30, 30, 15, 15],
-                   "block_pairs": batch.npairs, "result_blocks": batch.ntasks,
-                   "flop_per_step": flops, "path": PATH_NAME[batch.kind]},
+        "data": "synthetic (SplitMix64-seeded V ~ N(0,0.1^2), T ~ N(0,1e-2^2), "
+                "e_occ ~ U(-2,-0.3), e_vir ~ U(0.1,3))",
+        "config": {"workload": "(T) triples energy, whole unrestricted sum: W(abcijk) = sum over "
+                               "6 index-pair permutations of V(bcdk)T(adij) - V(lcjk)T(abil), "
+                               "E = sum W*W/D",
+                   "o": tri["o"], "v": tri["v"], "o_segments": tri["o_segments"],
+                   "v_segments": tri["v_segments"], "block_tuples": 12 ** 3 * 2 ** 3,
+                   "flop": tri["flops"], "energy": tri["energy"],
+                   "path": "plain FP64 MFMA, W kept on chip (5^6 boxes in LDS, six GEMMs per box, "
+                           "energy reduced in the epilogue); operands packed once into MFMA-order "
+                           "panels",
+                   "pack_ms": tri["pack_ms"], "slots": tri["slots"],
+                   "boxes_per_step": tri["boxes_per_launch"],
+                   "tflops_including_pack": tri["flops"] / (tri["wall_s"] + tri["pack_ms"] * 1e-3)
+                   / 1e12},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS,
                      "traffic": traffic,
-                     "kernel": GEMM_KERNEL[batch.kind], "avg_launch_ms": avg_launch_s * 1e3,
-                     "avg_step_ms": float(step_ms.mean()),
+                     "kernel": "triples_box_kernel", "avg_launch_ms": avg_launch_s * 1e3,
+                     "flop_per_launch": tri["flop_per_launch"],
+                     "mfma_flop_issued_per_box": tri["mfma_flop_issued_per_box"],
+                     "algorithmic_flop_per_box": tri["algorithmic_flop_per_box"],
                      "measured_ceiling": measured,
                      "frac_of_measured": achieved / mfma_meas if mfma_meas else None},
+        "triples_tuples": tri_tuples,
+        "ring": ring,
         "permute": perm,
         "block_ops": blockops,
-        "split": split,
         "sparse_ladder": sparse,
-        "latency": lat,
         "serving": serving,
-        "triples": trip,
         "graphs": graphs,
         "tuning": tuning,
         "cpu_baseline": cpu,

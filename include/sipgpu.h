@@ -449,10 +449,11 @@ int sipgpu_cache_end_program(sipgpu_cache_t cache, unsigned long long *kept);
  * makes its blocks resident and launches the batch: one grouped FP64
  * launch, or the split-precision stages with their planes in a server-owned
  * scratch buffer per stream (grown, after that stream drains, to the
- * largest batch's need).  Batches alternate over two device streams, so a
- * batch admitted while another runs starts beside it on the device; a batch
- * that reads a block an unfinished batch of the other stream writes (or
- * writes one it reads or writes) waits for it on the device first.  One
+ * largest batch's need).  Two device streams: large batches (>= 1e9 flop)
+ * run in order on the first, and a small batch admitted while the first is
+ * busy runs on the second, beside it on the device; a batch that reads a
+ * block an unfinished batch of the other stream writes (or writes one it
+ * reads or writes) waits for it on the device first.  One
  * completion thread per stream stamps each request's completion.  A batch
  * pins at most half the cache budget (one request beyond that alone).
  * Requests on the same blocks take effect in submission order.  Every block
