@@ -275,10 +275,11 @@ int sipgpu_batch_execute_stages(sipgpu_batch_t batch, int stages, sipgpu_stream_
  * persistent water-filled schedule; for the split path (nmoduli > 0) the
  * three INT8 GEMMs (tile 128: 128 x 128, four waves; 256: 256 x 256, four
  * waves of 128 x 128; 257: 256 x 256, eight waves of 64 x 128) with and
- * without merging -- times each (one warm-up and three
+ * without merging -- times each (one warm-up and five
  * timed executions on the library stream, into scratch copies of the
  * destinations, so the caller's blocks are not touched) and returns the
- * fastest plan.  The choice is remembered under the batch's signature (its
+ * fastest plan.  The untuned plan is kept (its fields -1, "default") unless
+ * the fastest candidate beats it by more than 2%.  The choice is remembered under the batch's signature (its
  * shapes, patterns, accumulate flags and which entries share blocks, not the
  * addresses), so later sipgpu_batch_plan / _plan_split calls on batches with
  * that signature plan the same way.  Synchronises the device. */

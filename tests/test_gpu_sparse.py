@@ -65,3 +65,54 @@ def test_fill_random_matches_host_generator():
     got = a.download()
     assert np.max(np.abs(got - host)) <= 1e-14 * np.max(np.abs(host))
     a.free()
+
+
+def full_size_sample(w, blocks):
+    """Oracle results of `blocks` result blocks of the full-size ladder, each
+    the sum over its pairs of the oracle's contraction (operand blocks read
+    back from the device, pairs in parallel: the oracle's C calls release
+    the GIL)."""
+    from concurrent.futures import ThreadPoolExecutor
+    V, T, R, ptrn = w["L"], w["R"], w["D"], w["ptrn"]
+    want = {}
+    for rb in blocks:
+        pairs = [(vb, tb) for db, vb, tb in w["entries"] if db == rb]
+
+        def one(p):
+            vb, tb = p
+            tmp, e = o.contract(ptrn, V.download_block(vb), V.dims(vb), T.download_block(tb),
+                                T.dims(tb), R.dims(rb))
+            assert e == 0
+            return tmp
+        with ThreadPoolExecutor(8) as ex:
+            tmps = list(ex.map(one, pairs))
+        acc = np.zeros(int(np.prod(R.dims(rb))))
+        for t in tmps:  # summed in pair order, as the SIP's R += TMP sequence
+            acc += t
+        want[rb] = acc
+    return want
+
+
+def test_sparse_ladder_full_size_sampled_blocks():
+    """The block-sparse ladder at its full size (o = 40, v = 200, segments
+    12..40, 22500 pairs): FP64 and split (14 moduli) batches, three result
+    blocks (seeded choice) against the oracle: 1e-13 and 1e-12."""
+    w = wl.sparse_ladder()
+    for k in ("D", "L", "R"):
+        w[k].to_device()
+    R = w["D"]
+    rng = np.random.default_rng(11)
+    ids = list(R.block_ids())
+    blocks = [ids[int(i)] for i in rng.choice(len(ids), size=3, replace=False)]
+    print("sparse ladder full size, blocks", blocks, flush=True)
+    want = full_size_sample(w, blocks)
+    for moduli, tol in ((None, 1e-13), (14, 1e-12)):
+        R.dev.zero()
+        b = wl.batch_for(w, moduli=moduli)
+        b.execute()
+        sg.synchronize()
+        for rb in blocks:
+            assert rel_fro(R.download_block(rb), want[rb]) <= tol, (moduli, rb)
+        b.destroy()
+    for k in ("D", "L", "R"):
+        w[k].dev.free()
