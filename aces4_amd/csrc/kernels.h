@@ -30,6 +30,19 @@ struct PermArgs {
 struct PermutePlan;
 hipError_t launch_permute(const PermutePlan &plan, const double *in, double *out,
                           hipStream_t stream);
+// Batched permutes: fills args[0..n), the workgroup offsets first[0..n]
+// (first[n] the grid), the launch's LDS needs (largest staged tile in
+// doubles, largest tile v extent) and its elements per thread (8 or 16)
+// from the plans; returns the total workgroup count.
+int64_t permute_batch_args(const PermutePlan *plans, const double *const *in,
+                           double *const *out, int n, PermArgs *args, int *first,
+                           int *tile_cap, int *tv_cap, int *items);
+// One launch of `grid` workgroups over the device copies of args, first and
+// the per-workgroup entry table entry[0..grid) (entry[b] = the e with
+// first[e] <= b < first[e + 1]).
+hipError_t launch_permute_batch(const PermArgs *args, const int *first, const int *entry,
+                                int grid, int tile_cap, int tv_cap, int items,
+                                hipStream_t stream);
 
 // ------------------------------------------------------------- contraction --
 // One operand pair of a task.  Table offsets index the plan's int32 pool.

@@ -2076,6 +2076,110 @@ int sipgpu_permute(sipgpu_stream_t s, int rank, const int *dims, const int *dim_
     return 0;
 }
 
+struct sipgpu_permute_batch_s {
+    void *dev = nullptr; // PermArgs[n] | int first[n + 1] | int entry[grid]
+    size_t first_off = 0, entry_off = 0;
+    int n = 0, grid = 0;
+    int tile_cap = 0, tv_cap = 1; // LDS of the launch
+    int items = 16;               // elements per thread
+    long long elements = 0;
+};
+
+int sipgpu_permute_batch_plan(const sipgpu_permutation *list, int n,
+                              sipgpu_permute_batch_t *batch) {
+    if (int rc = sipgpu::rt::refuse_in_capture("permute batch planning")) return rc;
+    if (!batch || n < 0 || (n > 0 && !list)) return fail(SIPGPU_EARG, "bad permute batch arguments");
+    *batch = nullptr;
+    std::vector<PermutePlan> plans(n);
+    std::vector<const double *> in(n);
+    std::vector<double *> out(n);
+    long long elements = 0;
+    for (int e = 0; e < n; ++e) {
+        const sipgpu_permutation &q = list[e];
+        if (!q.in || !q.out) return fail(SIPGPU_EARG, "null block pointer in a permute batch");
+        if (q.rank < 0 || q.rank > SIPGPU_MAX_RANK)
+            return fail(SIPGPU_ERANK, "permute batch: rank outside 0..6");
+        if (q.rank == 0) {
+            plans[e].size = 1;
+            plans[e].identity = true;
+        } else if (int rc = plan_permute(q.rank, q.dims, q.dim_transp, plans[e])) {
+            return fail(rc, "invalid permutation in a permute batch");
+        }
+        in[e] = q.in;
+        out[e] = q.out;
+        elements += plans[e].size;
+    }
+    Context &c = ctx();
+    std::lock_guard<std::mutex> g(c.mu);
+    if (int rc = ensure_init(c)) return rc;
+    auto round256 = [](size_t x) { return (x + 255) & ~(size_t)255; };
+    const size_t first_off = round256((size_t)n * sizeof(PermArgs));
+    const size_t entry_off = first_off + round256((size_t)(n + 1) * sizeof(int));
+    std::vector<unsigned char> host(entry_off);
+    int tile_cap = 0, tv_cap = 1, items = 16;
+    const int64_t grid = permute_batch_args(
+        plans.data(), in.data(), out.data(), n, reinterpret_cast<PermArgs *>(host.data()),
+        reinterpret_cast<int *>(host.data() + first_off), &tile_cap, &tv_cap, &items);
+    if (grid >= ((int64_t)1 << 31)) return fail(SIPGPU_ESIZE, "permute batch: too many tiles");
+    // the entry of every workgroup
+    host.resize(entry_off + (size_t)grid * sizeof(int));
+    {
+        const int *first = reinterpret_cast<const int *>(host.data() + first_off);
+        int *entry = reinterpret_cast<int *>(host.data() + entry_off);
+        for (int e = 0; e < n; ++e)
+            for (int w = first[e]; w < first[e + 1]; ++w) entry[w] = e;
+    }
+    auto b = std::make_unique<sipgpu_permute_batch_s>();
+    b->first_off = first_off;
+    b->entry_off = entry_off;
+    b->n = n;
+    b->tile_cap = tile_cap;
+    b->tv_cap = tv_cap;
+    b->items = items;
+    b->grid = (int)grid;
+    b->elements = elements;
+    if (grid > 0) {
+        HIP_TRY(hipMalloc(&b->dev, host.size()));
+        const hipError_t e = hipMemcpy(b->dev, host.data(), host.size(), hipMemcpyHostToDevice);
+        if (e != hipSuccess) {
+            (void)hipFree(b->dev);
+            return hip_fail(e, "permute batch upload");
+        }
+    }
+    *batch = b.release();
+    return 0;
+}
+
+int sipgpu_permute_batch_execute(sipgpu_permute_batch_t b, sipgpu_stream_t s) {
+    if (!b) return fail(SIPGPU_EARG, "null permute batch");
+    if (b->grid == 0) return 0;
+    Context &c = ctx();
+    const unsigned char *base = static_cast<const unsigned char *>(b->dev);
+    HIP_TRY(launch_permute_batch(reinterpret_cast<const PermArgs *>(base),
+                                 reinterpret_cast<const int *>(base + b->first_off),
+                                 reinterpret_cast<const int *>(base + b->entry_off), b->grid,
+                                 b->tile_cap, b->tv_cap, b->items, resolve(c, s).s));
+    return 0;
+}
+
+int sipgpu_permute_batch_info(sipgpu_permute_batch_t b, long long *elements, int *workgroups) {
+    if (!b) return fail(SIPGPU_EARG, "null permute batch");
+    if (elements) *elements = b->elements;
+    if (workgroups) *workgroups = b->grid;
+    return 0;
+}
+
+int sipgpu_permute_batch_destroy(sipgpu_permute_batch_t b) {
+    if (int rc = sipgpu::rt::refuse_in_capture("destroying a permute batch")) return rc;
+    if (!b) return fail(SIPGPU_EARG, "null permute batch");
+    // the tables may still be read by a launch in flight on any stream
+    const hipError_t e = hipDeviceSynchronize();
+    if (e != hipSuccess) return hip_fail(e, "sipgpu_permute_batch_destroy");
+    if (b->dev) (void)hipFree(b->dev);
+    delete b;
+    return 0;
+}
+
 int sipgpu_contract(sipgpu_stream_t s, const int *ptrn, const double *l, int lrank,
                     const int *ldims, const double *r, int rrank, const int *rdims, double *d,
                     int drank, const int *ddims, int accumulate) {

@@ -63,7 +63,8 @@ EXPORTED = [
     "sipgpu_worker_write", "sipgpu_worker_flush", "sipgpu_worker_get_stats",
     "sipgpu_permute_rhs_to_lhs", "sipgpu_triples_create", "sipgpu_triples_info",
     "sipgpu_triples_pack", "sipgpu_triples_run", "sipgpu_triples_energy", "sipgpu_triples_tuple",
-    "sipgpu_triples_destroy",
+    "sipgpu_triples_destroy", "sipgpu_permute_batch_plan", "sipgpu_permute_batch_execute",
+    "sipgpu_permute_batch_info", "sipgpu_permute_batch_destroy",
 ]
 
 
@@ -80,6 +81,24 @@ class Contraction(ctypes.Structure):
                 ("ldims", ctypes.c_int * MAX_RANK), ("rdims", ctypes.c_int * MAX_RANK),
                 ("ddims", ctypes.c_int * MAX_RANK), ("ptrn", ctypes.c_int * (2 * MAX_RANK)),
                 ("accumulate", ctypes.c_int)]
+
+
+class Permutation(ctypes.Structure):
+    """sipgpu_permutation of include/sipgpu.h."""
+    _fields_ = [("in_", _vp), ("out", _vp), ("rank", ctypes.c_int),
+                ("dims", ctypes.c_int * MAX_RANK), ("dim_transp", ctypes.c_int * (MAX_RANK + 1))]
+
+
+def permutation(dims, dim_transp, din, dout):
+    """A Permutation entry: block `din` (extents `dims`) into `dout`, with
+    dim_transp = [sign, new position of each old dimension (1-based)]."""
+    e = Permutation()
+    e.in_, e.out, e.rank = din, dout, len(dims)
+    for i, x in enumerate(dims):
+        e.dims[i] = int(x)
+    for i, x in enumerate(dim_transp):
+        e.dim_transp[i] = int(x)
+    return e
 
 
 class Request(ctypes.Structure):
@@ -158,6 +177,12 @@ def lib():
         L.sipgpu_event_synchronize.argtypes = [_vp]
         L.sipgpu_event_elapsed_ms.argtypes = [_vp, _vp, ctypes.POINTER(ctypes.c_float)]
         L.sipgpu_permute.argtypes = [_vp, ctypes.c_int, _ip, _ip, _vp, _vp]
+        L.sipgpu_permute_batch_plan.argtypes = [ctypes.POINTER(Permutation), ctypes.c_int,
+                                                ctypes.POINTER(_vp)]
+        L.sipgpu_permute_batch_execute.argtypes = [_vp, _vp]
+        L.sipgpu_permute_batch_info.argtypes = [_vp, ctypes.POINTER(ctypes.c_longlong),
+                                                ctypes.POINTER(ctypes.c_int)]
+        L.sipgpu_permute_batch_destroy.argtypes = [_vp]
         L.sipgpu_contract.argtypes = [_vp, _ip, _vp, ctypes.c_int, _ip, _vp, ctypes.c_int, _ip,
                                       _vp, ctypes.c_int, _ip, ctypes.c_int]
         L.sipgpu_contract_noalloc.argtypes = [_vp, _ip, _vp, ctypes.c_int, _ip, _vp, ctypes.c_int, _ip,
@@ -737,6 +762,34 @@ def permute(rank_dims, dim_transp, din, dout, stream=None):
     t = _ints(dim_transp)
     _check(lib().sipgpu_permute(stream.handle if stream else None, len(rank_dims), _iptr(d),
                                 _iptr(t), din, dout))
+
+
+class PermuteBatch:
+    """Permutes run as one launch (sipgpu_permute_batch_*): a list of
+    Permutation entries (see `permutation`) over device blocks."""
+
+    def __init__(self, entries):
+        arr = (Permutation * len(entries))(*entries)
+        p = _vp()
+        _check(lib().sipgpu_permute_batch_plan(arr, len(entries), ctypes.byref(p)))
+        self.handle = p.value
+        el, wg = ctypes.c_longlong(0), ctypes.c_int(0)
+        _check(lib().sipgpu_permute_batch_info(self.handle, ctypes.byref(el), ctypes.byref(wg)))
+        self.elements, self.workgroups = el.value, wg.value
+
+    def execute(self, stream=None):
+        _check(lib().sipgpu_permute_batch_execute(self.handle, stream.handle if stream else None))
+
+    def destroy(self):
+        if self.handle:
+            _check(lib().sipgpu_permute_batch_destroy(self.handle))
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
 
 
 def contract(ptrn, l, ldims, r, rdims, d, ddims, accumulate=False, stream=None):

@@ -124,12 +124,13 @@ def cpu_baseline(w, budget_s, gpu_result, steps_run):
                            "nthreads=8" if ref else "C restatement, 1 thread"))
 
 
-def traffic_from_profile(path=PROFILE_TRAFFIC):
-    """HBM bytes per launch measured by rocprofv3 --pmc (FETCH_SIZE +
-    WRITE_SIZE, scaled by the copy-kernel calibration), if recorded."""
+def traffic_from_profile(path=PROFILE_TRAFFIC, key="hbm_bytes_per_launch"):
+    """HBM bytes per launch (or per `key` unit) measured by rocprofv3 --pmc
+    (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel calibration), if
+    recorded."""
     try:
         for line in open(path):
-            if line.startswith("hbm_bytes_per_launch:"):
+            if line.startswith(key + ":"):
                 return float(line.split(":", 1)[1])
     except OSError:
         pass
@@ -168,9 +169,10 @@ def latency_sample(w, n):
 
 
 def permute_bench(reps=20):
-    """Achieved HBM GB/s of the permute kernel (16 B moved per element):
-    a 64^4 block (134 MB) with the full index reversal, and the ring block
-    shape 30x15x30x15 with perm (3,4,1,2), timed with HIP events."""
+    """Achieved HBM GB/s of the permute kernels (16 B moved per element):
+    a 64^4 block (134 MB) with the full index reversal, the ring block shape
+    30x15x30x15 with perm (3,4,1,2), and all 144 ring T blocks permuted one
+    launch per block and as one batched launch, timed with HIP events."""
     out = {}
     for name, dims, perm in (("64^4 reversal", [64, 64, 64, 64], [4, 3, 2, 1]),
                              ("ring block (3,4,1,2)", [30, 15, 30, 15], [3, 4, 1, 2])):
@@ -189,6 +191,45 @@ def permute_bench(reps=20):
         out[name] = {"dims": dims, "perm": perm, "ms": ms, "GBps": 16.0 * n / (ms * 1e-3) / 1e9}
         din.free()
         dout.free()
+    # every T(a,i,c,k) block of the ring workload (v segments 30 x 4, 15 x 2;
+    # o segments 15, 15: 144 blocks, 20.25 M elements) permuted (3,4,1,2):
+    # block by block, and as one batched launch
+    vs, os_ = [30, 30, 30, 30, 15, 15], [15, 15]
+    shapes = [[a, i, c, k] for a in vs for i in os_ for c in vs for k in os_]
+    sizes = [int(np.prod(d)) for d in shapes]
+    offs = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int64)
+    n = int(offs[-1])
+    din = sg.DeviceArray.from_host(wl.uniform(12, n))
+    dout = sg.DeviceArray(n)
+    perm = [1, 3, 4, 1, 2]
+    ents = [sg.permutation(d, perm, din.ptr + 8 * int(offs[q]), dout.ptr + 8 * int(offs[q]))
+            for q, d in enumerate(shapes)]
+    b = sg.PermuteBatch(ents)
+    e0, e1 = sg.Event(), sg.Event()
+
+    def timed(fn):
+        fn()
+        sg.synchronize()
+        e0.record()
+        for _ in range(reps):
+            fn()
+        e1.record()
+        e1.synchronize()
+        return e0.elapsed_ms(e1) / reps
+
+    def one_by_one():
+        for q, d in enumerate(shapes):
+            sg.permute(d, perm, din.ptr + 8 * int(offs[q]), dout.ptr + 8 * int(offs[q]))
+    ms_each = timed(one_by_one)
+    ms_batch = timed(b.execute)
+    out["ring T blocks (3,4,1,2), 144 blocks"] = {
+        "blocks": len(shapes), "elements": n,
+        "block_by_block": {"ms": ms_each, "GBps": 16.0 * n / (ms_each * 1e-3) / 1e9},
+        "batched": {"ms": ms_batch, "GBps": 16.0 * n / (ms_batch * 1e-3) / 1e9,
+                    "workgroups": b.workgroups}}
+    b.destroy()
+    din.free()
+    dout.free()
     return out
 
 
@@ -294,12 +335,26 @@ def sparse_bench(reps=5, moduli=14):
            "surviving_fraction": wl.flops(w) / w["dense_flops"],
            "v_segments": w["v_segs"], "o_segments": w["o_segs"]}
     ref = None
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from checker import Oracle
+    o = Oracle()
+    # one result block (seeded choice) against the oracle's sum over its pairs
+    ids = list(D.block_ids())
+    rb = ids[int(np.random.default_rng(11).integers(len(ids)))]
+    want = np.zeros(int(np.prod(D.dims(rb))))
+    for db, vb, tb in w["entries"]:
+        if db == rb:
+            tmp, _ = o.contract(w["ptrn"], w["L"].download_block(vb), w["L"].dims(vb),
+                                w["R"].download_block(tb), w["R"].dims(tb), D.dims(rb))
+            want += tmp
+    out["oracle_block"] = list(rb)
     for name, nm in (("fp64", None), ("split", moduli)):
         b, tres = sg.tune(wl.contractions_for(w), moduli=nm)  # planned by the autotuner
         D.dev.zero()
         b.execute()
         sg.synchronize()
         got = D.dev.download()
+        oracle_err = rel_fro(D.download_block(rb, got), want)
         if ref is None:
             ref = got
             err = None
@@ -314,7 +369,8 @@ def sparse_bench(reps=5, moduli=14):
         e1.synchronize()
         ms = e0.elapsed_ms(e1) / reps
         out[name] = {"ms": ms, "tflops_fp64_equiv": b.flops / (ms * 1e-3) / 1e12,
-                     "moduli": b.moduli, "bits": b.bits, "tuned": tune_summary(tres)}
+                     "moduli": b.moduli, "bits": b.bits, "tuned": tune_summary(tres),
+                     "rel_err_vs_oracle_on_oracle_block": oracle_err}
         if err is not None:
             out[name]["max_block_rel_err_vs_fp64"] = err
         b.destroy()
@@ -699,7 +755,7 @@ def main():
     launch_ms = tri.pop("launch_ms")
     avg_launch_s = float(launch_ms.mean()) / 1e3
     achieved = tri["flop_per_launch"] / avg_launch_s / 1e12
-    per_box = traffic_from_profile(PROFILE_TRIPLES_TRAFFIC)
+    per_box = traffic_from_profile(PROFILE_TRIPLES_TRAFFIC, "hbm_bytes_per_box")
     traffic = per_box * tri["boxes_per_launch"] if per_box else None
     shape_classes = [(0, 1, 2, i, j, k) for i in (0, 1) for j in (0, 1) for k in (0, 1)]
     tri_tuples = None if a.no_triples else triples_tuple_bench(w5, ft, shape_classes)
@@ -718,6 +774,14 @@ def main():
                                  2000.0, 8, 64, moduli=14),
                    serving_bench("LRU spill (budget 60% of working set)", 8.5, 1500, 4000.0,
                                  8, 40)]
+        # offered rate swept to saturation: throughput against p50 / p99
+        sweep = []
+        for rate in (4000.0, 8000.0, 16000.0, 64000.0):
+            r = serving_bench("resident, rate sweep", 200.0, 3000, rate, 8, 64)
+            sweep.append({k: r[k] for k in ("offered_rate_per_s", "p50_ms", "p99_ms", "tflops",
+                                            "wall_s", "mean_batch")})
+        serving.append({"scenario": "resident (HBM budget 200 GB), offered rate swept to "
+                                    "saturation, plain FP64", "points": sweep})
     graphs = None if a.no_graphs else graph_bench()
     perm = permute_bench()
     blockops = blockops_bench()

@@ -177,6 +177,35 @@ int sipgpu_event_elapsed_ms(sipgpu_event_t start, sipgpu_event_t stop, float *ms
 int sipgpu_permute(sipgpu_stream_t s, int rank, const int *dims,
                    const int *dim_transp, const double *in, double *out);
 
+/* A batch of permutes run as one launch.  The reference permutes one block
+ * per call (tensor_block_copy__, tensor_ops_c_prototypes.h:152, from the
+ * permute super-instruction and Block::transpose_copy, block.cpp:216-232);
+ * a batch is that call repeated over its entries, so that many small blocks
+ * (whose single launches are latency-bound) fill the device together.
+ * Each entry has sipgpu_permute's meaning: `in` (rank, dims) into `out` with
+ * dim_transp[0..rank] (sign slot first, then the 1-based new position of
+ * each old dimension).  Entries run concurrently: no entry's output may
+ * overlap another entry's input or output. */
+typedef struct {
+    const double *in;
+    double *out;
+    int rank;
+    int dims[SIPGPU_MAX_RANK];
+    int dim_transp[SIPGPU_MAX_RANK + 1];
+} sipgpu_permutation;
+typedef struct sipgpu_permute_batch_s *sipgpu_permute_batch_t;
+/* Validates every entry (SIPGPU_EARG null pointer, SIPGPU_ERANK rank outside
+ * 0..6, SIPGPU_EPERM bad dim_transp, SIPGPU_ESIZE block of 2^31 elements or
+ * more) and uploads the batch's tile table; the blocks stay the caller's. */
+int sipgpu_permute_batch_plan(const sipgpu_permutation *list, int n,
+                              sipgpu_permute_batch_t *batch);
+/* One launch over every entry's tiles, asynchronously on `s`. */
+int sipgpu_permute_batch_execute(sipgpu_permute_batch_t batch, sipgpu_stream_t s);
+/* Elements moved per execution (reads = writes) and workgroups launched. */
+int sipgpu_permute_batch_info(sipgpu_permute_batch_t batch, long long *elements,
+                              int *workgroups);
+int sipgpu_permute_batch_destroy(sipgpu_permute_batch_t batch);
+
 /* Contract device blocks, asynchronously on `s`.  accumulate = 0 gives the
  * reference's assign semantics (D = L*R); accumulate = 1 gives D += L*R
  * (the block_contract_accumulate_op semantics of opcode.h:117, which the

@@ -105,3 +105,58 @@ def test_permute_errors():
 def test_scalar_permute():
     out, ierr = sg.tensor_block_copy([], [1], np.array([3.25]))
     assert ierr == 0 and out[0] == 3.25
+
+
+def _batch_cases():
+    """Entries of one permute batch: random ranks 1..6, the shapes above, an
+    identity, extent-1 dimensions and a scalar."""
+    cases = list(_random_perm_cases(43, 40, 24))
+    cases += [([30, 15, 30, 15], [1, 3, 4, 1, 2]), ([64, 64, 32, 16], [1, 4, 3, 2, 1]),
+              ([7, 5, 3], [1, 1, 2, 3]), ([1, 9, 1, 4], [1, 2, 4, 3, 1]),
+              ([3, 2000, 7], [1, 2, 3, 1]), ([], [1])]
+    return cases
+
+
+def test_permute_batch_vs_oracle():
+    """One launch over 46 permutes of mixed ranks and shapes, each output bit
+    for bit the oracle's copy; a second execution gives the same bits."""
+    cases = _batch_cases()
+    sizes = [int(np.prod(d)) if d else 1 for d, _ in cases]
+    offs = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int64)
+    x = uniform(91, int(offs[-1]), lo=-1.0, hi=1.0)
+    din = sg.DeviceArray.from_host(x)
+    dout = sg.DeviceArray(int(offs[-1]))
+    entries = [sg.permutation(d, t, din.ptr + 8 * int(offs[i]), dout.ptr + 8 * int(offs[i]))
+               for i, (d, t) in enumerate(cases)]
+    b = sg.PermuteBatch(entries)
+    assert b.elements == int(offs[-1])
+    print("permute batch", len(cases), "entries", b.workgroups, "workgroups", flush=True)
+    for rep in range(2):
+        dout.zero()
+        b.execute()
+        sg.synchronize()
+        got = dout.download()
+        for i, (d, t) in enumerate(cases):
+            seg = x[offs[i]:offs[i + 1]]
+            want = seg.copy() if not d else o.copy(d, t, seg)[0]
+            assert np.array_equal(_bits(got[offs[i]:offs[i + 1]]), _bits(want)), (rep, d, t)
+    b.destroy()
+    din.free()
+    dout.free()
+
+
+def test_permute_batch_errors():
+    din, dout = sg.DeviceArray(6), sg.DeviceArray(6)
+    good = sg.permutation([2, 3], [1, 2, 1], din.ptr, dout.ptr)
+    with pytest.raises(sg.SipGpuError) as e:
+        sg.PermuteBatch([good, sg.permutation([2, 3], [1, 1, 1], din.ptr, dout.ptr)])
+    assert e.value.code == -103
+    with pytest.raises(sg.SipGpuError) as e:
+        sg.PermuteBatch([sg.permutation([2, 3], [1, 2, 1], 0, dout.ptr)])
+    assert e.value.code == -107  # SIPGPU_EARG
+    empty = sg.PermuteBatch([])
+    assert empty.elements == 0 and empty.workgroups == 0
+    empty.execute()
+    empty.destroy()
+    din.free()
+    dout.free()
