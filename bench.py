@@ -170,11 +170,14 @@ def latency_sample(w, n):
 
 def permute_bench(reps=20):
     """Achieved HBM GB/s of the permute kernels (16 B moved per element):
-    a 64^4 block (134 MB) with the full index reversal, the ring block shape
+    a 64^4 block (134 MB: input and output together fit the 256 MB
+    last-level cache, so repeated runs read part of the input from it) and a
+    120^4 block (1.66 GB, HBM only) with the full index reversal, the ring block shape
     30x15x30x15 with perm (3,4,1,2), and all 144 ring T blocks permuted one
     launch per block and as one batched launch, timed with HIP events."""
     out = {}
     for name, dims, perm in (("64^4 reversal", [64, 64, 64, 64], [4, 3, 2, 1]),
+                             ("120^4 reversal", [120, 120, 120, 120], [4, 3, 2, 1]),
                              ("ring block (3,4,1,2)", [30, 15, 30, 15], [3, 4, 1, 2])):
         n = int(np.prod(dims))
         din = sg.DeviceArray.from_host(wl.uniform(11, n))
