@@ -121,6 +121,14 @@ struct CBlock {
     std::vector<std::pair<hipStream_t, hipEvent_t>> uses;
     std::list<unsigned long long>::iterator lru_it;
     bool in_lru = false;
+    hipEvent_t spilled = nullptr;  // recorded after the last spill into `home`
+};
+
+// An arena range [off, off + len) that becomes reusable once `ev` completes
+// (the spill that reads it, or the last kernel that used it).
+struct Fence {
+    hipEvent_t ev;
+    size_t off, len;
 };
 
 } // namespace
@@ -130,9 +138,10 @@ struct sipgpu_cache_s {
     Arena arena;
     void *dev = nullptr;
     hipStream_t copy = nullptr;                 // spill (device -> host) copies
+    hipStream_t h2d = nullptr;                  // refill (host -> device) copies
     std::unordered_map<unsigned long long, CBlock> blocks;
     std::list<unsigned long long> lru;          // front: least recently released
-    std::vector<hipEvent_t> fences;             // spills whose copies may be in flight
+    std::vector<Fence> fences;                  // freed ranges whose readers may be in flight
     sipgpu_cache_stats st{};
     std::vector<std::vector<unsigned long long>> scopes; // temp blocks of each open scope
     std::unordered_set<unsigned long long> persistent;   // kept by the end of a program
@@ -154,6 +163,11 @@ int wait_uses(hipStream_t s, const CBlock &b) {
 void drop_uses(CBlock &b) {
     for (auto &u : b.uses) (void)hipEventDestroy(u.second);
     b.uses.clear();
+}
+
+void drop_spilled(CBlock &b) {
+    if (b.spilled) (void)hipEventDestroy(b.spilled);
+    b.spilled = nullptr;
 }
 
 // Cache calls are host-side bookkeeping around uploads, spills, fences and
@@ -187,7 +201,12 @@ int evict_locked(sipgpu_cache_s &c, unsigned long long key, CBlock &b) {
         hipEvent_t f;
         if (int rc = make_event(&f)) return rc;
         SIPGPU_HIP_TRY(hipEventRecord(f, c.copy));
-        c.fences.push_back(f);
+        c.fences.push_back(Fence{f, b.off, Arena::round(bytes)});
+        // a refill of this block reads `home`: it waits for this copy
+        if (!b.spilled) {
+            if (int rc = make_event(&b.spilled)) return rc;
+        }
+        SIPGPU_HIP_TRY(hipEventRecord(b.spilled, c.copy));
         b.home_valid = true;
         c.st.spills += 1;
         c.st.bytes_d2h += bytes;
@@ -197,7 +216,7 @@ int evict_locked(sipgpu_cache_s &c, unsigned long long key, CBlock &b) {
         if (int rc = make_event(&f)) return rc;
         if (int rc = wait_uses(c.copy, b)) return rc;
         SIPGPU_HIP_TRY(hipEventRecord(f, c.copy));
-        c.fences.push_back(f);
+        c.fences.push_back(Fence{f, b.off, Arena::round(bytes)});
     }
     c.arena.release(b.off, bytes);
     b.off = kNotResident;
@@ -223,30 +242,35 @@ int remove_locked(sipgpu_cache_s &c, unsigned long long key) {
             if (int rc = make_event(&f)) return rc;
             if (int rc = wait_uses(c.copy, b)) return rc;
             SIPGPU_HIP_TRY(hipEventRecord(f, c.copy));
-            c.fences.push_back(f);
+            c.fences.push_back(Fence{f, b.off, Arena::round(b.elems * sizeof(double))});
         }
         c.arena.release(b.off, b.elems * sizeof(double));
         b.off = kNotResident;
     }
     if (b.home_owned) {
+        // the copy streams may still be filling or reading the spill buffer
         SIPGPU_HIP_TRY(hipStreamSynchronize(c.copy));
+        SIPGPU_HIP_TRY(hipStreamSynchronize(c.h2d));
         SIPGPU_HIP_TRY(hipHostFree(b.home));
     }
     drop_uses(b);
+    drop_spilled(b);
     c.blocks.erase(it);
     c.persistent.erase(key);
     return 0;
 }
 
-// Make `stream` wait for every spill or last use that freed arena space.
-int fence_locked(sipgpu_cache_s &c, hipStream_t stream) {
-    std::vector<hipEvent_t> keep;
-    for (hipEvent_t f : c.fences) {
-        if (hipEventQuery(f) == hipSuccess) {
-            (void)hipEventDestroy(f);
+// Make `stream` wait for the spills and last uses that freed any part of the
+// arena range [off, off + len) (completed fences are dropped on the way).
+int fence_locked(sipgpu_cache_s &c, hipStream_t stream, size_t off, size_t len) {
+    std::vector<Fence> keep;
+    for (const Fence &f : c.fences) {
+        if (hipEventQuery(f.ev) == hipSuccess) {
+            (void)hipEventDestroy(f.ev);
             continue;
         }
-        SIPGPU_HIP_TRY(hipStreamWaitEvent(stream, f, 0));
+        if (f.off < off + len && off < f.off + f.len)
+            SIPGPU_HIP_TRY(hipStreamWaitEvent(stream, f.ev, 0));
         keep.push_back(f);
     }
     c.fences.swap(keep);
@@ -273,15 +297,29 @@ int acquire_locked(sipgpu_cache_s &c, hipStream_t stream, unsigned long long key
         size_t off;
         if (int rc = reserve_locked(c, bytes, off)) return rc;
         b.off = off;
-        if (int rc = fence_locked(c, stream)) return rc;
         double *d = reinterpret_cast<double *>(c.arena.base + off);
         const bool need_value = mode != SIPGPU_CACHE_WRITE;
-        if (b.initialised && b.home_valid) {
-            if (need_value) {
-                SIPGPU_HIP_TRY(hipMemcpyAsync(d, b.home, bytes, hipMemcpyHostToDevice, stream));
-                c.st.bytes_h2d += bytes;
+        if (b.initialised && b.home_valid && need_value) {
+            // Refill on the cache's own host-to-device stream, so that it
+            // overlaps the kernels already queued on `stream` (and the spills
+            // on the device-to-host stream); `stream` waits for the copy.
+            if (int rc = fence_locked(c, c.h2d, off, Arena::round(bytes))) return rc;
+            if (b.spilled) {
+                SIPGPU_HIP_TRY(hipStreamWaitEvent(c.h2d, b.spilled, 0));
+                drop_spilled(b);
             }
-        } else if (!b.initialised) {
+            SIPGPU_HIP_TRY(hipMemcpyAsync(d, b.home, bytes, hipMemcpyHostToDevice, c.h2d));
+            hipEvent_t done;
+            if (int rc = make_event(&done)) return rc;
+            const hipError_t e1 = hipEventRecord(done, c.h2d);
+            const hipError_t e2 = e1 == hipSuccess ? hipStreamWaitEvent(stream, done, 0) : e1;
+            (void)hipEventDestroy(done); // released once the recorded copy completes
+            SIPGPU_HIP_TRY(e2);
+            c.st.bytes_h2d += bytes;
+        } else if (int rc = fence_locked(c, stream, off, Arena::round(bytes))) {
+            return rc;
+        }
+        if (!b.initialised) {
             if (b.init == SIPGPU_CACHE_ZERO) {
                 if (need_value)
                     SIPGPU_HIP_TRY(sipgpu::launch_block_op(sipgpu::kOpFill, d, nullptr,
@@ -358,6 +396,12 @@ int sipgpu_cache_create(size_t budget_bytes, sipgpu_cache_t *cache) {
         (void)hipFree(c->dev);
         return hip_fail(e, "cache: copy stream");
     }
+    e = hipStreamCreateWithFlags(&c->h2d, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        (void)hipStreamDestroy(c->copy);
+        (void)hipFree(c->dev);
+        return hip_fail(e, "cache: copy stream");
+    }
     c->st.budget_bytes = bytes;
     *cache = c.release();
     return 0;
@@ -370,9 +414,11 @@ int sipgpu_cache_destroy(sipgpu_cache_t c) {
     for (auto &kv : c->blocks) {
         if (kv.second.home_owned) (void)hipHostFree(kv.second.home);
         drop_uses(kv.second);
+        drop_spilled(kv.second);
     }
-    for (hipEvent_t f : c->fences) (void)hipEventDestroy(f);
+    for (const Fence &f : c->fences) (void)hipEventDestroy(f.ev);
     (void)hipStreamDestroy(c->copy);
+    (void)hipStreamDestroy(c->h2d);
     (void)hipFree(c->dev);
     delete c;
     return 0;
