@@ -427,9 +427,11 @@ int sipgpu_cache_destroy(sipgpu_cache_t cache);
 int sipgpu_cache_insert(sipgpu_cache_t cache, unsigned long long key, size_t elems,
                         const double *host, int init, unsigned long long seed,
                         unsigned long long stream, double a, double b);
-/* Makes the block resident (uploading on `s` on a miss, evicting as needed),
- * pins it and returns its device address.  WRITE marks the device copy as
- * the current value without uploading it. */
+/* Makes the block resident (on a miss: evicting as needed, then uploading
+ * on the cache's host-to-device stream, which `s` waits for), pins it and
+ * returns its device address.  Work enqueued on `s` after the call sees the
+ * block's value; other streams must be ordered after `s`.  WRITE marks the
+ * device copy as the current value without uploading it. */
 int sipgpu_cache_acquire(sipgpu_cache_t cache, sipgpu_stream_t s, unsigned long long key,
                          int mode, double **dev);
 /* Unpins the block after the work enqueued on `s` that uses it. */
