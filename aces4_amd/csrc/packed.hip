@@ -211,15 +211,19 @@ __global__ __launch_bounds__(256, 2) void gemm_f64_packed_kernel(OzGemmArgs a) {
         }
 
         if (tile.part >= 0) {
-            // one K segment of a split tile: raw accumulators, register order
+            // one K segment of a split tile: raw accumulators, register order;
+            // elements outside the task (which the fixup skips) are not written
             double *w = a.work + (long long)tile.part * (BM * BN);
 #pragma unroll
             for (int i = 0; i < FM; ++i)
 #pragma unroll
-                for (int j = 0; j < FN; ++j)
+                for (int j = 0; j < FN; ++j) {
+                    const bool nin = n0 + wn * TN + j * 16 + li < task.N;
 #pragma unroll
                     for (int r = 0; r < 4; ++r)
-                        w[(((wave * FM + i) * FN + j) * 4 + r) * 64 + lane] = acc[i][j][r];
+                        if (nin && m0 + wm * TM + i * 16 + 4 * r + lk < task.M)
+                            w[(((wave * FM + i) * FN + j) * 4 + r) * 64 + lane] = acc[i][j][r];
+                }
             continue;
         }
         // scatter into the destination block(s), 16 elements at a time:
@@ -283,8 +287,18 @@ __global__ __launch_bounds__(256) void fixup_f64_kernel(OzGemmArgs a) {
         const int n = f.nt * BN + wn * TN + j * 16 + (lane & 15);
         if (m >= task.M || n >= task.N) continue;
         const double *w0 = a.work + (long long)f.part * (BM * BN) + e;
-        double v = w0[0];
-        for (int q = 1; q < f.s0; ++q) v += w0[(long long)q * (BM * BN)];
+        // segments summed in order; loaded eight at a time so the loads are
+        // in flight together
+        double v = 0.0;
+        for (int q0 = 0; q0 < f.s0; q0 += 8) {
+            double x[8];
+#pragma unroll
+            for (int q = 0; q < 8; ++q)
+                x[q] = q0 + q < f.s0 ? w0[(long long)(q0 + q) * (BM * BN)] : 0.0;
+#pragma unroll
+            for (int q = 0; q < 8; ++q)
+                if (q0 + q < f.s0) v = q0 + q == 0 ? x[q] : v + x[q];
+        }
         int beta;
         double *p = oz_dest(task, a.subs, a.tab, m, n, beta);
         if (beta) v += *p;

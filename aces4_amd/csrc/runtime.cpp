@@ -1300,8 +1300,12 @@ int sipgpu::rt::build_panel_plan(const sipgpu_contraction *list, int n, int nmod
     int nslots = 0;
     const bool tail = opt.tail >= 0 ? opt.tail != 0 : !(tail_env && !std::strcmp(tail_env, "0"));
     if (!nmod && tail) {
-        // the packed FP64 GEMM runs two workgroups per CU, 32 CUs per XCD
-        persistent_schedule(lists, costs, 64, tiles, wg_first, fix, nslots);
+        // the packed FP64 GEMM runs two workgroups per CU, 32 CUs per XCD;
+        // tail = 2, 3, 4 run 32, 16, 8 workgroups per XCD, which cuts a small
+        // batch's tiles into fewer K segments: less partial-sum traffic
+        // through the fixup pass where the segments are short
+        const int slots = 64 >> (std::min(std::max(opt.tail, 1), 4) - 1);
+        persistent_schedule(lists, costs, slots, tiles, wg_first, fix, nslots);
     } else {
         tiles = interleave(lists);
     }
@@ -2318,7 +2322,7 @@ int sipgpu_batch_tune(const sipgpu_contraction *list, int n, int nmoduli, sipgpu
             cands.push_back(ch);
         }
         for (int merge : {1, 0})
-            for (int tail : {1, 0}) {
+            for (int tail : {1, 0, 2, 3, 4}) {
                 PlanChoice ch;
                 ch.path = 1;
                 ch.merge = merge;
@@ -2428,7 +2432,7 @@ int sipgpu_batch_plan_choice(const sipgpu_contraction *list, int n, int nmoduli,
     if (int rc = sipgpu::rt::refuse_in_capture("batch planning")) return rc;
     if (!batch || (n > 0 && !list) || n < 0) return fail(SIPGPU_EARG, "bad batch arguments");
     if (path < 0 || path > 2 || (path == 2) != (nmoduli != 0) || config < -1 ||
-        config >= kGemmConfigs ||
+        config >= kGemmConfigs || tail < -1 || tail > 4 ||
         (tile != -1 && tile != kOzBM && tile != kOzBigTile && tile != kOzWideTile))
         return fail(SIPGPU_EARG, "bad plan choice");
     if (nmoduli != 0 && (nmoduli < 6 || nmoduli > 14))

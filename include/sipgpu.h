@@ -300,8 +300,10 @@ enum { SIPGPU_STAGE_PREPARE = 1, SIPGPU_STAGE_PRODUCT = 2, SIPGPU_STAGE_FINISH =
 int sipgpu_batch_execute_stages(sipgpu_batch_t batch, int stages, sipgpu_stream_t s);
 /* Autotuner (no reference counterpart).  Plans the batch every candidate
  * way -- the table kernel in each of its tile configurations, packed FP64
- * panels with and without merged result blocks and with and without the
- * persistent water-filled schedule; for the split path (nmoduli > 0) the
+ * panels with and without merged result blocks, without the persistent
+ * water-filled schedule (tail 0) and with it over 64, 32, 16 or 8
+ * workgroups per XCD (tail 1, 2, 3, 4: fewer workgroups cut a small batch's
+ * tiles into fewer K segments); for the split path (nmoduli > 0) the
  * three INT8 GEMMs (tile 128: 128 x 128, four waves; 256: 256 x 256, four
  * waves of 128 x 128; 257: 256 x 256, eight waves of 64 x 128) with and
  * without merging -- times each (one warm-up and five
