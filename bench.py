@@ -57,7 +57,19 @@ def parse():
     p.add_argument("--no-serve", action="store_true")
     p.add_argument("--no-triples", action="store_true")
     p.add_argument("--no-graphs", action="store_true")
+    p.add_argument("--no-traffic", action="store_true",
+                   help="take roofline.traffic from the committed profile instead of measuring it")
+    p.add_argument("--traffic-probe", action="store_true",
+                   help=argparse.SUPPRESS)  # the child run that the counters are read from
     return p.parse_args()
+
+
+_T0 = time.time()
+
+
+def progress(msg):
+    """One line per finished leg on stderr (stdout carries only the JSON)."""
+    print("[bench %6.1f s] %s" % (time.time() - _T0, msg), file=sys.stderr, flush=True)
 
 
 def cpu_info():
@@ -475,6 +487,76 @@ def graph_bench(reps=20):
     return res
 
 
+# chunks the traffic probe runs (of the bench's chunk size, spread over the sweep;
+# indices past the last chunk wrap round)
+PROBE_LAUNCHES = (0, 5, 10, 15)
+
+
+def traffic_probe(steps):
+    """Child process under rocprofv3 --pmc: the calibration copy (1 GiB read,
+    1 GiB written by copy_kernel, via sipgpu_measure_peaks), then launches of
+    the headline kernel over the same slot chunks as the bench."""
+    from aces4_amd import triples as tp
+    sg.init(0)
+    sg.measure_peaks(1 << 30)
+    w = tp.triples()
+    tp.to_device(w)
+    ft = tp.FusedTriples(w)
+    ft.pack()
+    chunk = -(-ft.slots // steps)
+    for i in PROBE_LAUNCHES:
+        i %= steps
+        ft.run(i * chunk, max(0, min(chunk, ft.slots - i * chunk)))
+    sg.synchronize()
+    ft.destroy()
+    print("traffic probe done", flush=True)
+
+
+def measure_traffic(steps, timeout_s=240):
+    """HBM bytes per launch of triples_box_kernel, measured now: two child
+    runs of this script under rocprofv3 --pmc (FETCH_SIZE, then WRITE_SIZE),
+    each counter scaled by what it reads for the calibration copy of known
+    size in the same run.  Returns (bytes per launch, details) or (None,
+    reason) when the profiler is unavailable or fails."""
+    import csv
+    import shutil
+    import tempfile
+    prof = shutil.which("rocprofv3") or "/opt/rocm/bin/rocprofv3"
+    if not os.path.exists(prof):
+        return None, "rocprofv3 not found"
+    known = float(1 << 30)  # bytes the calibration copy reads, and writes
+    out = {}
+    for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+        d = tempfile.mkdtemp(prefix="traffic_", dir="/tmp")
+        cmd = [prof, "--pmc", counter, "--kernel-include-regex", "triples_box_kernel|copy_kernel",
+               "--output-format", "csv", "-d", d, "-o", "t", "--",
+               sys.executable, os.path.abspath(__file__), "--traffic-probe", "--steps", str(steps)]
+        env = dict(os.environ, TMPDIR="/tmp")
+        try:
+            r = subprocess.run(cmd, cwd="/tmp", env=env, capture_output=True, text=True,
+                               timeout=timeout_s)
+        except subprocess.TimeoutExpired:
+            shutil.rmtree(d, ignore_errors=True)
+            return None, "rocprofv3 %s run timed out" % counter
+        rows = []
+        for root, _, files in os.walk(d):
+            for f in files:
+                if f.endswith("counter_collection.csv"):
+                    rows += list(csv.DictReader(open(os.path.join(root, f))))
+        shutil.rmtree(d, ignore_errors=True)
+        copy = [float(x["Counter_Value"]) for x in rows if "copy_kernel" in x["Kernel_Name"]]
+        box = [float(x["Counter_Value"]) for x in rows if "triples_box_kernel" in x["Kernel_Name"]]
+        if r.returncode != 0 and not (copy and box):
+            return None, "rocprofv3 %s run failed (%d)" % (counter, r.returncode)
+        if not copy or len(box) != len(PROBE_LAUNCHES):
+            return None, "rocprofv3 %s: calibration or kernel dispatches missing" % counter
+        scale = known / (1024.0 * float(np.median(copy)))  # counter reads KiB
+        out[counter] = {"kib_per_launch": [float(b) for b in box], "calibration_kib": copy[0],
+                        "scale": scale, "bytes_per_launch": float(np.mean(box)) * 1024.0 * scale}
+    total = out["FETCH_SIZE"]["bytes_per_launch"] + out["WRITE_SIZE"]["bytes_per_launch"]
+    return total, out
+
+
 def triples_fused_bench(steps, warmup):
     """The headline: the whole unrestricted (T) sum with W on chip.  The box
     slots are cut into `steps` equal launches, timed between device
@@ -744,6 +826,9 @@ def ring_bench(steps, warmup, cpu_seconds, no_cpu, no_split, latency_samples):
 
 def main():
     a = parse()
+    if a.traffic_probe:
+        traffic_probe(a.steps)
+        return
     if a.gpus != 1:
         print("bench.py measures one GPU: this build runs the contraction path on a single "
               "MI355X (multi-GPU sharding is a design note only).", file=sys.stderr)
@@ -752,24 +837,31 @@ def main():
     # the ladder of the CPU-runnable configuration, tuned first: its choice
     # is remembered for the graph leg
     tuning = {"ladder_cfg1": tune_report(wl.ladder())}
+    progress("ladder tuned")
 
     # headline: the whole (T) sum with W on chip
     w5, ft, tri = triples_fused_bench(a.steps, a.warmup)
+    progress("whole (T) sum: %.1f TFLOP/s" % (tri["flops"] / tri["wall_s"] / 1e12))
     launch_ms = tri.pop("launch_ms")
     avg_launch_s = float(launch_ms.mean()) / 1e3
     achieved = tri["flop_per_launch"] / avg_launch_s / 1e12
     per_box = traffic_from_profile(PROFILE_TRIPLES_TRAFFIC, "hbm_bytes_per_box")
     traffic = per_box * tri["boxes_per_launch"] if per_box else None
+    traffic_src = {"source": "profiles/triples_box_traffic.txt (committed rocprofv3 --pmc runs)"}
     shape_classes = [(0, 1, 2, i, j, k) for i in (0, 1) for j in (0, 1) for k in (0, 1)]
     tri_tuples = None if a.no_triples else triples_tuple_bench(w5, ft, shape_classes)
+    progress("triples tuples")
     tri_cpu = None if a.no_cpu else triples_cpu_baseline(w5, a.cpu_seconds / 2)
+    progress("triples CPU baseline")
     ft.destroy()
     for k in ("V1", "V2", "T"):
         w5[k].dev.free()
 
     ring = ring_bench(a.ring_steps, 5, a.cpu_seconds, a.no_cpu, a.no_split, a.latency_samples)
     tuning["ring"] = ring.pop("tuning")
+    progress("ring")
     sparse = None if a.no_sparse else sparse_bench()
+    progress("sparse ladder")
     serving = None
     if not a.no_serve:
         serving = [serving_bench("resident (HBM budget 200 GB)", 200.0, 3000, 2000.0, 8, 64),
@@ -785,11 +877,24 @@ def main():
                                             "wall_s", "mean_batch")})
         serving.append({"scenario": "resident (HBM budget 200 GB), offered rate swept to "
                                     "saturation, plain FP64", "points": sweep})
+        progress("serving")
     graphs = None if a.no_graphs else graph_bench()
     perm = permute_bench()
     blockops = blockops_bench()
+    progress("graphs, permutes, block operations")
     cores, model = cpu_info()
     mfma_meas, copy_meas = sg.measure_peaks(1 << 30)
+    if not a.no_traffic:
+        sg.synchronize()
+        live, info = measure_traffic(a.steps)
+        progress("traffic: %s" % ("%.3e bytes per launch" % live if live else info))
+        if live is not None:
+            traffic = live
+            traffic_src = {"source": "measured in this run: rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE "
+                                     "child runs of launches %s of the same chunk size, scaled by the "
+                                     "calibration copy" % (list(PROBE_LAUNCHES),), "counters": info}
+        else:
+            traffic_src["live_measurement"] = info
     measured = {"mfma_f64_tflops": mfma_meas, "clock_mhz": sg.measured_clock_mhz(),
                 "copy_gbs": copy_meas, "copy_bytes": 2 * (1 << 30),
                 "note": "MFMA-only FP64 loop over all CUs; streaming copy of 1 GiB (read+write)"}
@@ -829,7 +934,7 @@ def main():
                    / 1e12},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS,
-                     "traffic": traffic,
+                     "traffic": traffic, "traffic_source": traffic_src,
                      "kernel": "triples_box_kernel", "avg_launch_ms": avg_launch_s * 1e3,
                      "flop_per_launch": tri["flop_per_launch"],
                      "mfma_flop_issued_per_box": tri["mfma_flop_issued_per_box"],
