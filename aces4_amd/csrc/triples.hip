@@ -36,7 +36,7 @@ typedef double f64x2 __attribute__((ext_vector_type(2)));
 constexpr int kBoxE = kTriplesBox;                 // box extent per index
 constexpr int kBoxRows = kBoxE * kBoxE * kBoxE;     // 125 rows / columns per P
 constexpr int kBoxElems = kBoxRows * kBoxRows;      // 15625
-constexpr int kThreads = 256;
+constexpr int kThreads = 512;
 
 // the permutations P of (0, 1, 2), in the order W sums them
 __constant__ int kPerm[6][3] = {{0, 1, 2}, {0, 2, 1}, {1, 0, 2}, {1, 2, 0}, {2, 0, 1}, {2, 1, 0}};
@@ -152,7 +152,8 @@ __device__ __forceinline__ int box_of(const TriplesArgs &a, int (&bc)[6], long l
 }
 
 // One workgroup: one 5^6 box of W, all six GEMMs, the energy partial.
-// Four waves in 2 x 2, each a 64 x 64 tile (4 x 4 MFMA blocks).
+// Eight waves in 2 x 4, each a 64 x 32 tile (4 x 2 MFMA blocks): two waves
+// per SIMD, so one issues MFMAs while the other waits for its operands.
 __global__ __launch_bounds__(kThreads) void triples_box_kernel(TriplesArgs a) {
     __shared__ double W[kBoxElems];
     __shared__ double eps[6][kBoxE];
@@ -187,14 +188,14 @@ __global__ __launch_bounds__(kThreads) void triples_box_kernel(TriplesArgs a) {
         const long long rowset = ((long long)bc[p0] * a.nob + bc[3 + p0]) * a.nob + bc[3 + p1];
         const long long colset = ((long long)bc[p1] * a.nvb + bc[p2]) * a.nob + bc[3 + p2];
         pa = reinterpret_cast<const f64x2 *>(a.PA) + ((rowset * 8 + wm * 4) * nkp) * 64 + lane;
-        pb = reinterpret_cast<const f64x2 *>(a.PB) + ((colset * 8 + wn * 4) * nkp) * 64 + lane;
+        pb = reinterpret_cast<const f64x2 *>(a.PB) + ((colset * 8 + wn * 2) * nkp) * 64 + lane;
     };
-    f64x2 fa[2][4], fb[2][4];
+    f64x2 fa[2][4], fb[2][2];
     auto load = [&](int st, const f64x2 *pa, const f64x2 *pb, int kp) {
 #pragma unroll
         for (int i = 0; i < 4; ++i) fa[st][i] = pa[((long long)i * nkp + kp) * 64];
 #pragma unroll
-        for (int j = 0; j < 4; ++j) fb[st][j] = pb[((long long)j * nkp + kp) * 64];
+        for (int j = 0; j < 2; ++j) fb[st][j] = pb[((long long)j * nkp + kp) * 64];
     };
     const f64x2 *pa, *pb;
     panels(0, pa, pb);
@@ -204,18 +205,18 @@ __global__ __launch_bounds__(kThreads) void triples_box_kernel(TriplesArgs a) {
         // which nothing reads)
         const f64x2 *na, *nb;
         panels(g < 5 ? g + 1 : g, na, nb);
-        f64x4 acc[4][4];
+        f64x4 acc[4][2];
 #pragma unroll
         for (int i = 0; i < 4; ++i)
 #pragma unroll
-            for (int j = 0; j < 4; ++j) acc[i][j] = f64x4{0.0, 0.0, 0.0, 0.0};
+            for (int j = 0; j < 2; ++j) acc[i][j] = f64x4{0.0, 0.0, 0.0, 0.0};
         auto mma = [&](int st) {
 #pragma unroll
             for (int h = 0; h < 2; ++h)
 #pragma unroll
                 for (int i = 0; i < 4; ++i)
 #pragma unroll
-                    for (int j = 0; j < 4; ++j)
+                    for (int j = 0; j < 2; ++j)
                         acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa[st][i][h], fb[st][j][h],
                                                                          acc[i][j], 0, 0, 0);
         };
@@ -239,11 +240,11 @@ __global__ __launch_bounds__(kThreads) void triples_box_kernel(TriplesArgs a) {
         const int p0 = kPerm[g][0], p1 = kPerm[g][1], p2 = kPerm[g][2];
         const int sx1 = box_stride(p0), sy1 = box_stride(3 + p0), sy2 = box_stride(3 + p1);
         const int sx2 = box_stride(p1), sx3 = box_stride(p2), sy3 = box_stride(3 + p2);
-        int coff[4];
-        bool cok[4];
+        int coff[2];
+        bool cok[2];
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            const int c = (wn * 4 + j) * 16 + (lane & 15);
+        for (int j = 0; j < 2; ++j) {
+            const int c = (wn * 2 + j) * 16 + (lane & 15);
             cok[j] = c < kBoxRows;
             coff[j] = (c % 5) * sx2 + ((c / 5) % 5) * sx3 + (c / 25) * sy3;
         }
@@ -255,7 +256,7 @@ __global__ __launch_bounds__(kThreads) void triples_box_kernel(TriplesArgs a) {
                 if (row >= kBoxRows) continue;
                 const int roff = (row % 5) * sx1 + ((row / 5) % 5) * sy1 + (row / 25) * sy2;
 #pragma unroll
-                for (int j = 0; j < 4; ++j) {
+                for (int j = 0; j < 2; ++j) {
                     if (!cok[j]) continue;
                     double &w = W[roff + coff[j]];
                     w = g == 0 ? acc[i][j][r] : w + acc[i][j][r];
