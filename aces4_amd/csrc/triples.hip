@@ -190,53 +190,53 @@ __global__ __launch_bounds__(kThreads) void triples_box_kernel(TriplesArgs a) {
         pa = reinterpret_cast<const f64x2 *>(a.PA) + ((rowset * 8 + wm * 4) * nkp) * 64 + lane;
         pb = reinterpret_cast<const f64x2 *>(a.PB) + ((colset * 8 + wn * 2) * nkp) * 64 + lane;
     };
-    f64x2 fa[2][4], fb[2][2];
+    f64x2 fa[3][4], fb[3][2];
     auto load = [&](int st, const f64x2 *pa, const f64x2 *pb, int kp) {
 #pragma unroll
         for (int i = 0; i < 4; ++i) fa[st][i] = pa[((long long)i * nkp + kp) * 64];
 #pragma unroll
         for (int j = 0; j < 2; ++j) fb[st][j] = pb[((long long)j * nkp + kp) * 64];
     };
-    const f64x2 *pa, *pb;
-    panels(0, pa, pb);
-    load(0, pa, pb, 0);
-    for (int g = 0; g < 6; ++g) {
-        // the next permutation's operands (the last one reloads its own,
-        // which nothing reads)
-        const f64x2 *na, *nb;
-        panels(g < 5 ? g + 1 : g, na, nb);
-        f64x4 acc[4][2];
-#pragma unroll
-        for (int i = 0; i < 4; ++i)
-#pragma unroll
-            for (int j = 0; j < 2; ++j) acc[i][j] = f64x4{0.0, 0.0, 0.0, 0.0};
-        auto mma = [&](int st) {
-#pragma unroll
-            for (int h = 0; h < 2; ++h)
-#pragma unroll
-                for (int i = 0; i < 4; ++i)
-#pragma unroll
-                    for (int j = 0; j < 2; ++j)
-                        acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa[st][i][h], fb[st][j][h],
-                                                                         acc[i][j], 0, 0, 0);
-        };
-        // nkp is even: two k-pairs per trip, the next one's loads in flight
-        // while the current one's MFMAs run; the last trip fetches the next
-        // permutation's first k-pair, which then lands during the adds below
-        for (int kp = 0; kp < nkp; kp += 2) {
-            load(1, pa, pb, kp + 1);
-            mma(0);
-            if (kp + 2 < nkp)
-                load(0, pa, pb, kp + 2);
-            else
-                load(0, na, nb, 0);
-            mma(1);
+    // The k-pairs of the six permutations form one stream; loads run two
+    // k-pairs ahead of the MFMAs through three register sets, across the
+    // permutation boundaries.  Load cursor: permutation lg, k-pair lkp
+    // (past the last k-pair of the last permutation it reloads that one,
+    // which nothing reads).
+    int lg = 0, lkp = 0;
+    const f64x2 *lpa, *lpb;
+    panels(0, lpa, lpb);
+    auto load_next = [&](int st) {
+        load(st, lpa, lpb, lkp);
+        if (++lkp == nkp) {
+            if (lg < 5) {
+                lkp = 0;
+                ++lg;
+                panels(lg, lpa, lpb);
+            } else {
+                lkp = nkp - 1;
+            }
         }
-        pa = na;
-        pb = nb;
+    };
+    f64x4 acc[4][2];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) acc[i][j] = f64x4{0.0, 0.0, 0.0, 0.0};
+    auto mma = [&](int st) {
+#pragma unroll
+        for (int h = 0; h < 2; ++h)
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int j = 0; j < 2; ++j)
+                    acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa[st][i][h], fb[st][j][h],
+                                                                     acc[i][j], 0, 0, 0);
+    };
+    // add permutation g's product into the box: row (x1, y1, y2), column
+    // (x2, x3, y3); the first permutation assigns
+    auto scatter = [&](int g) {
         // the previous P's adds into the box are complete
         __syncthreads();
-        // add the product into the box: row (x1, y1, y2), column (x2, x3, y3)
         const int p0 = kPerm[g][0], p1 = kPerm[g][1], p2 = kPerm[g][2];
         const int sx1 = box_stride(p0), sy1 = box_stride(3 + p0), sy2 = box_stride(3 + p1);
         const int sx2 = box_stride(p1), sx3 = box_stride(p2), sy3 = box_stride(3 + p2);
@@ -262,6 +262,35 @@ __global__ __launch_bounds__(kThreads) void triples_box_kernel(TriplesArgs a) {
                     w = g == 0 ? acc[i][j][r] : w + acc[i][j][r];
                 }
             }
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int j = 0; j < 2; ++j) acc[i][j] = f64x4{0.0, 0.0, 0.0, 0.0};
+    };
+    // compute cursor: permutation cg, k-pairs done ckp
+    int cg = 0, ckp = 0;
+    auto step = [&](int st) {
+        mma(st);
+        if (++ckp == nkp) {
+            scatter(cg);
+            ckp = 0;
+            ++cg;
+        }
+    };
+    load_next(0);
+    load_next(1);
+    const int total = 6 * nkp;
+    for (int t = 0; t < total; t += 3) {
+        load_next(2);
+        step(0);
+        if (t + 1 < total) {
+            load_next(0);
+            step(1);
+        }
+        if (t + 2 < total) {
+            load_next(1);
+            step(2);
+        }
     }
     __syncthreads();
 
