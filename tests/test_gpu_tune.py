@@ -2,7 +2,8 @@
 
 Every plan the tuner may pick is checked against the oracle first: the
 table kernel in each tile configuration the tuner times, and the packed
-panels with and without merged result blocks and the persistent schedule.
+panels with and without merged result blocks and the persistent schedule
+(over 64, 32, 16 and 8 workgroups per XCD).
 The tuner must leave the caller's blocks untouched, return a plan that
 matches the oracle, remember its choice by batch signature (also through a
 saved and reloaded table) and never pick a plan slower than the untuned one
@@ -51,7 +52,8 @@ def test_table_config_vs_oracle(make, cfg):
 
 
 @pytest.mark.parametrize("make", list(MAKERS), ids=list(MAKERS))
-@pytest.mark.parametrize("merge,tail", [(1, 1), (1, 0), (0, 1), (0, 0)])
+@pytest.mark.parametrize("merge,tail", [(1, 1), (1, 0), (0, 1), (0, 0), (0, 2), (1, 2), (0, 3),
+                                       (1, 4)])
 def test_packed_choice_vs_oracle(make, merge, tail):
     w = MAKERS[make]()
     prefill(w)
@@ -119,3 +121,12 @@ def test_tune_load_rejects_malformed(tmp_path):
         sg.tune_load(p)
     with pytest.raises(sg.SipGpuError):
         sg.tune_load(tmp_path / "missing.txt")
+
+
+def test_plan_choice_rejects_unknown_tail():
+    w = wl.ladder()
+    cs = contractions(w)
+    for tail in (-2, 5):
+        with pytest.raises(sg.SipGpuError) as e:
+            sg.plan_choice(cs, "packed", tail=tail)
+        assert e.value.code == -107  # SIPGPU_EARG
