@@ -109,16 +109,16 @@ def test_scalar_permute():
 
 def _batch_cases():
     """Entries of one permute batch: random ranks 1..6, the shapes above, an
-    identity, extent-1 dimensions and a scalar."""
+    identity, extent-1 dimensions, a scalar and an empty block."""
     cases = list(_random_perm_cases(43, 40, 24))
     cases += [([30, 15, 30, 15], [1, 3, 4, 1, 2]), ([64, 64, 32, 16], [1, 4, 3, 2, 1]),
               ([7, 5, 3], [1, 1, 2, 3]), ([1, 9, 1, 4], [1, 2, 4, 3, 1]),
-              ([3, 2000, 7], [1, 2, 3, 1]), ([], [1])]
+              ([3, 2000, 7], [1, 2, 3, 1]), ([], [1]), ([3, 0, 4], [1, 2, 1, 3])]
     return cases
 
 
 def test_permute_batch_vs_oracle():
-    """One launch over 46 permutes of mixed ranks and shapes, each output bit
+    """One launch over 47 permutes of mixed ranks and shapes, each output bit
     for bit the oracle's copy; a second execution gives the same bits."""
     cases = _batch_cases()
     sizes = [int(np.prod(d)) if d else 1 for d, _ in cases]
@@ -138,7 +138,7 @@ def test_permute_batch_vs_oracle():
         got = dout.download()
         for i, (d, t) in enumerate(cases):
             seg = x[offs[i]:offs[i + 1]]
-            want = seg.copy() if not d else o.copy(d, t, seg)[0]
+            want = seg.copy() if not d or seg.size == 0 else o.copy(d, t, seg)[0]
             assert np.array_equal(_bits(got[offs[i]:offs[i + 1]]), _bits(want)), (rep, d, t)
     b.destroy()
     din.free()
