@@ -13,7 +13,9 @@
 // cache keys) are queued by any thread; a scheduler thread takes everything
 // that has arrived (up to a batch limit and half the cache budget of pinned
 // blocks, stopping at the first request that would read a block an earlier
-// one in the batch writes, or vice versa; a permute is a batch of its own),
+// one in the batch writes, or vice versa; permutes and contractions form
+// separate batches, and a permute's destination is touched by no other
+// permute of its batch),
 // makes its blocks resident, and launches the batch as one grouped launch,
 // its work list uploaded through a small ring of descriptor slots (no
 // allocation per batch).  Two device streams ("lanes"): large batches run
@@ -44,6 +46,7 @@
 #include "../../include/sipgpu.h"
 #include "internal.h"
 #include "kernels.h"
+#include "plan.h"
 
 using sipgpu::rt::fail;
 using sipgpu::rt::hip_fail;
@@ -778,32 +781,103 @@ int check_sizes(sipgpu_cache_s *c, const Pending &p) {
     return 0;
 }
 
-int server_permute(sipgpu_server_s &S, int lane, const PermuteReq &p) {
-    if (int rc = check_sizes(S.cache, Pending{0, kKindPermute, {}, p, {}})) return rc;
+// A batch of permutes (sources distinct from every destination, each
+// destination written once: the scheduler's hazard rule) as one launch of
+// the batched permute kernel: each block acquired once (sources for
+// reading, destinations for writing), the kernel's tables through a
+// descriptor slot.
+int server_permutes(sipgpu_server_s &S, int lane, const std::vector<Pending> &batch) {
+    for (const Pending &p : batch)
+        if (int rc = check_sizes(S.cache, p)) return rc;
+    const int n = (int)batch.size();
+    std::vector<sipgpu::PermutePlan> plans(n);
+    for (int e = 0; e < n; ++e) {
+        const PermuteReq &q = batch[e].perm;
+        if (int rc = sipgpu::plan_permute(q.rank, q.dims, q.transp, plans[e]))
+            return fail(rc, "server: invalid permutation");
+    }
+    const hipStream_t stream = S.stream[lane];
     sipgpu_stream_s sw;
-    sw.s = S.stream[lane];
-    double *src = nullptr, *dst = nullptr;
-    int rc = sipgpu_cache_acquire(S.cache, &sw, p.src, SIPGPU_CACHE_READ, &src);
-    if (rc) return rc;
-    rc = sipgpu_cache_acquire(S.cache, &sw, p.dst, SIPGPU_CACHE_WRITE, &dst);
+    sw.s = stream;
+    std::unordered_map<unsigned long long, double *> ptr;
+    std::vector<unsigned long long> acquired;
+    int rc = 0;
+    for (const Pending &p : batch) {
+        const unsigned long long keys[2] = {p.perm.src, p.perm.dst};
+        for (int q = 0; q < 2 && !rc; ++q) {
+            if (ptr.count(keys[q])) continue;
+            double *d = nullptr;
+            rc = sipgpu_cache_acquire(S.cache, &sw, keys[q],
+                                      q ? SIPGPU_CACHE_WRITE : SIPGPU_CACHE_READ, &d);
+            if (!rc) {
+                ptr[keys[q]] = d;
+                acquired.push_back(keys[q]);
+            }
+        }
+        if (rc) break;
+    }
     if (!rc) {
-        rc = sipgpu_permute(&sw, p.rank, p.dims, p.transp, src, dst);
-        const int r2 = sipgpu_cache_release(S.cache, &sw, p.dst);
+        std::vector<const double *> in(n);
+        std::vector<double *> out(n);
+        for (int e = 0; e < n; ++e) {
+            in[e] = ptr[batch[e].perm.src];
+            out[e] = ptr[batch[e].perm.dst];
+        }
+        auto round256 = [](size_t x) { return (x + 255) & ~(size_t)255; };
+        const size_t first_off = round256((size_t)n * sizeof(sipgpu::PermArgs));
+        const size_t entry_off = first_off + round256((size_t)(n + 1) * sizeof(int));
+        std::vector<unsigned char> host(entry_off);
+        int tile_cap = 0, tv_cap = 1, items = 16;
+        const int64_t grid = sipgpu::permute_batch_args(
+            plans.data(), in.data(), out.data(), n,
+            reinterpret_cast<sipgpu::PermArgs *>(host.data()),
+            reinterpret_cast<int *>(host.data() + first_off), &tile_cap, &tv_cap, &items);
+        const size_t bytes = entry_off + (size_t)std::max<int64_t>(grid, 0) * sizeof(int);
+        if (grid >= ((int64_t)1 << 31) || bytes > kSlotBytes) {
+            // tables beyond a descriptor slot: one launch per permute
+            for (int e = 0; e < n && !rc; ++e) {
+                const hipError_t he = sipgpu::launch_permute(plans[e], in[e], out[e], stream);
+                if (he != hipSuccess) rc = hip_fail(he, "server: permute");
+            }
+        } else if (grid > 0) {
+            host.resize(bytes);
+            const int *first = reinterpret_cast<const int *>(host.data() + first_off);
+            int *entry = reinterpret_cast<int *>(host.data() + entry_off);
+            for (int e = 0; e < n; ++e)
+                for (int w = first[e]; w < first[e + 1]; ++w) entry[w] = e;
+            int slot = 0;
+            rc = take_slot(S, slot);
+            if (!rc) {
+                std::memcpy(S.slot_host[slot], host.data(), bytes);
+                const unsigned char *dev = static_cast<const unsigned char *>(S.slot_dev[slot]);
+                hipError_t e = hipMemcpyAsync(S.slot_dev[slot], S.slot_host[slot], bytes,
+                                              hipMemcpyHostToDevice, stream);
+                if (e == hipSuccess)
+                    e = sipgpu::launch_permute_batch(
+                        reinterpret_cast<const sipgpu::PermArgs *>(dev),
+                        reinterpret_cast<const int *>(dev + first_off),
+                        reinterpret_cast<const int *>(dev + entry_off), (int)grid, tile_cap,
+                        tv_cap, items, stream);
+                if (e == hipSuccess) e = hipEventRecord(S.slot_free[slot], stream);
+                if (e != hipSuccess) rc = hip_fail(e, "server: permute launch");
+                S.slot_used[slot] = true;
+            }
+        }
+    }
+    for (unsigned long long k : acquired) {
+        const int r2 = sipgpu_cache_release(S.cache, &sw, k);
         if (!rc) rc = r2;
     }
-    const int r3 = sipgpu_cache_release(S.cache, &sw, p.src);
-    if (!rc) rc = r3;
     if (!rc) {
         S.st.batches += 1;
-        S.st.requests += 1;
-        S.st.max_batch = std::max<unsigned long long>(S.st.max_batch, 1);
+        S.st.requests += batch.size();
+        S.st.max_batch = std::max<unsigned long long>(S.st.max_batch, batch.size());
     }
     return rc;
 }
 
 int server_batch(sipgpu_server_s &S, int lane, std::vector<Pending> &batch) {
-    if (batch.size() == 1 && batch[0].kind == kKindPermute)
-        return server_permute(S, lane, batch[0].perm);
+    if (!batch.empty() && batch[0].kind == kKindPermute) return server_permutes(S, lane, batch);
     for (const Pending &p : batch)
         if (int rc = check_sizes(S.cache, p)) return rc;
     const hipStream_t stream = S.stream[lane];
@@ -999,13 +1073,24 @@ void scheduler_loop(sipgpu_server_s *S) {
             size_t pinned = 0;
             while (!S->queue.empty() && (int)batch.size() < S->max_batch) {
                 const Pending &p = S->queue.front();
-                // a permute is a batch of its own
-                if (p.kind == kKindPermute) {
-                    if (batch.empty()) {
-                        batch.push_back(p);
-                        S->queue.pop_front();
-                    }
+                // permutes batch with permutes (one launch), contractions
+                // with contractions
+                if (!batch.empty() && (p.kind == kKindPermute) != (batch[0].kind == kKindPermute))
                     break;
+                if (p.kind == kKindPermute) {
+                    const PermuteReq &q = p.perm;
+                    // program order: a destination is neither read nor
+                    // written by another permute of the batch
+                    if (writes.count(q.src) || reads.count(q.dst) || writes.count(q.dst)) break;
+                    const size_t add = (reads.count(q.src) ? 0 : 8 * elems_of(q.rank, q.dims)) +
+                                       8 * elems_of(q.rank, q.dims);
+                    if (!batch.empty() && pinned + add > pin_limit) break;
+                    pinned += add;
+                    reads.insert(q.src);
+                    writes.insert(q.dst);
+                    batch.push_back(p);
+                    S->queue.pop_front();
+                    continue;
                 }
                 const sipgpu_request &r = p.req;
                 // keep program order: no block both written and read in one batch

@@ -479,8 +479,10 @@ int sipgpu_cache_end_program(sipgpu_cache_t cache, unsigned long long *kept);
  * (and permutes) whose blocks live in a cache.  A scheduler thread takes
  * every request that has arrived (up to `max_batch`, stopping at the first
  * one that would read a block an earlier request of the batch writes, or the
- * reverse, and at a change of precision; a permute is a batch of its own),
- * makes its blocks resident and launches the batch: one grouped FP64
+ * reverse, and at a change of precision; consecutive permutes form a batch
+ * of their own, in which no permute's destination is another's source or
+ * destination), makes its blocks resident and launches the batch: one
+ * batched permute launch, one grouped FP64
  * launch, or the split-precision stages with their planes in a server-owned
  * scratch buffer per stream (grown, after that stream drains, to the
  * largest batch's need).  Two device streams: large batches (>= 1e9 flop)

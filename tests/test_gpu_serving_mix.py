@@ -244,3 +244,47 @@ def test_server_dependency_chain_across_streams():
         want, _ = o.contract(ptrn, want, [n, n], C, [n, n], [n, n])
     assert rel_fro(c.read(17, n * n), want) <= 1e-13
     c.destroy()
+
+
+def test_server_permute_batches_bit_exact():
+    """Forty permutes of ranks 2..5 submitted back to back (consecutive
+    permutes run as one batched launch), a source read by two of them, and a
+    chain X -> Y -> Z whose second permute reads the first one's destination
+    (it must wait for it): every destination equals the oracle's copy bit for
+    bit."""
+    rng = np.random.default_rng(17)
+    c = sg.Cache(256 << 20)
+    cases = []
+    for q in range(40):
+        rank = 2 + q % 4
+        dims = [int(x) for x in rng.integers(2, 13, size=rank)]
+        perm = [int(x) + 1 for x in rng.permutation(rank)]
+        x = rng.standard_normal(int(np.prod(dims)))
+        c.insert(1000 + q, x.size, host=x)
+        c.insert(2000 + q, x.size, init="zero")
+        cases.append((dims, [1] + perm, x, 2000 + q, 1000 + q))
+    # fan-out: block 1000 also into 3000
+    d0, t0, x0 = cases[0][0], cases[0][1], cases[0][2]
+    c.insert(3000, x0.size, init="zero")
+    cases.append((d0, t0, x0, 3000, 1000))
+    # chain: X -> Y (reverse), Y -> Z (reverse back)
+    dims = [6, 7, 8]
+    X = rng.standard_normal(int(np.prod(dims)))
+    c.insert(4000, X.size, host=X)
+    c.insert(4001, X.size, init="zero")
+    c.insert(4002, X.size, init="zero")
+    srv = sg.Server(c, max_batch=64)
+    tickets = [srv.submit_permute(d, t, src, dst) for d, t, _, dst, src in cases]
+    tickets.append(srv.submit_permute(dims, [1, 3, 2, 1], 4000, 4001))
+    tickets.append(srv.submit_permute(dims[::-1], [1, 3, 2, 1], 4001, 4002))
+    for tk in tickets:
+        srv.wait(tk)
+    print("server stats", srv.stats(), flush=True)
+    srv.destroy()
+    for d, t, x, dst, _ in cases:
+        want, e = o.copy(d, t, x)
+        assert e == 0 and np.array_equal(c.read(dst, x.size), want), (d, t)
+    Y, _ = o.copy(dims, [1, 3, 2, 1], X)
+    assert np.array_equal(c.read(4001, X.size), Y)
+    assert np.array_equal(c.read(4002, X.size), X)
+    c.destroy()
