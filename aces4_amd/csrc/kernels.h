@@ -119,6 +119,9 @@ hipError_t launch_scalar_mul(double *D, const double *L, const double *R, hipStr
 
 // Calibration kernels (calib.hip).
 hipError_t launch_mfma_peak(double *out, long long *ticks, int blocks, int iters, hipStream_t s);
+// LDS read bandwidth loop, 8 or 16 waves per workgroup (calib.hip).
+hipError_t launch_lds_read(int waves, int blocks, int *out, long long *ticks, int iters,
+                           hipStream_t s);
 hipError_t launch_copy(const double *in, double *out, size_t n, hipStream_t s);
 // Orbital-energy denominators: rank <= kEnergyMaxRank block, one energy
 // vector and one sign (+1 occupied, -1 virtual) per index.

@@ -2837,6 +2837,58 @@ int sipgpu_measure_peaks(double *mfma_f64_tflops, double *copy_gbs, size_t copy_
 
 double sipgpu_measured_clock_mhz(void) { return g_last_clock_mhz; }
 
+int sipgpu_measure_lds(double *bytes_per_clock_per_cu) {
+    if (!bytes_per_clock_per_cu) return fail(SIPGPU_EARG, "null argument");
+    Context &c = ctx();
+    std::lock_guard<std::mutex> g(c.mu);
+    if (int rc = ensure_init(c)) return rc;
+    hipStream_t s = c.stream.s;
+    hipDeviceProp_t prop;
+    HIP_TRY(hipGetDeviceProperties(&prop, c.device));
+    int wall_khz = 0;
+    HIP_TRY(hipDeviceGetAttribute(&wall_khz, hipDeviceAttributeWallClockRate, c.device));
+    const int blocks = prop.multiProcessorCount * 2, iters = 4096;
+    int *out = nullptr;
+    long long *ticks = nullptr;
+    HIP_TRY(hipMalloc(&out, (size_t)blocks * sizeof(int)));
+    HIP_TRY(hipMalloc(&ticks, (size_t)blocks * 2 * sizeof(long long)));
+    double best = 0.0;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    hipError_t e = hipEventCreate(&e0);
+    if (e == hipSuccess) e = hipEventCreate(&e1);
+    if (e == hipSuccess) e = launch_lds_read(8, blocks, out, ticks, 16, s); // warm-up
+    for (int waves : {8, 16}) {
+        if (e != hipSuccess) break;
+        e = hipEventRecord(e0, s);
+        if (e == hipSuccess) e = launch_lds_read(waves, blocks, out, ticks, iters, s);
+        if (e == hipSuccess) e = hipEventRecord(e1, s);
+        if (e == hipSuccess) e = hipEventSynchronize(e1);
+        float ms = 0.f;
+        if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+        std::vector<long long> t((size_t)blocks * 2);
+        if (e == hipSuccess)
+            e = hipMemcpy(t.data(), ticks, t.size() * sizeof(long long), hipMemcpyDeviceToHost);
+        if (e != hipSuccess || ms <= 0.f) break;
+        // the shader clock the loops ran at (their clock and wall-clock
+        // ticks), then all bytes over the kernel's duration per CU and clock
+        double cyc = 0.0, wall = 0.0;
+        for (int b = 0; b < blocks; ++b) {
+            cyc += (double)t[2 * b];
+            wall += (double)t[2 * b + 1];
+        }
+        const double hz = wall > 0.0 ? cyc / wall * wall_khz * 1e3 : prop.clockRate * 1e3;
+        const double bytes = (double)blocks * waves * 64 * iters * 8 * 16;
+        best = std::max(best, bytes / (ms * 1e-3) / prop.multiProcessorCount / hz);
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    (void)hipFree(out);
+    (void)hipFree(ticks);
+    if (e != hipSuccess) return hip_fail(e, "sipgpu_measure_lds");
+    *bytes_per_clock_per_cu = best;
+    return 0;
+}
+
 int sipgpu_plan_tables(const int *ptrn, int lrank, const int *ldims, int rrank, const int *rdims,
                        int drank, const int *ddims, long long *mnk, int *o2n, int *koffL,
                        int *moffL, int *koffR, int *noffR, int *moffD, int *noffD) {

@@ -64,7 +64,7 @@ EXPORTED = [
     "sipgpu_permute_rhs_to_lhs", "sipgpu_triples_create", "sipgpu_triples_info",
     "sipgpu_triples_pack", "sipgpu_triples_run", "sipgpu_triples_energy", "sipgpu_triples_tuple",
     "sipgpu_triples_destroy", "sipgpu_permute_batch_plan", "sipgpu_permute_batch_execute",
-    "sipgpu_permute_batch_info", "sipgpu_permute_batch_destroy",
+    "sipgpu_permute_batch_info", "sipgpu_permute_batch_destroy", "sipgpu_measure_lds",
 ]
 
 
@@ -220,6 +220,7 @@ def lib():
                                          _ip, _ip, _ip, _ip, _ip, _ip, _ip]
         L.sipgpu_plan_permute.argtypes = [ctypes.c_int, _ip, _ip, _ip]
         L.sipgpu_measured_clock_mhz.restype = ctypes.c_double
+        L.sipgpu_measure_lds.argtypes = [ctypes.POINTER(ctypes.c_double)]
         _u64 = ctypes.c_ulonglong
         L.sipgpu_cache_create.argtypes = [ctypes.c_size_t, ctypes.POINTER(_vp)]
         L.sipgpu_cache_destroy.argtypes = [_vp]
@@ -539,6 +540,13 @@ def energy_sum_parts(dims, t, parts, eps, signs, factor, out, s=None):
 def measured_clock_mhz():
     """Shader clock (MHz) seen by the last measure_peaks MFMA loop."""
     return lib().sipgpu_measured_clock_mhz()
+
+
+def measure_lds():
+    """Measured LDS read bandwidth, bytes per shader clock per CU."""
+    v = ctypes.c_double(0)
+    _check(lib().sipgpu_measure_lds(ctypes.byref(v)))
+    return v.value
 
 
 def synchronize():

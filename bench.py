@@ -897,7 +897,9 @@ def main():
             traffic_src["live_measurement"] = info
     measured = {"mfma_f64_tflops": mfma_meas, "clock_mhz": sg.measured_clock_mhz(),
                 "copy_gbs": copy_meas, "copy_bytes": 2 * (1 << 30),
-                "note": "MFMA-only FP64 loop over all CUs; streaming copy of 1 GiB (read+write)"}
+                "lds_read_bytes_per_clock_per_cu": sg.measure_lds(),
+                "note": "MFMA-only FP64 loop over all CUs; streaming copy of 1 GiB (read+write); "
+                        "16-byte LDS reads, two workgroups per CU"}
     cpu = None
     if tri_cpu:
         ref = tri_cpu.get("reference_no_blas")

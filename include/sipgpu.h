@@ -679,6 +679,11 @@ int sipgpu_fill_random(sipgpu_stream_t s, double *out, long long n, unsigned lon
 int sipgpu_measure_peaks(double *mfma_f64_tflops, double *copy_gbs, size_t copy_bytes);
 /* Effective shader clock (MHz) during the last MFMA calibration. */
 double sipgpu_measured_clock_mhz(void);
+/* Measured LDS read bandwidth: bytes per shader clock per CU of 16-byte
+ * reads by two workgroups of 8 or 16 waves per CU (the better of the two):
+ * all bytes over the kernel's duration, at the shader clock its loops ran
+ * at. */
+int sipgpu_measure_lds(double *bytes_per_clock_per_cu);
 
 /* Host-only introspection (no device work).  For one contraction shape:
  * mnk[0..2] = M, N, K of the product; o2n receives lo2n[1..lrank],
