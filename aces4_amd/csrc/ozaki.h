@@ -12,6 +12,10 @@ constexpr int kOzBM = 128, kOzBN = 128; // INT8 GEMM workgroup tile
 constexpr int kOzBigTile = 256;         // the large-task GEMM's tile (both dims)
 // tile code of the 256 x 256 GEMM with 8 waves of 64 x 128 (two per SIMD)
 constexpr int kOzWideTile = 257;
+// tile code (and edge) of the packed FP64 GEMM's 64 x 64 tile
+constexpr int kOzSmallTile = 64;
+// doubles per workspace slot of a split tile's K segment (any tile edge)
+constexpr int kOzWorkSlot = kOzBM * kOzBN;
 // edge of the output tile a GEMM tile code stands for
 constexpr int oz_tile_size(int code) { return code == kOzWideTile ? kOzBigTile : code; }
 constexpr int kOzChunk = 16;            // bytes of k per panel chunk
@@ -126,10 +130,10 @@ struct OzGemmArgs {
     uint8_t *res_planes;
     long long res_plane_bytes;
     int ntiles;
-    int tile; // kOzBM (= kOzBN) or kOzBigTile
+    int tile; // tile code: kOzBM (= kOzBN), kOzBigTile, kOzWideTile, kOzSmallTile (FP64)
     const int32_t *tab; // destination tables (packed FP64 path)
     const OzSub *subs;  // result blocks of merged tasks (packed FP64 path)
-    double *work;       // split-tile partial products, kOzBM * kOzBN per slot
+    double *work;       // split-tile partial products, kOzWorkSlot per slot
     const OzTile *fix;  // split tiles to sum and store
     int nfix;
     const int *wg_first; // persistent launch: units [wg_first[w], wg_first[w+1]) of

@@ -88,15 +88,18 @@ __global__ __launch_bounds__(256) void pack_kernel(OzConvArgs a) {
     }
 }
 
-// Grouped FP64 GEMM over packed panels.  Workgroup tile 128 x 128 (4 waves,
-// 2 x 2, each 64 x 64: 16 accumulator fragments); stages of
+// Grouped FP64 GEMM over packed panels.  Workgroup tile BM x BN = 128 x 128
+// (4 waves, 2 x 2, each 64 x 64: 16 accumulator fragments) or 64 x 64 (each
+// wave 32 x 32: for small batches, whose tiles are cut into K segments:
+// a segment's partial sums are a quarter of the bytes); stages of
 // 128 bytes (16 k) double-buffered in LDS with rows XOR-swizzled by 16-byte
 // chunk; loads run one stage ahead through registers; chunk entries come
 // from an LDS window (their plane offsets count 16-byte units, so panels may
 // reach 64 GiB).  Rows past the task's M / N read row M-1 / N-1 and are
 // never stored; chunks past the pairs' K read the plane's zero bytes.
+template <int BM, int BN>
 __global__ __launch_bounds__(256, 2) void gemm_f64_packed_kernel(OzGemmArgs a) {
-    constexpr int BM = 128, BN = 128, WM = 2, WN = 2;
+    constexpr int WM = 2, WN = 2;
     constexpr int TM = BM / WM, TN = BN / WN, FM = TM / 16, FN = TN / 16;
     constexpr int KS = kOzStage, CH = kOzStageChunks;
     constexpr int IA = BM / 32, IB = BN / 32;
@@ -213,7 +216,7 @@ __global__ __launch_bounds__(256, 2) void gemm_f64_packed_kernel(OzGemmArgs a) {
         if (tile.part >= 0) {
             // one K segment of a split tile: raw accumulators, register order;
             // elements outside the task (which the fixup skips) are not written
-            double *w = a.work + (long long)tile.part * (BM * BN);
+            double *w = a.work + (long long)tile.part * kOzWorkSlot;
 #pragma unroll
             for (int i = 0; i < FM; ++i)
 #pragma unroll
@@ -267,26 +270,30 @@ __global__ __launch_bounds__(256, 2) void gemm_f64_packed_kernel(OzGemmArgs a) {
     }
 }
 
-// Split tiles: one thread per element of a split tile (64 workgroups of 256
-// per tile) sums the tile's segments in segment order -- the layout the GEMM
-// wrote: element e = (((wave * 4 + i) * 4 + j) * 4 + r) * 64 + lane holds
-// fragment (i, j, r) of that wave and lane -- and stores the sum like the
-// GEMM epilogue.  One element per thread keeps thousands of the dependent
-// destination lookups in flight.
+// Split tiles: one thread per element of a split tile (BM * BN / 256
+// workgroups of 256 per tile) sums the tile's segments in segment order --
+// the layout the GEMM wrote: element e = (((wave * FM + i) * FN + j) * 4 + r)
+// * 64 + lane holds fragment (i, j, r) of that wave and lane -- and stores
+// the sum like the GEMM epilogue.  One element per thread keeps thousands of
+// the dependent destination lookups in flight.
+template <int BM, int BN>
 __global__ __launch_bounds__(256) void fixup_f64_kernel(OzGemmArgs a) {
-    constexpr int BM = 128, BN = 128, TM = 64, TN = 64;
+    constexpr int TM = BM / 2, TN = BN / 2, FM = TM / 16, FN = TN / 16;
+    constexpr int LFM = FM == 4 ? 2 : 1, LFN = FN == 4 ? 2 : 1; // log2
+    static_assert((FM == 4 || FM == 2) && (FN == 4 || FN == 2), "tile shape");
     constexpr int kPer = BM * BN / 256; // workgroups per tile
     const long long nwork = (long long)a.nfix * kPer;
     for (long long w = blockIdx.x; w < nwork; w += gridDim.x) {
         const OzTile f = a.fix[w / kPer];
         const int e = (int)(w % kPer) * 256 + threadIdx.x;
-        const int lane = e & 63, r = (e >> 6) & 3, j = (e >> 8) & 3, i = (e >> 10) & 3;
-        const int wave = e >> 12, wm = wave & 1, wn = wave >> 1;
+        const int lane = e & 63, r = (e >> 6) & 3, j = (e >> 8) & (FN - 1);
+        const int i = (e >> (8 + LFN)) & (FM - 1);
+        const int wave = e >> (8 + LFN + LFM), wm = wave & 1, wn = wave >> 1;
         const OzTask task = a.tasks[f.task];
         const int m = f.mt * BM + wm * TM + i * 16 + 4 * r + (lane >> 4);
         const int n = f.nt * BN + wn * TN + j * 16 + (lane & 15);
         if (m >= task.M || n >= task.N) continue;
-        const double *w0 = a.work + (long long)f.part * (BM * BN) + e;
+        const double *w0 = a.work + (long long)f.part * kOzWorkSlot + e;
         // segments summed in order; loaded eight at a time so the loads are
         // in flight together
         double v = 0.0;
@@ -294,7 +301,7 @@ __global__ __launch_bounds__(256) void fixup_f64_kernel(OzGemmArgs a) {
             double x[8];
 #pragma unroll
             for (int q = 0; q < 8; ++q)
-                x[q] = q0 + q < f.s0 ? w0[(long long)(q0 + q) * (BM * BN)] : 0.0;
+                x[q] = q0 + q < f.s0 ? w0[(long long)(q0 + q) * kOzWorkSlot] : 0.0;
 #pragma unroll
             for (int q = 0; q < 8; ++q)
                 if (q0 + q < f.s0) v = q0 + q == 0 ? x[q] : v + x[q];
@@ -310,9 +317,14 @@ __global__ __launch_bounds__(256) void fixup_f64_kernel(OzGemmArgs a) {
 
 hipError_t oz_fixup_f64(const OzGemmArgs &a, hipStream_t s) {
     if (a.nfix <= 0) return hipSuccess;
-    const long long work = (long long)a.nfix * (kOzBM * kOzBN / 256);
-    hipLaunchKernelGGL(fixup_f64_kernel, dim3((unsigned)(work < 1048576 ? work : 1048576)),
-                       dim3(256), 0, s, a);
+    const int edge = a.tile == kOzSmallTile ? kOzSmallTile : kOzBM;
+    const long long work = (long long)a.nfix * (edge * edge / 256);
+    const dim3 grid((unsigned)(work < 1048576 ? work : 1048576));
+    if (a.tile == kOzSmallTile)
+        hipLaunchKernelGGL((fixup_f64_kernel<kOzSmallTile, kOzSmallTile>), grid, dim3(256), 0, s,
+                           a);
+    else
+        hipLaunchKernelGGL((fixup_f64_kernel<kOzBM, kOzBN>), grid, dim3(256), 0, s, a);
     return hipGetLastError();
 }
 
@@ -324,8 +336,12 @@ hipError_t oz_pack(const OzConvArgs &a, hipStream_t s) {
 
 hipError_t oz_gemm_f64(const OzGemmArgs &a, hipStream_t s) {
     if (a.ntiles <= 0) return hipSuccess;
-    hipLaunchKernelGGL(gemm_f64_packed_kernel, dim3(a.wg_first ? a.nwg : a.ntiles), dim3(256), 0,
-                       s, a);
+    const dim3 grid(a.wg_first ? a.nwg : a.ntiles);
+    if (a.tile == kOzSmallTile)
+        hipLaunchKernelGGL((gemm_f64_packed_kernel<kOzSmallTile, kOzSmallTile>), grid, dim3(256), 0,
+                           s, a);
+    else
+        hipLaunchKernelGGL((gemm_f64_packed_kernel<kOzBM, kOzBN>), grid, dim3(256), 0, s, a);
     return hipGetLastError();
 }
 

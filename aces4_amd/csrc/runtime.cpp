@@ -1277,6 +1277,9 @@ int sipgpu::rt::build_panel_plan(const sipgpu_contraction *list, int n, int nmod
     }
     if (nmod && (opt.tile == kOzBM || opt.tile == kOzBigTile || opt.tile == kOzWideTile))
         tile = opt.tile;
+    // the packed FP64 GEMM: 128 x 128 tiles, or 64 x 64 when chosen (small
+    // batches cut into K segments write a quarter of the partial sums)
+    if (!nmod && opt.tile == kOzSmallTile) tile = kOzSmallTile;
     const int tsize = oz_tile_size(tile);
     // tiles: the same XCD schedule as the FP64 batch
     const char *tail_env = std::getenv("SIPGPU_SPLIT_TAIL"); // "0": one tile per workgroup
@@ -2329,6 +2332,15 @@ int sipgpu_batch_tune(const sipgpu_contraction *list, int n, int nmoduli, sipgpu
                 ch.tail = tail;
                 cands.push_back(ch);
             }
+        for (int merge : {1, 0})
+            for (int tail : {1, 2}) {
+                PlanChoice ch;
+                ch.path = 1;
+                ch.merge = merge;
+                ch.tail = tail;
+                ch.tile = kOzSmallTile;
+                cands.push_back(ch);
+            }
     } else {
         for (int tile : {kOzBM, kOzBigTile, kOzWideTile})
             for (int merge : {1, 0}) {
@@ -2433,7 +2445,8 @@ int sipgpu_batch_plan_choice(const sipgpu_contraction *list, int n, int nmoduli,
     if (!batch || (n > 0 && !list) || n < 0) return fail(SIPGPU_EARG, "bad batch arguments");
     if (path < 0 || path > 2 || (path == 2) != (nmoduli != 0) || config < -1 ||
         config >= kGemmConfigs || tail < -1 || tail > 4 ||
-        (tile != -1 && tile != kOzBM && tile != kOzBigTile && tile != kOzWideTile))
+        (path == 2 && tile != -1 && tile != kOzBM && tile != kOzBigTile && tile != kOzWideTile) ||
+        (path != 2 && tile != -1 && !(path == 1 && tile == kOzSmallTile)))
         return fail(SIPGPU_EARG, "bad plan choice");
     if (nmoduli != 0 && (nmoduli < 6 || nmoduli > 14))
         return fail(SIPGPU_EARG, "split path: nmoduli must be in 6..14");

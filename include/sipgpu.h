@@ -303,7 +303,9 @@ int sipgpu_batch_execute_stages(sipgpu_batch_t batch, int stages, sipgpu_stream_
  * panels with and without merged result blocks, without the persistent
  * water-filled schedule (tail 0) and with it over 64, 32, 16 or 8
  * workgroups per XCD (tail 1, 2, 3, 4: fewer workgroups cut a small batch's
- * tiles into fewer K segments); for the split path (nmoduli > 0) the
+ * tiles into fewer K segments), in 128 x 128 tiles and in 64 x 64 tiles
+ * (tile 64: a small batch's K segments write a quarter of the partial
+ * sums); for the split path (nmoduli > 0) the
  * three INT8 GEMMs (tile 128: 128 x 128, four waves; 256: 256 x 256, four
  * waves of 128 x 128; 257: 256 x 256, eight waves of 64 x 128) with and
  * without merging -- times each (one warm-up and five
@@ -325,7 +327,8 @@ int sipgpu_batch_tune(const sipgpu_contraction *list, int n, int nmoduli, sipgpu
                       sipgpu_tune_result *result);
 /* Plans the batch one given way (the fields of sipgpu_tune_result: path,
  * config for the table kernel, merge / tail / tile for the panel paths, -1
- * for a default): pins a plan, e.g. one found by tuning elsewhere. */
+ * for a default; tile 64 or -1 for packed FP64, 128 / 256 / 257 or -1 for
+ * the split path): pins a plan, e.g. one found by tuning elsewhere. */
 int sipgpu_batch_plan_choice(const sipgpu_contraction *list, int n, int nmoduli, int path,
                              int config, int merge, int tail, int tile, sipgpu_batch_t *batch);
 /* The tuning table as text (one "signature path config merge tail tile" line

@@ -3,7 +3,7 @@
 Every plan the tuner may pick is checked against the oracle first: the
 table kernel in each tile configuration the tuner times, and the packed
 panels with and without merged result blocks and the persistent schedule
-(over 64, 32, 16 and 8 workgroups per XCD).
+(over 64, 32, 16 and 8 workgroups per XCD), in 128 x 128 and 64 x 64 tiles.
 The tuner must leave the caller's blocks untouched, return a plan that
 matches the oracle, remember its choice by batch signature (also through a
 saved and reloaded table) and never pick a plan slower than the untuned one
@@ -52,12 +52,13 @@ def test_table_config_vs_oracle(make, cfg):
 
 
 @pytest.mark.parametrize("make", list(MAKERS), ids=list(MAKERS))
-@pytest.mark.parametrize("merge,tail", [(1, 1), (1, 0), (0, 1), (0, 0), (0, 2), (1, 2), (0, 3),
-                                       (1, 4)])
-def test_packed_choice_vs_oracle(make, merge, tail):
+@pytest.mark.parametrize("merge,tail,tile", [(1, 1, -1), (1, 0, -1), (0, 1, -1), (0, 0, -1),
+                                            (0, 2, -1), (1, 2, -1), (0, 3, -1), (1, 4, -1),
+                                            (1, 1, 64), (0, 2, 64), (0, 0, 64), (1, 4, 64)])
+def test_packed_choice_vs_oracle(make, merge, tail, tile):
     w = MAKERS[make]()
     prefill(w)
-    b = sg.plan_choice(contractions(w), "packed", merge=merge, tail=tail)
+    b = sg.plan_choice(contractions(w), "packed", merge=merge, tail=tail, tile=tile)
     assert b.kind == "packed"
     b.execute()
     sg.synchronize()
@@ -123,10 +124,10 @@ def test_tune_load_rejects_malformed(tmp_path):
         sg.tune_load(tmp_path / "missing.txt")
 
 
-def test_plan_choice_rejects_unknown_tail():
+def test_plan_choice_rejects_unknown_tail_and_tile():
     w = wl.ladder()
     cs = contractions(w)
-    for tail in (-2, 5):
+    for tail, tile in ((-2, -1), (5, -1), (1, 128), (1, 257)):
         with pytest.raises(sg.SipGpuError) as e:
-            sg.plan_choice(cs, "packed", tail=tail)
+            sg.plan_choice(cs, "packed", tail=tail, tile=tile)
         assert e.value.code == -107  # SIPGPU_EARG
