@@ -56,7 +56,8 @@ namespace {
 
 using Clock = std::chrono::steady_clock;
 
-// First-fit sub-allocator over one device region, 256-byte granules.
+// First-fit sub-allocator over one region (device arena or pinned host
+// slab), 256-byte granules.
 struct Arena {
     char *base = nullptr;
     size_t size = 0;
@@ -111,7 +112,8 @@ struct CBlock {
     size_t elems = 0;
     size_t off = kNotResident;     // arena offset while resident
     double *home = nullptr;        // host copy: the caller's, or a pinned spill buffer
-    bool home_owned = false;
+    bool home_owned = false;       // home is a range of pinned slab home_slab
+    int home_slab = -1;
     bool home_valid = false;       // home holds the block's current value
     int init = SIPGPU_CACHE_UNDEFINED;
     unsigned long long seed = 0, stream = 0;
@@ -142,6 +144,10 @@ struct sipgpu_cache_s {
     void *dev = nullptr;
     hipStream_t copy = nullptr;                 // spill (device -> host) copies
     hipStream_t h2d = nullptr;                  // refill (host -> device) copies
+    // Spill buffers: ranges of pinned host slabs (one pinned allocation per
+    // slab, not per block: pinning is slow, and a stream of first spills
+    // would otherwise pay it per block).
+    std::vector<Arena> slabs;
     std::unordered_map<unsigned long long, CBlock> blocks;
     std::list<unsigned long long> lru;          // front: least recently released
     std::vector<Fence> fences;                  // freed ranges whose readers may be in flight
@@ -173,6 +179,46 @@ void drop_spilled(CBlock &b) {
     b.spilled = nullptr;
 }
 
+constexpr size_t kHostSlab = (size_t)256 << 20; // pinned bytes per slab (at least)
+
+// A pinned spill buffer for block b (bytes long) from the first slab with
+// room, or from a new slab.
+int home_alloc(sipgpu_cache_s &c, CBlock &b, size_t bytes) {
+    size_t off = 0;
+    for (size_t i = 0; i < c.slabs.size(); ++i)
+        if (c.slabs[i].alloc(bytes, off)) {
+            b.home = reinterpret_cast<double *>(c.slabs[i].base + off);
+            b.home_slab = (int)i;
+            b.home_owned = true;
+            return 0;
+        }
+    const size_t n = std::max(Arena::round(bytes), kHostSlab);
+    void *p = nullptr;
+    if (hipHostMalloc(&p, n, hipHostMallocDefault) != hipSuccess || !p) {
+        (void)hipGetLastError();
+        return fail(SIPGPU_ENOMEM, "cache: pinned spill buffer");
+    }
+    Arena a;
+    a.reset(static_cast<char *>(p), n);
+    a.alloc(bytes, off);
+    c.slabs.push_back(std::move(a));
+    b.home = reinterpret_cast<double *>(c.slabs.back().base + off);
+    b.home_slab = (int)c.slabs.size() - 1;
+    b.home_owned = true;
+    return 0;
+}
+
+// Returns block b's spill buffer to its slab (the caller has made sure no
+// copy still reads or writes it).
+void home_free(sipgpu_cache_s &c, CBlock &b) {
+    if (!b.home_owned) return;
+    Arena &a = c.slabs[b.home_slab];
+    a.release((size_t)(reinterpret_cast<char *>(b.home) - a.base), b.elems * sizeof(double));
+    b.home = nullptr;
+    b.home_owned = false;
+    b.home_slab = -1;
+}
+
 // Cache calls are host-side bookkeeping around uploads, spills, fences and
 // synchronisation: none of it may be recorded into a graph (a replay would
 // also reuse arena offsets that a later eviction hands to another block).
@@ -193,10 +239,7 @@ int evict_locked(sipgpu_cache_s &c, unsigned long long key, CBlock &b) {
     const size_t bytes = b.elems * sizeof(double);
     if (b.initialised && !b.home_valid) {
         if (!b.home) {
-            hipError_t e = hipHostMalloc((void **)&b.home, std::max<size_t>(bytes, 8),
-                                         hipHostMallocDefault);
-            if (e != hipSuccess) return fail(SIPGPU_ENOMEM, "cache: pinned spill buffer");
-            b.home_owned = true;
+            if (int rc = home_alloc(c, b, bytes)) return rc;
         }
         if (int rc = wait_uses(c.copy, b)) return rc;
         SIPGPU_HIP_TRY(hipMemcpyAsync(b.home, c.arena.base + b.off, bytes, hipMemcpyDeviceToHost,
@@ -254,7 +297,7 @@ int remove_locked(sipgpu_cache_s &c, unsigned long long key) {
         // the copy streams may still be filling or reading the spill buffer
         SIPGPU_HIP_TRY(hipStreamSynchronize(c.copy));
         SIPGPU_HIP_TRY(hipStreamSynchronize(c.h2d));
-        SIPGPU_HIP_TRY(hipHostFree(b.home));
+        home_free(c, b);
     }
     drop_uses(b);
     drop_spilled(b);
@@ -415,10 +458,10 @@ int sipgpu_cache_destroy(sipgpu_cache_t c) {
     if (!c) return fail(SIPGPU_EARG, "null cache");
     SIPGPU_HIP_TRY(hipDeviceSynchronize());
     for (auto &kv : c->blocks) {
-        if (kv.second.home_owned) (void)hipHostFree(kv.second.home);
         drop_uses(kv.second);
         drop_spilled(kv.second);
     }
+    for (Arena &a : c->slabs) (void)hipHostFree(a.base);
     for (const Fence &f : c->fences) (void)hipEventDestroy(f.ev);
     (void)hipStreamDestroy(c->copy);
     (void)hipStreamDestroy(c->h2d);
@@ -498,10 +541,7 @@ int sipgpu_cache_flush(sipgpu_cache_t c, unsigned long long key, int all) {
         if (b.off == kNotResident || !b.initialised || b.home_valid) continue;
         const size_t bytes = b.elems * sizeof(double);
         if (!b.home) {
-            hipError_t e = hipHostMalloc((void **)&b.home, std::max<size_t>(bytes, 8),
-                                         hipHostMallocDefault);
-            if (e != hipSuccess) return fail(SIPGPU_ENOMEM, "cache: pinned spill buffer");
-            b.home_owned = true;
+            if (int rc = home_alloc(*c, b, bytes)) return rc;
         }
         SIPGPU_HIP_TRY(hipMemcpy(b.home, c->arena.base + b.off, bytes, hipMemcpyDeviceToHost));
         b.home_valid = true;
