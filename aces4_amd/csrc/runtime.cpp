@@ -2494,7 +2494,11 @@ int sipgpu_tune_load(const char *path) {
         PlanChoice ch;
         if (std::sscanf(line, "%llx %d %d %d %d %d", &sig, &ch.path, &ch.cfg, &ch.merge, &ch.tail,
                         &ch.tile) != 6 ||
-            ch.path < 0 || ch.path > 2 || ch.cfg < -1 || ch.cfg >= kGemmConfigs) {
+            ch.path < 0 || ch.path > 2 || ch.cfg < -1 || ch.cfg >= kGemmConfigs ||
+            ch.merge < -1 || ch.merge > 1 || ch.tail < -1 || ch.tail > 4 ||
+            (ch.tile != -1 && !(ch.path == 1 && ch.tile == kOzSmallTile) &&
+             !(ch.path == 2 && (ch.tile == kOzBM || ch.tile == kOzBigTile ||
+                                ch.tile == kOzWideTile)))) {
             ++bad;
             continue;
         }

@@ -317,3 +317,18 @@ def test_slice_plans_beyond_the_sip_rank():
     with pytest.raises(sg.SipGpuError) as ei:
         sg.plan_slice([3] * 9, [2] * 9, [0] * 9)
     assert ei.value.code == -102
+
+
+def test_tune_load_validates_fields(tmp_path):
+    """A tuning table line is 'signature path config merge tail tile'; out of
+    range fields (tail 5, a split tile on the packed path, an unknown tile)
+    are refused, well-formed lines load (host-only, no device work)."""
+    good = tmp_path / "good.txt"
+    good.write_text("# sipgpu tuning table\n1a 1 -1 0 2 64\n2b 2 -1 1 -1 257\n3c 0 5 -1 -1 -1\n")
+    sg.tune_load(good)
+    for line in ("4d 1 -1 0 5 -1", "5e 1 -1 0 1 257", "6f 2 -1 0 -1 64", "70 1 -1 2 1 -1"):
+        bad = tmp_path / "bad.txt"
+        bad.write_text(line + "\n")
+        with pytest.raises(sg.SipGpuError):
+            sg.tune_load(bad)
+    sg.tune_clear()
