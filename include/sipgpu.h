@@ -223,8 +223,9 @@ int sipgpu_contract(sipgpu_stream_t s, const int *ptrn, const double *l,
  * held until the process ends.  Preconditions: the library is initialised
  * (SIPGPU_EARG otherwise) and the shape (pattern and extents) has been run
  * once through sipgpu_contract, which uploads its offset tables
- * (SIPGPU_ENOTFOUND otherwise); a workspace need beyond 16 MiB is refused
- * with SIPGPU_ESIZE.  All calls on one device share that array: calls that
+ * (SIPGPU_ENOTFOUND otherwise).  Both entry points cap a contraction's K
+ * split so its segments fit the 16 MiB (a larger need would be refused with
+ * SIPGPU_ESIZE).  All calls on one device share that array: calls that
  * overlap on the device (other streams, other threads) race. */
 int sipgpu_contract_noalloc(sipgpu_stream_t s, const int *ptrn, const double *l,
                             int lrank, const int *ldims, const double *r,
