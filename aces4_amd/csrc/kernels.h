@@ -221,6 +221,7 @@ struct TriplesArgs {
     int wdims[6];
 };
 hipError_t launch_triples_pack(const TriplesPackArgs &a, hipStream_t s);
+// grid workgroups of 512; hipErrorInvalidConfiguration at 2^23 workgroups or more
 hipError_t launch_triples_boxes(const TriplesArgs &a, long long grid, hipStream_t s);
 // out[0] += factor * sum of partial[0 .. n) in a fixed order (sums: kTriplesSums doubles)
 hipError_t launch_triples_reduce(const double *partial, long long n, double *sums, double factor,

@@ -271,10 +271,16 @@ int sipgpu_triples_run(sipgpu_triples_t P, long long box0, long long nbox, sipgp
     if (nbox == 0) return 0;
     sipgpu::TriplesArgs a = box_args(*P);
     a.partial = P->partial;
-    a.box0 = box0;
-    a.nbox = nbox;
-    a.per_xcd = (nbox + 7) / 8;
-    SIPGPU_HIP_TRY(sipgpu::launch_triples_boxes(a, 8 * a.per_xcd, sipgpu::rt::stream_of(s)));
+    const hipStream_t hs = sipgpu::rt::stream_of(s);
+    // one launch per SIPGPU_TRIPLES_LAUNCH_BOXES slots: a dispatch holds
+    // fewer than 2^32 work-items (8.4M boxes of 512), and a launch stays
+    // near a second of the whole sum's ~20 s on one GPU
+    for (long long b = box0; b < box0 + nbox; b += SIPGPU_TRIPLES_LAUNCH_BOXES) {
+        a.box0 = b;
+        a.nbox = std::min<long long>(SIPGPU_TRIPLES_LAUNCH_BOXES, box0 + nbox - b);
+        a.per_xcd = (a.nbox + 7) / 8;
+        SIPGPU_HIP_TRY(sipgpu::launch_triples_boxes(a, 8 * a.per_xcd, hs));
+    }
     return 0;
 }
 

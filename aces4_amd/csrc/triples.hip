@@ -365,6 +365,8 @@ hipError_t launch_triples_pack(const TriplesPackArgs &a, hipStream_t s) {
 
 hipError_t launch_triples_boxes(const TriplesArgs &a, long long grid, hipStream_t s) {
     if (grid <= 0) return hipSuccess;
+    // a dispatch holds at most 2^32 - 1 work-items per dimension
+    if (grid * kThreads >= (1LL << 32)) return hipErrorInvalidConfiguration;
     triples_box_kernel<<<(unsigned)grid, kThreads, 0, s>>>(a);
     return hipGetLastError();
 }

@@ -32,6 +32,8 @@ from .workloads import DeviceBlocks, uniform
 PAIRS = (("a", "i"), ("b", "j"), ("c", "k"))
 W_LABELS = "abcijk"
 W_SIGNS = (-1, -1, -1, 1, 1, 1)
+# box slots per kernel launch of sipgpu_triples_run (SIPGPU_TRIPLES_LAUNCH_BOXES)
+LAUNCH_BOXES = 1 << 21
 
 
 def term_labels():

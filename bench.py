@@ -548,12 +548,14 @@ def measure_traffic(steps, timeout_s=240):
         box = [float(x["Counter_Value"]) for x in rows if "triples_box_kernel" in x["Kernel_Name"]]
         if r.returncode != 0 and not (copy and box):
             return None, "rocprofv3 %s run failed (%d)" % (counter, r.returncode)
-        if not copy or len(box) != len(PROBE_LAUNCHES):
+        if not copy or len(box) < len(PROBE_LAUNCHES):
             return None, "rocprofv3 %s: calibration or kernel dispatches missing" % counter
         scale = known / (1024.0 * float(np.median(copy)))  # counter reads KiB
-        out[counter] = {"kib_per_launch": [float(b) for b in box], "calibration_kib": copy[0],
-                        "scale": scale, "bytes_per_launch": float(np.mean(box)) * 1024.0 * scale}
-    total = out["FETCH_SIZE"]["bytes_per_launch"] + out["WRITE_SIZE"]["bytes_per_launch"]
+        # bytes per step: a step's boxes may take several kernel launches
+        per_step = float(np.sum(box)) / len(PROBE_LAUNCHES)
+        out[counter] = {"kib_per_dispatch": [float(b) for b in box], "calibration_kib": copy[0],
+                        "scale": scale, "bytes_per_step": per_step * 1024.0 * scale}
+    total = out["FETCH_SIZE"]["bytes_per_step"] + out["WRITE_SIZE"]["bytes_per_step"]
     return total, out
 
 
@@ -594,10 +596,14 @@ def triples_fused_bench(steps, warmup):
     energy = float(out.download()[0])
     v, o = sum(w["v_segs"]), sum(w["o_segs"])
     flops = ft.flops  # 12 terms x 2 v^3 o^3 (v or o): every box is full here
-    per_launch = flops * chunk / ft.slots
+    # sipgpu_triples_run makes one kernel launch per tp.LAUNCH_BOXES slots:
+    # the roofline divides a step by its launches (one at 12 steps or more)
+    kernels = -(-chunk // tp.LAUNCH_BOXES)
+    per_launch = flops * chunk / ft.slots / kernels
     res = {"wall_s": wall, "flops": flops, "energy": energy, "pack_ms": pack_ms,
-           "slots": ft.slots, "boxes_per_launch": chunk,
-           "launch_ms": launch_ms, "flop_per_launch": per_launch,
+           "slots": ft.slots, "boxes_per_launch": chunk / kernels, "boxes_per_step": chunk,
+           "kernels_per_step": kernels,
+           "launch_ms": launch_ms / kernels, "flop_per_launch": per_launch,
            "panel_bytes": ft.panel_bytes,
            "mfma_flop_issued_per_box": ft.mfma_flop_per_box,
            "algorithmic_flop_per_box": flops / ft.slots,
@@ -847,6 +853,7 @@ def main():
     achieved = tri["flop_per_launch"] / avg_launch_s / 1e12
     per_box = traffic_from_profile(PROFILE_TRIPLES_TRAFFIC, "hbm_bytes_per_box")
     traffic = per_box * tri["boxes_per_launch"] if per_box else None
+    kernels = tri["kernels_per_step"]
     traffic_src = {"source": "profiles/triples_box_traffic.txt (committed rocprofv3 --pmc runs)"}
     shape_classes = [(0, 1, 2, i, j, k) for i in (0, 1) for j in (0, 1) for k in (0, 1)]
     tri_tuples = None if a.no_triples else triples_tuple_bench(w5, ft, shape_classes)
@@ -889,7 +896,7 @@ def main():
         live, info = measure_traffic(a.steps)
         progress("traffic: %s" % ("%.3e bytes per launch" % live if live else info))
         if live is not None:
-            traffic = live
+            traffic = live / kernels
             traffic_src = {"source": "measured in this run: rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE "
                                      "child runs of launches %s of the same chunk size, scaled by the "
                                      "calibration copy" % (list(PROBE_LAUNCHES),), "counters": info}
@@ -931,7 +938,8 @@ def main():
                            "energy reduced in the epilogue); operands packed once into MFMA-order "
                            "panels",
                    "pack_ms": tri["pack_ms"], "slots": tri["slots"],
-                   "boxes_per_step": tri["boxes_per_launch"],
+                   "boxes_per_step": tri["boxes_per_step"],
+                   "kernel_launches_per_step": kernels,
                    "tflops_including_pack": tri["flops"] / (tri["wall_s"] + tri["pack_ms"] * 1e-3)
                    / 1e12},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,

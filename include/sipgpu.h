@@ -626,12 +626,16 @@ int sipgpu_permute_rhs_to_lhs(int lrank, const int *lhs_ids, const int *rhs_ids,
  * (sipgpu_triples_info: their bytes, ~4.3 GB for o = 30, v = 240) and one
  * energy partial per box slot; sipgpu_triples_pack (re)packs them from the
  * blocks and zeroes the partials; sipgpu_triples_run computes the partials of
- * box slots [box0, box0 + nbox) (all `slots` slots cover the whole sum);
+ * box slots [box0, box0 + nbox) (all `slots` slots cover the whole sum), in
+ * one kernel launch per SIPGPU_TRIPLES_LAUNCH_BOXES slots (a launch of
+ * 512-thread workgroups holds fewer than 2^32 work-items; 2^21 boxes of
+ * o = 30, v = 240 run ~1.8 s on one MI355X);
  * sipgpu_triples_energy adds factor * (sum of all partials, in slot order)
  * to *out (a device double).  sipgpu_triples_tuple computes one tuple's
  * energy into *out (+=) and, if w_out is not null, writes its W block
  * (column-major a, b, c, i, j, k).  All calls on one plan must be ordered
  * (one stream). */
+#define SIPGPU_TRIPLES_LAUNCH_BOXES (1LL << 21)
 typedef struct sipgpu_triples_s *sipgpu_triples_t;
 int sipgpu_triples_create(int nv, const int *v_seg, int no, const int *o_seg,
                           const double *const *T, const double *const *V1,

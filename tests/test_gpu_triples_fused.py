@@ -106,7 +106,8 @@ def test_fused_full_size_tuple():
     """One 20^6 tuple of the triples configuration (o = 30 in 20, 10; v = 240
     in 12 x 20): its energy against the twelve-batch FP64 path (oracle-pinned
     at small sizes) within 1e-13, and 64 sampled W elements against direct
-    dot products of the twelve terms from the blocks."""
+    dot products of the twelve terms from the blocks; a slot range across a
+    kernel-launch boundary against two calls that stop at it."""
     w = tp.triples()
     tp.to_device(w)
     ft = tp.FusedTriples(w)
@@ -131,6 +132,22 @@ def test_fused_full_size_tuple():
     E, E2 = out.download()[0], out2.download()[0]
     assert abs(E - E2) <= 1e-13 * abs(E2)
     te.destroy()
+    # a range across a launch boundary of sipgpu_triples_run (one kernel
+    # launch per tp.LAUNCH_BOXES slots) gives the bits of two calls that
+    # stop at the boundary
+    print("fused range across a launch boundary", flush=True)
+    assert ft.slots > tp.LAUNCH_BOXES + 1000
+    sums = []
+    for cuts in ([(tp.LAUNCH_BOXES - 1000, 2000)],
+                 [(tp.LAUNCH_BOXES - 1000, 1000), (tp.LAUNCH_BOXES, 1000)]):
+        ft.pack()  # zeroes the partials
+        for b0, nb in cuts:
+            ft.run(b0, nb)
+        out.zero()
+        ft.energy(out.ptr)
+        sg.synchronize()
+        sums.append(out.download()[0])
+    assert sums[0] == sums[1] and sums[0] != 0.0
     ft.destroy()
     # direct dot products of the twelve terms
     segs = {"v": w["v_segs"], "o": w["o_segs"]}

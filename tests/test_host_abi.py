@@ -332,3 +332,14 @@ def test_tune_load_validates_fields(tmp_path):
         with pytest.raises(sg.SipGpuError):
             sg.tune_load(bad)
     sg.tune_clear()
+
+
+def test_triples_launch_size_matches_the_header():
+    """sipgpu_triples_run cuts a slot range into launches of
+    SIPGPU_TRIPLES_LAUNCH_BOXES slots; the Python mirror agrees, and a launch
+    of 512-thread workgroups stays below 2^32 work-items."""
+    from aces4_amd import triples as tp
+    text = open(os.path.join(ROOT, "include", "sipgpu.h")).read()
+    m = re.search(r"#define SIPGPU_TRIPLES_LAUNCH_BOXES \(1LL << (\d+)\)", text)
+    assert m and tp.LAUNCH_BOXES == 1 << int(m.group(1))
+    assert (tp.LAUNCH_BOXES + 7) // 8 * 8 * 512 < 1 << 32
