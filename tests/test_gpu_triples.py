@@ -94,6 +94,22 @@ def test_denominator_kernel_bitwise():
     assert abs(out.download()[0] - ref) <= 1e-14 * abs(ref)
 
 
+def test_denominator_kernel_matches_the_reference_loops():
+    """The GPU denominator against the reference's do_rhf_den2/4/6 outputs
+    (tests/golden/reference_energy_den.npz, non-dyadic energies), bit for bit."""
+    from test_oracle_energy import den_cases
+    for routine, dims, signs, eps, x, want in den_cases():
+        print("denominator fixture", routine, dims, flush=True)
+        xd = sg.DeviceArray.from_host(x)
+        ed = [sg.DeviceArray.from_host(e) for e in eps]
+        sg.energy_denominator(dims, xd.ptr, [e.ptr for e in ed], signs)
+        sg.synchronize()
+        assert np.array_equal(xd.download(), want), routine
+        xd.free()
+        for e in ed:
+            e.free()
+
+
 @pytest.mark.parametrize("moduli", [None, 14], ids=["fp64", "split14"])
 def test_tuple_with_partial_buffers_is_bitwise_equal(small, moduli):
     """Batches assigned into separate buffers and summed in batch order give

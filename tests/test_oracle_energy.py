@@ -3,14 +3,59 @@
 energy_denominator (energy_denominator_rhf.F:399-444) and fcontract
 (tensor_dil_omp.F90:910-938) are pinned by exact cases: small-integer and
 dyadic energies make every denominator exact in any summation order, so the
-index-to-sign mapping and the division are checked exactly; the
-left-to-right order of the sums is pinned by reading the cited lines.
+index-to-sign mapping and the division are checked exactly.  The
+left-to-right order of the sums is pinned by the reference's own loops:
+tests/golden/reference_energy_den.npz holds do_rhf_den2/4/6 outputs on
+non-dyadic energies (written by tests/golden/make_energy_golden.py from the
+reference build), which the oracle must reproduce bit for bit while two
+plausible other orders do not.
 """
+import json
+import os
+
 import numpy as np
+import pytest
 
 from checker import Oracle
 
 o = Oracle()
+HERE = os.path.dirname(os.path.abspath(__file__))
+DEN_GOLDEN = os.path.join(HERE, "golden", "reference_energy_den.npz")
+
+
+def den_cases():
+    z = np.load(DEN_GOLDEN)
+    meta = json.loads(str(z["meta"]))
+    out = []
+    for n, c in enumerate(meta["cases"]):
+        dims = [int(d) for d in z["c%d_dims" % n]]
+        out.append((c["routine"], dims, [int(s) for s in z["c%d_signs" % n]],
+                    [z["c%d_eps%d" % (n, d)] for d in range(len(dims))],
+                    z["c%d_x" % n], z["c%d_out" % n]))
+    return out
+
+
+@pytest.mark.parametrize("case", den_cases(), ids=lambda c: "%s-%s" % (c[0], "x".join(map(str, c[1]))))
+def test_denominator_matches_the_reference_loops(case):
+    """Bit-exact against do_rhf_den2/4/6; for rank 4 and 6 the fixture tells
+    the reference's order ((b + d + f) - a - c - e, left to right) from
+    (b + d + f) - (a + c + e) and from index order (-a + b - c + d ...)."""
+    routine, dims, signs, eps, x, want = case
+    got = o.energy_denominator(dims, x.copy(), eps, signs)
+    assert np.array_equal(got, want)
+    if len(dims) < 4:
+        return
+    grids = np.meshgrid(*[np.arange(d) for d in dims], indexing="ij")
+    e = [eps[d][grids[d]] for d in range(len(dims))]
+    occ = sum(e[d] for d in range(len(dims)) if signs[d] > 0)
+    vir = sum(e[d] for d in range(len(dims)) if signs[d] < 0)
+    grouped = x / (occ - vir).ravel(order="F")
+    inorder = np.zeros(dims)
+    for d in range(len(dims)):
+        inorder = inorder + signs[d] * e[d]
+    inorder = x / inorder.ravel(order="F")
+    assert not np.array_equal(grouped, want)
+    assert not np.array_equal(inorder, want)
 
 
 def test_denominator_rank6_exact():
