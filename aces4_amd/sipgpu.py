@@ -878,6 +878,10 @@ class Batch:
     `moduli` set, the split-precision (INT8 / CRT) path over that many moduli."""
 
     def __init__(self, contractions, moduli=None, _handle=None, tol=None):
+        # the batch reads and writes the blocks these entries address: holding
+        # the list holds whatever it keeps alive (workloads.contractions_for
+        # attaches the workload that owns the device arrays)
+        self.contractions = contractions
         arr = (Contraction * len(contractions))(*contractions)
         p = _vp()
         if _handle is not None:

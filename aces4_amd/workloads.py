@@ -159,16 +159,27 @@ def ladder(o_segs=(10,), v_segs=(10, 10, 10, 10), seed=1):
                 labels=("abij", "abcd", "cdij"))
 
 
+class ContractionList(list):
+    """Contractions on a workload's device arrays; `owner` (the workload)
+    stays alive with the list, so the arrays are not freed while a batch
+    planned from the list can still read them."""
+    owner = None
+
+
 def contractions_for(w, accumulate=True):
     """The workload's block contractions on its device arrays (uploaded on
-    demand)."""
+    demand).  The list holds the workload: a batch planned from it (which
+    keeps the list) keeps the arrays."""
     for k in ("D", "L", "R"):
         if w[k].dev is None:
             w[k].to_device()
     D, L, R, ptrn = w["D"], w["L"], w["R"], w["ptrn"]
-    return [sg.make_contraction(ptrn, L.addr(lb), L.dims(lb), R.addr(rb), R.dims(rb), D.addr(db),
-                                D.dims(db), accumulate=accumulate)
-            for db, lb, rb in w["entries"]]
+    out = ContractionList(
+        sg.make_contraction(ptrn, L.addr(lb), L.dims(lb), R.addr(rb), R.dims(rb), D.addr(db),
+                            D.dims(db), accumulate=accumulate)
+        for db, lb, rb in w["entries"])
+    out.owner = w
+    return out
 
 
 def batch_for(w, accumulate=True, moduli=None):

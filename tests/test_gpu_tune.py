@@ -100,6 +100,23 @@ def test_tuner_picks_a_plan_and_remembers_it(tmp_path):
     sg.tune_clear()
 
 
+def test_batch_from_a_temporary_workload_keeps_its_blocks():
+    """A batch tuned from the contractions of a workload nobody else holds
+    keeps that workload (and its device arrays) alive: executing it after a
+    garbage collection reads and writes live blocks, and matches the oracle."""
+    import gc
+    sg.tune_clear()
+    b, res = sg.tune(wl.contractions_for(wl.ladder()))
+    gc.collect()
+    print("temporary workload", res, flush=True)
+    b.execute()
+    sg.synchronize()
+    w = b.contractions.owner
+    assert per_block_err(w, w["D"].dev.download(), oracle_batch(w)) <= 1e-13
+    b.destroy()
+    sg.tune_clear()
+
+
 def test_tuner_split_path():
     sg.tune_clear()
     w = wl.ring(o_segs=(9, 6), v_segs=(14, 11, 8))

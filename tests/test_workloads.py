@@ -55,3 +55,41 @@ def test_sparse_ladder_structure(sl):
 def test_sparse_ladder_pattern(sl):
     # R(a,b,i,j) = V(a,b,c,d) T(c,d,i,j): a->1, b->2, c,d contracted with T's 1,2
     assert sl["ptrn"] == [1, 2, -1, -2, -3, -4, 3, 4]
+
+
+def test_contraction_list_keeps_the_workload_alive():
+    """A contraction list made from a temporary workload keeps it, so its
+    device arrays are not freed while a batch planned from the list (which
+    holds the list) can read them.  Host-only: the device arrays are stand-in
+    objects that record when they are released."""
+    import gc
+
+    released = []
+
+    class FakeDev:
+        def __init__(self, name):
+            self.name = name
+
+        def at(self, offset):
+            return (1 << 20) + 8 * int(offset)
+
+        def __del__(self):
+            released.append(self.name)
+
+    def temporary():
+        w = wl.ladder()
+        for k in ("D", "L", "R"):
+            w[k].dev = FakeDev(k)
+        return w
+
+    cs = wl.contractions_for(temporary())
+    gc.collect()
+    assert released == [] and len(cs) == 256
+    held = type("Holder", (), {})()
+    held.contractions = cs  # what sipgpu.Batch keeps
+    del cs
+    gc.collect()
+    assert released == []
+    del held
+    gc.collect()
+    assert sorted(released) == ["D", "L", "R"]
