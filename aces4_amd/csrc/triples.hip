@@ -159,6 +159,7 @@ __global__ __launch_bounds__(kThreads) void triples_box_kernel(TriplesArgs a) {
     __shared__ double eps[6][kBoxE];
     __shared__ int ext[6];
     __shared__ double red[kThreads / 64];
+    __shared__ double occ[kBoxRows];
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = wave & 1, wn = wave >> 1;
@@ -292,11 +293,37 @@ __global__ __launch_bounds__(kThreads) void triples_box_kernel(TriplesArgs a) {
             step(2);
         }
     }
+    // energy of the box: sum over its elements inside the blocks' extents.
+    // Element e = abc + 125 ijk (abc = a + 5 b + 25 c, ijk = i + 5 j + 25 k).
+    // D = ((((e_i + e_j) + e_k) - e_a) - e_b) - e_c: the occupied sums of the
+    // box's 125 (i, j, k) are formed once (NaN marks an (i, j, k) outside the
+    // extents); a thread keeps one abc, its virtual energies in registers,
+    // and runs over ijk.
+    double sum = 0.0;
+    if (!a.wout) {
+        if (tid < kBoxRows) {
+            const int i = tid % kBoxE, j = (tid / kBoxE) % kBoxE, k = tid / (kBoxE * kBoxE);
+            const bool in = i < ext[3] && j < ext[4] && k < ext[5];
+            occ[tid] = in ? eps[3][i] + eps[4][j] + eps[5][k] : __builtin_nan("");
+        }
+        __syncthreads(); // the box and the occupied sums are complete
+        const int abc = tid & 127;
+        const int ea_i = abc % kBoxE, eb_i = (abc / kBoxE) % kBoxE, ec_i = abc / (kBoxE * kBoxE);
+        if (abc < kBoxRows && ea_i < ext[0] && eb_i < ext[1] && ec_i < ext[2]) {
+            const double ea = eps[0][ea_i], eb = eps[1][eb_i], ec = eps[2][ec_i];
+            for (int ijk = tid >> 7; ijk < kBoxRows; ijk += kThreads / 128) {
+                const double o = occ[ijk];
+                if (o != o) continue; // (i, j, k) outside the extents
+                const double w = W[abc + kBoxRows * ijk];
+                const double D = ((o - ea) - eb) - ec;
+                sum += (w / D) * w;
+            }
+        }
+    }
     __syncthreads();
 
-    // energy of the box: sum over its elements inside the blocks' extents
-    double sum = 0.0;
-    for (int e = tid; e < kBoxElems; e += kThreads) {
+    // tuple mode with W written out: every element's global position
+    for (int e = a.wout ? tid : kBoxElems; e < kBoxElems; e += kThreads) {
         int q = e, dg[6];
         bool in = true;
 #pragma unroll
