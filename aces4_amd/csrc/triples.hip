@@ -160,6 +160,8 @@ __global__ __launch_bounds__(kThreads) void triples_box_kernel(TriplesArgs a) {
     __shared__ int ext[6];
     __shared__ double red[kThreads / 64];
     __shared__ double occ[kBoxRows];
+    // box offsets of product row / column x (< 128) under permutation g
+    __shared__ int rofs[6][128], cofs[6][128];
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = wave & 1, wn = wave >> 1;
@@ -179,6 +181,15 @@ __global__ __launch_bounds__(kThreads) void triples_box_kernel(TriplesArgs a) {
         const TriplesBlk b = d < 3 ? a.vblk[bc[d]] : a.oblk[bc[d]];
         const double *e = d < 3 ? a.e_vir : a.e_occ;
         eps[d][q] = q < b.ext ? e[b.gstart + q] : 0.0;
+    }
+
+    for (int t = tid; t < 6 * 128; t += kThreads) {
+        const int g = t >> 7, x = t & 127;
+        const int p0 = kPerm[g][0], p1 = kPerm[g][1], p2 = kPerm[g][2];
+        const int d0 = x % kBoxE, d1 = (x / kBoxE) % kBoxE, d2 = x / (kBoxE * kBoxE);
+        // row (x1, y1, y2), column (x2, x3, y3)
+        rofs[g][x] = d0 * box_stride(p0) + d1 * box_stride(3 + p0) + d2 * box_stride(3 + p1);
+        cofs[g][x] = d0 * box_stride(p1) + d1 * box_stride(p2) + d2 * box_stride(3 + p2);
     }
 
     const int nkp = a.nkp;
@@ -238,16 +249,13 @@ __global__ __launch_bounds__(kThreads) void triples_box_kernel(TriplesArgs a) {
     auto scatter = [&](int g) {
         // the previous P's adds into the box are complete
         __syncthreads();
-        const int p0 = kPerm[g][0], p1 = kPerm[g][1], p2 = kPerm[g][2];
-        const int sx1 = box_stride(p0), sy1 = box_stride(3 + p0), sy2 = box_stride(3 + p1);
-        const int sx2 = box_stride(p1), sx3 = box_stride(p2), sy3 = box_stride(3 + p2);
         int coff[2];
         bool cok[2];
 #pragma unroll
         for (int j = 0; j < 2; ++j) {
             const int c = (wn * 2 + j) * 16 + (lane & 15);
             cok[j] = c < kBoxRows;
-            coff[j] = (c % 5) * sx2 + ((c / 5) % 5) * sx3 + (c / 25) * sy3;
+            coff[j] = cofs[g][c];
         }
 #pragma unroll
         for (int i = 0; i < 4; ++i)
@@ -255,7 +263,7 @@ __global__ __launch_bounds__(kThreads) void triples_box_kernel(TriplesArgs a) {
             for (int r = 0; r < 4; ++r) {
                 const int row = (wm * 4 + i) * 16 + 4 * r + (lane >> 4);
                 if (row >= kBoxRows) continue;
-                const int roff = (row % 5) * sx1 + ((row / 5) % 5) * sy1 + (row / 25) * sy2;
+                const int roff = rofs[g][row];
 #pragma unroll
                 for (int j = 0; j < 2; ++j) {
                     if (!cok[j]) continue;
