@@ -266,7 +266,7 @@ sipgpu::TriplesArgs box_args(const sipgpu_triples_s &P) {
 int sipgpu_triples_run(sipgpu_triples_t P, long long box0, long long nbox, sipgpu_stream_t s) {
     if (!P) return fail(SIPGPU_EARG, "null triples plan");
     if (!P->packed) return fail(SIPGPU_EARG, "triples: pack the operands first");
-    if (box0 < 0 || nbox < 0 || box0 + nbox > P->slots)
+    if (box0 < 0 || nbox < 0 || box0 > P->slots || nbox > P->slots - box0)
         return fail(SIPGPU_EARG, "triples: box slots outside the plan");
     if (nbox == 0) return 0;
     sipgpu::TriplesArgs a = box_args(*P);
