@@ -233,6 +233,28 @@ __global__ __launch_bounds__(kOpThreads) void slice_kernel(SliceArgs a, double *
     }
 }
 
+// Workgroup (x, y): one element per thread, elements [x 256, (x + 1) 256)
+// of entry y; the parts are loaded eight at a time (in flight together) and
+// added in order.
+constexpr int kAddPartsChunk = kOpThreads;
+
+__global__ __launch_bounds__(kOpThreads) void add_parts_kernel(const AddPartsEntry *entries) {
+    const AddPartsEntry e = entries[blockIdx.y];
+    const long long i = (long long)blockIdx.x * kAddPartsChunk + threadIdx.x;
+    if (i >= e.n) return;
+    double v = e.t[i];
+    for (int q0 = 0; q0 < e.nparts; q0 += 8) {
+        double x[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q)
+            x[q] = q0 + q < e.nparts ? e.p[(long long)(q0 + q) * e.n + i] : 0.0;
+#pragma unroll
+        for (int q = 0; q < 8; ++q)
+            if (q0 + q < e.nparts) v = v + x[q];
+    }
+    e.t[i] = v;
+}
+
 } // namespace
 
 hipError_t launch_block_op(int op, double *a, const double *b, long long n, double s,
@@ -256,6 +278,17 @@ hipError_t launch_block_add_many(double *t, const AddManyArgs &a, long long n, h
                            n);
     else
         hipLaunchKernelGGL(add_many_kernel<1>, dim3(op_blocks(n)), dim3(kOpThreads), 0, st, t, a, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_add_parts(const AddPartsEntry *entries, int count, long long max_n,
+                            hipStream_t st) {
+    if (count <= 0 || max_n <= 0) return hipSuccess;
+    if (count > 65535) return hipErrorInvalidValue;
+    const long long gx = (max_n + kAddPartsChunk - 1) / kAddPartsChunk;
+    if (gx * kOpThreads >= (1LL << 32)) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(add_parts_kernel, dim3((unsigned)gx, (unsigned)count), dim3(kOpThreads), 0,
+                       st, entries);
     return hipGetLastError();
 }
 

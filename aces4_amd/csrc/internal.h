@@ -46,9 +46,24 @@ hipStream_t stream_of(sipgpu_stream_t s);
 // aligned, with device pointers resolved against wherever it is uploaded.
 struct BatchImage {
     std::vector<unsigned char> host;
-    size_t o_tiles = 0, o_tasks = 0, o_pairs = 0, o_tab = 0;
+    size_t o_tiles = 0, o_tasks = 0, o_pairs = 0, o_tab = 0, o_parts = 0;
     int cfg = 0, ntiles = 0, ntasks = 0, npairs = 0, real_tiles = 0;
+    // Result blocks cut into pair groups (a small batch): the GEMM tasks of
+    // groups after the first write workspace copies of their block, at
+    // ws_off[i] doubles for task ws_task[i]; nparts entries at o_parts add
+    // them into the blocks after the GEMM (part offsets in parts_off until
+    // bind_workspace).
+    std::vector<int> ws_task;
+    std::vector<long long> ws_off, parts_off;
+    int nparts = 0;
+    long long parts_max_n = 0, ws_doubles = 0;
     double flops = 0.0;
+    void bind_workspace(double *ws) {
+        GemmTask *t = reinterpret_cast<GemmTask *>(host.data() + o_tasks);
+        for (size_t i = 0; i < ws_task.size(); ++i) t[ws_task[i]].D = ws + ws_off[i];
+        AddPartsEntry *e = reinterpret_cast<AddPartsEntry *>(host.data() + o_parts);
+        for (int i = 0; i < nparts; ++i) e[i].p = ws + parts_off[i];
+    }
     GemmArgs args_at(const void *dev) const {
         const unsigned char *base = static_cast<const unsigned char *>(dev);
         GemmArgs a{};
@@ -62,8 +77,12 @@ struct BatchImage {
 };
 
 // Plans a list of block contractions (same rules as sipgpu_batch_plan) for
-// the table kernel; cfg >= 0 pins its tile configuration.
-int build_batch_image(const sipgpu_contraction *list, int n, BatchImage &img, int cfg = -1);
+// the table kernel; cfg >= 0 pins its tile configuration.  split_cus > 0:
+// when the tiles fill at most half of that many CUs, each result block's
+// pairs are cut into groups that run side by side (workspace copies summed
+// into the block afterwards, in group order).
+int build_batch_image(const sipgpu_contraction *list, int n, BatchImage &img, int cfg = -1,
+                      int split_cus = 0);
 
 // Choices of the panel planner; -1 takes the default (environment, then the
 // built-in rule).

@@ -156,6 +156,18 @@ struct AddManyArgs {
     int n;
 };
 hipError_t launch_block_add_many(double *t, const AddManyArgs &a, long long n, hipStream_t st);
+// Result blocks of a batch split into pair groups: t = ((t + p[0]) + p[1]) +
+// ... + p[nparts - 1] over n elements, part q at p + q n.
+struct AddPartsEntry {
+    double *t;
+    const double *p;
+    long long n;
+    int nparts, pad;
+};
+// One launch over `count` device entries (count <= 65535), each of at most
+// max_n elements.
+hipError_t launch_add_parts(const AddPartsEntry *entries, int count, long long max_n,
+                            hipStream_t st);
 // launch_block_add_many, then launch_energy_sum with x = y = t, in one pass
 // (the same result bits as the two calls).
 hipError_t launch_energy_sum_parts(const EnergyArgs &a, double *t, const AddManyArgs &ps,

@@ -382,8 +382,14 @@ struct sipgpu_batch_s {
     int ntasks = 0, npairs = 0, real_tiles = 0;
     int ngroups = 0;    // panel paths: GEMM tasks after merging result blocks
     double flops = 0.0;
-    void *dev = nullptr; // tiles | tasks | pairs | tab
+    void *dev = nullptr; // tiles | tasks | pairs | tab | parts
     GemmArgs args{};
+    // table kernel, result blocks cut into pair groups: the groups' workspace
+    // copies and the entries that add them into the blocks
+    double *ws = nullptr;
+    const AddPartsEntry *parts = nullptr;
+    int nparts = 0;
+    long long parts_max_n = 0;
     bool split = false;
     bool packed = false; // FP64 on packed operand panels
     SplitData sd;
@@ -676,7 +682,7 @@ int check_hazards(const sipgpu_contraction *list, int n, const ShapeVec &shapes,
 } // namespace
 
 int sipgpu::rt::build_batch_image(const sipgpu_contraction *list, int n, BatchImage &b,
-                                  int cfg) {
+                                  int cfg, int split_cus) {
     struct Entry {
         const sipgpu_contraction *e;
         std::string key;
@@ -775,11 +781,58 @@ int sipgpu::rt::build_batch_image(const sipgpu_contraction *list, int n, BatchIm
         sumM += task.M;
         sumN += task.N;
     }
-    const int ntasks = (int)tasks.size();
+    const int nblocks = (int)tasks.size();
     b.cfg = cfg >= 0 && cfg < kGemmConfigs
                 ? cfg
-                : pick_config(ntasks ? sumM / ntasks : 0, ntasks ? sumN / ntasks : 0);
+                : pick_config(nblocks ? sumM / nblocks : 0, nblocks ? sumN / nblocks : 0);
     const int bm = gemm_tile_m(b.cfg), bn = gemm_tile_n(b.cfg);
+
+    // Too few tiles to fill the CUs: each result block's pairs are cut into
+    // up to 16 contiguous groups, about four tiles per CU in all (the ladder
+    // of 16 blocks of 100 x 100 x 16 pairs of K 100: 0.171 ms unsplit, 0.044
+    // with two tiles per CU, 0.037 with four).  Group 0 is
+    // the block's own task; group q > 0 assigns into a workspace copy of the
+    // block, and after the GEMM the copies are added into the block in group
+    // order (a fixed summation order).
+    std::vector<AddPartsEntry> parts;
+    if (split_cus > 0) {
+        long long tiles0 = 0;
+        for (const GemmTask &t : tasks)
+            tiles0 += (long long)((t.M + bm - 1) / bm) * ((t.N + bn - 1) / bn);
+        if (tiles0 > 0 && tiles0 * 2 <= split_cus) {
+            const int G = (int)std::min<long long>(16, 4 * split_cus / tiles0);
+            auto group_cost = [&](const GemmTask &t, int p0, int p1) {
+                double c = 0.0;
+                for (int p = p0; p < p1; ++p) c += 2.0 * (double)t.M * t.N * pairs[p].K;
+                return c;
+            };
+            for (int t = 0; t < nblocks; ++t) {
+                const int np = tasks[t].npairs, first = tasks[t].first;
+                const int g = std::min(G, np);
+                if (g < 2) continue;
+                const long long dsize = (long long)tasks[t].M * tasks[t].N;
+                AddPartsEntry pe{tasks[t].D, nullptr, dsize, g - 1, 0};
+                b.parts_off.push_back(b.ws_doubles);
+                for (int q = 1; q < g; ++q) {
+                    GemmTask sub = tasks[t];
+                    sub.first = first + (int)((long long)q * np / g);
+                    sub.npairs = first + (int)((long long)(q + 1) * np / g) - sub.first;
+                    sub.beta = 0;
+                    sub.D = nullptr; // bound to the workspace
+                    b.ws_task.push_back((int)tasks.size());
+                    b.ws_off.push_back(b.ws_doubles);
+                    b.ws_doubles += dsize;
+                    task_cost.push_back(group_cost(sub, sub.first, sub.first + sub.npairs));
+                    tasks.push_back(sub);
+                }
+                tasks[t].npairs = (int)((long long)np / g);
+                task_cost[t] = group_cost(tasks[t], first, first + tasks[t].npairs);
+                parts.push_back(pe);
+                b.parts_max_n = std::max(b.parts_max_n, dsize);
+            }
+        }
+    }
+    const int ntasks = (int)tasks.size();
 
     std::vector<int> tms(ntasks), tns(ntasks);
     for (int t = 0; t < ntasks; ++t) {
@@ -790,25 +843,30 @@ int sipgpu::rt::build_batch_image(const sipgpu_contraction *list, int n, BatchIm
     const std::vector<GemmTile> tiles = xcd_schedule<GemmTile>(tms, tns, task_cost, real_tiles);
     b.real_tiles = real_tiles;
 
-    // Upload: tiles | tasks | pairs | tables, each 256-byte aligned.
+    // Upload: tiles | tasks | pairs | tables | parts, each 256-byte aligned.
     auto align = [](size_t v) { return (v + 255) & ~(size_t)255; };
     const size_t o_tiles = 0;
     const size_t o_tasks = align(o_tiles + tiles.size() * sizeof(GemmTile));
     const size_t o_pairs = align(o_tasks + tasks.size() * sizeof(GemmTask));
     const size_t o_tab = align(o_pairs + pairs.size() * sizeof(GemmPair));
-    const size_t total = align(o_tab + pool.size() * sizeof(int32_t)) + 256;
+    const size_t o_parts = align(o_tab + pool.size() * sizeof(int32_t));
+    const size_t total = align(o_parts + parts.size() * sizeof(AddPartsEntry)) + 256;
     std::vector<unsigned char> host(total, 0);
     std::memcpy(host.data() + o_tiles, tiles.data(), tiles.size() * sizeof(GemmTile));
     std::memcpy(host.data() + o_tasks, tasks.data(), tasks.size() * sizeof(GemmTask));
     std::memcpy(host.data() + o_pairs, pairs.data(), pairs.size() * sizeof(GemmPair));
     std::memcpy(host.data() + o_tab, pool.data(), pool.size() * sizeof(int32_t));
+    if (!parts.empty())
+        std::memcpy(host.data() + o_parts, parts.data(), parts.size() * sizeof(AddPartsEntry));
     b.host = std::move(host);
     b.o_tiles = o_tiles;
     b.o_tasks = o_tasks;
     b.o_pairs = o_pairs;
     b.o_tab = o_tab;
+    b.o_parts = o_parts;
+    b.nparts = (int)parts.size();
     b.ntiles = (int)tiles.size();
-    b.ntasks = ntasks;
+    b.ntasks = nblocks;
     b.npairs = (int)pairs.size();
     (void)maxM;
     (void)maxN;
@@ -818,12 +876,22 @@ int sipgpu::rt::build_batch_image(const sipgpu_contraction *list, int n, BatchIm
 namespace {
 
 int plan_batch_locked(Context &, const sipgpu_contraction *list, int n, sipgpu_batch_s &b,
-                      int cfg = -1) {
+                      int cfg = -1, bool groups = false) {
     BatchImage img;
-    if (int rc = sipgpu::rt::build_batch_image(list, n, img, cfg)) return rc;
+    if (int rc = sipgpu::rt::build_batch_image(list, n, img, cfg, groups ? (int)g_device_cus : 0))
+        return rc;
+    if (img.ws_doubles) {
+        HIP_TRY(hipMalloc(&b.ws, (size_t)img.ws_doubles * sizeof(double)));
+        img.bind_workspace(b.ws);
+    }
     HIP_TRY(hipMalloc(&b.dev, img.host.size()));
     HIP_TRY(hipMemcpy(b.dev, img.host.data(), img.host.size(), hipMemcpyHostToDevice));
     b.args = img.args_at(b.dev);
+    b.parts = img.nparts ? reinterpret_cast<const AddPartsEntry *>(
+                               static_cast<unsigned char *>(b.dev) + img.o_parts)
+                         : nullptr;
+    b.nparts = img.nparts;
+    b.parts_max_n = img.parts_max_n;
     b.cfg = img.cfg;
     b.flops = img.flops;
     b.ntiles = img.ntiles;
@@ -1484,6 +1552,10 @@ int plan_panels_locked(const sipgpu_contraction *list, int n, int nmod, sipgpu_b
 
 void free_batch_memory(sipgpu_batch_s &b) {
     if (b.dev) (void)hipFree(b.dev);
+    if (b.ws) (void)hipFree(b.ws);
+    b.ws = nullptr;
+    b.parts = nullptr;
+    b.nparts = 0;
     if (b.sd.planes) (void)hipFree(b.sd.planes);
     if (b.sd.res_planes) (void)hipFree(b.sd.res_planes);
     if (b.sd.work) (void)hipFree(b.sd.work);
@@ -1497,7 +1569,10 @@ int execute_batch(Context &c, sipgpu_batch_s &b, int stages, sipgpu_stream_t s) 
     if (b.ntiles == 0) return 0;
     const hipStream_t hs = resolve(c, s).s;
     if (b.split || b.packed) return sipgpu::rt::execute_panels(b.sd, b.split, stages, hs);
-    if (stages & SIPGPU_STAGE_PRODUCT) HIP_TRY(launch_gemm(b.cfg, b.args, b.ntiles, hs));
+    if (stages & SIPGPU_STAGE_PRODUCT) {
+        HIP_TRY(launch_gemm(b.cfg, b.args, b.ntiles, hs));
+        if (b.nparts) HIP_TRY(launch_add_parts(b.parts, b.nparts, b.parts_max_n, hs));
+    }
     return 0;
 }
 
@@ -1574,7 +1649,7 @@ int plan_choice_locked(Context &c, const sipgpu_contraction *list, int n, int nm
     opt.tile = ch.tile;
     int rc;
     if (ch.path == 0)
-        rc = plan_batch_locked(c, list, n, *b, ch.cfg);
+        rc = plan_batch_locked(c, list, n, *b, ch.cfg, ch.tail == 1);
     else
         rc = plan_panels_locked(list, n, ch.path == 2 ? nmod : 0, *b, opt);
     if (rc) {
@@ -2328,6 +2403,15 @@ int sipgpu_batch_tune(const sipgpu_contraction *list, int n, int nmoduli, sipgpu
             PlanChoice ch;
             ch.path = 0;
             ch.cfg = cfg;
+            cands.push_back(ch);
+        }
+        // the table kernel with each result block's pairs cut into groups
+        // (a batch whose tiles fill at most half of the CUs)
+        for (int cfg : {0, 11}) {
+            PlanChoice ch;
+            ch.path = 0;
+            ch.cfg = cfg;
+            ch.tail = 1;
             cands.push_back(ch);
         }
         for (int merge : {1, 0})

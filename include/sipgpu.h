@@ -300,7 +300,11 @@ enum { SIPGPU_STAGE_PREPARE = 1, SIPGPU_STAGE_PRODUCT = 2, SIPGPU_STAGE_FINISH =
        SIPGPU_STAGE_ALL = 7 };
 int sipgpu_batch_execute_stages(sipgpu_batch_t batch, int stages, sipgpu_stream_t s);
 /* Autotuner (no reference counterpart).  Plans the batch every candidate
- * way -- the table kernel in each of its tile configurations, packed FP64
+ * way -- the table kernel in each of its tile configurations (and, for
+ * configurations 0 and 11, with each result block's pairs cut into groups
+ * that run side by side when the tiles fill at most half of the CUs: tail 1
+ * on path 0; the groups' workspace copies are added into the block after the
+ * GEMM in group order), packed FP64
  * panels with and without merged result blocks, without the persistent
  * water-filled schedule (tail 0) and with it over 64, 32, 16 or 8
  * workgroups per XCD (tail 1, 2, 3, 4: fewer workgroups cut a small batch's
@@ -327,9 +331,10 @@ typedef struct {
 int sipgpu_batch_tune(const sipgpu_contraction *list, int n, int nmoduli, sipgpu_batch_t *batch,
                       sipgpu_tune_result *result);
 /* Plans the batch one given way (the fields of sipgpu_tune_result: path,
- * config for the table kernel, merge / tail / tile for the panel paths, -1
- * for a default; tile 64 or -1 for packed FP64, 128 / 256 / 257 or -1 for
- * the split path): pins a plan, e.g. one found by tuning elsewhere. */
+ * config and tail (1: pair groups) for the table kernel, merge / tail / tile
+ * for the panel paths, -1 for a default; tile 64 or -1 for packed FP64,
+ * 128 / 256 / 257 or -1 for the split path): pins a plan, e.g. one found by
+ * tuning elsewhere. */
 int sipgpu_batch_plan_choice(const sipgpu_contraction *list, int n, int nmoduli, int path,
                              int config, int merge, int tail, int tile, sipgpu_batch_t *batch);
 /* The tuning table as text (one "signature path config merge tail tile" line
