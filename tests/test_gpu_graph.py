@@ -1,7 +1,7 @@
 """GPU tests of CC iterations captured as graphs.
 
-One iteration: a packed FP64 batch, a table-kernel batch, a split-precision
-batch, a single contraction, a permute, a seeded fill and an energy sum, all
+One iteration: a packed FP64 batch, a table-kernel batch (result blocks cut
+into pair groups), a split-precision batch, a single contraction, a permute, a seeded fill and an energy sum, all
 on one stream.  It is run eagerly once, then captured and replayed three
 times; before every replay the blocks are reset from host memory and other
 eager work runs on the same stream, and every replay must reproduce the
@@ -28,11 +28,13 @@ def test_graph_replays_match_eager(monkeypatch):
         w["D"].host[:] = rng.standard_normal(w["D"].host.size)
     monkeypatch.setenv("SIPGPU_FP64_PATH", "packed")
     b_packed = wl.batch_for(w1)
-    monkeypatch.setenv("SIPGPU_FP64_PATH", "tables")
-    b_tab = wl.batch_for(w2)
     monkeypatch.delenv("SIPGPU_FP64_PATH")
+    # the table kernel with the ladder's result blocks cut into pair groups
+    # (a GEMM launch, then the groups' sums)
+    b_tab = sg.plan_choice(wl.contractions_for(w2), "tables", tail=1)
     b_split = wl.batch_for(w3, moduli=13)
     assert (b_packed.kind, b_tab.kind, b_split.kind) == ("packed", "tables", "split")
+    assert b_tab.ntiles > 64
     # a single contraction, a permute, a fill and an energy sum on their own buffers
     n = 24
     A = sg.DeviceArray.from_host(rng.standard_normal(n * n))
